@@ -1,0 +1,145 @@
+#!/usr/bin/env python3
+"""Throughput of the velocity Verlet step (md_advance_vv) on one MI355X, in atom-steps/s.
+
+Workload (default): 1,048,576-molecule LJ fluid, FCC 64^3 at density 0.8, rc 2.5,
+delta_rneighbr 0.3, dt 0.005, seeded T=1.0 velocities, NVE, displacement-triggered rebuilds
+(REBUILD_CRITERIA 2) plus the forced rebuild every TPLOT=10 steps, FP64.
+
+A step is one full md_advance_vv: VV pass 1, rebuild check, (rebuild: sendmols, cell binning,
+neighbour list), forces, VV pass 2.  Inputs are resident in HBM before the timed region; the
+timed region is exactly --steps steps bracketed by device synchronisation.
+
+Prints ONE JSON line.  The force kernel's average duration is measured with HIP events on the
+context's stream over the timed region (roofline); cpu_baseline times the CPU restatement of the
+reference on a bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X vector FP64
+FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X vector FP32
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nunits", type=int, default=64, help="FCC units per side (64 -> 1,048,576 molecules)")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU sample (after its setup)")
+    ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP event timing")
+    return ap.parse_args()
+
+
+def cpu_baseline(p, r, v, steps):
+    """The CPU restatement of the reference (oracle, one core) on the same workload."""
+    from oracle.oracle import Oracle
+
+    o = Oracle(p, r, v)
+    o.setup()
+    o.compute_forces(0)
+    t0 = time.perf_counter()
+    o.run_vv(1, steps)
+    dt = time.perf_counter() - t0
+    n = r.shape[0]
+    return {"value": n * steps / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"same {n}-molecule workload, {steps} md_advance_vv steps after setup, "
+                      f"{dt:.1f} s, single-threaded C restatement (gcc -O3) on this host"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (the path has no multi-GPU form here); got --gpus {a.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    from flowmol_amd import config
+    from flowmol_amd._abi import FM_PREC_FP64, FM_PREC_MIXED
+    from flowmol_amd.md import MDContext
+
+    prec = FM_PREC_MIXED if a.precision == "mixed" else FM_PREC_FP64
+    n_u = a.nunits
+    p, r, v = config.config_fcc((n_u, n_u, n_u), 0.8, 2.5, seed=2, tplot=10, precision=prec)
+    N = r.shape[0]
+    g = MDContext(p, r, v)
+    g.setup(0)
+    g.run_vv(1, a.warmup)
+    g.synchronize()
+    if not a.no_timing:
+        g.kernel_timing(True)
+    it0 = 1 + a.warmup
+    t0 = time.perf_counter()
+    _, nrb = g.run_vv(it0, a.steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    dt = t1 - t0
+    kt = g.get_kernel_timing() if not a.no_timing else {}
+    value = N * a.steps / dt
+
+    # neighbour statistics of the current list (full list: two entries per pair)
+    import ctypes as C
+    from flowmol_amd._lib import load
+    cnt = C.c_int64()
+    load().fm_download_neighbourlist(g._h, None, None, None, 0, C.byref(cnt))
+    p_half = cnt.value / 2.0 / N
+    geo = g.geometry()
+    rn = math.sqrt(geo["rneighbr2"])
+    p_rc = p_half * (2.5 / rn) ** 3     # uniform-density share of list pairs inside rc
+
+    roof = None
+    if "forces" in kt and kt["forces"][1] > 0:
+        avg_s = kt["forces"][0] * 1e-3
+        bytes_per = N * (24 + 24 + 4 * p_half)         # r read, a written, list words read
+        flops_per = N * (8 * p_half + 16 * p_rc)        # per half-list pair: 8 (+16 inside rc)
+        gbs = bytes_per / avg_s / 1e9
+        tfl = flops_per / avg_s / 1e12
+        peak_tf = FP64_VALU_PEAK_TFLOPS if prec == FM_PREC_FP64 else FP32_VALU_PEAK_TFLOPS
+        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "forces", "avg_ms": round(kt["forces"][0], 5), "launches": kt["forces"][1],
+                "bytes_per_launch": round(bytes_per), "flops_per_launch": round(flops_per),
+                "valu": {"achieved": round(tfl, 2), "peak": peak_tf, "unit": "TFLOP/s",
+                         "frac": round(tfl / peak_tf, 4),
+                         "dtype": "f64" if prec == FM_PREC_FP64 else "f32"}}
+    out = {
+        "metric": "atom-steps/s (1M LJ, 1 GPU)",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == FM_PREC_FP64 else "f32-pair/f64-state",
+        "data": "synthetic",
+        "config": {"workload": f"LJ NVE FCC {n_u}^3 ({N} molecules), density 0.8, rc 2.5, delta 0.3, "
+                               f"dt 0.005, T 1.0, TPLOT 10, displacement rebuild",
+                   "molecules": N, "rebuilds_in_timed_steps": nrb, "p_half": round(p_half, 2),
+                   "precision": a.precision},
+        "kernels_ms": {k: round(vv[0], 5) for k, vv in kt.items() if vv[1] > 0},
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+    g.close()
+    if a.cpu_baseline == "on":
+        p2, r2, v2 = config.config_fcc((n_u, n_u, n_u), 0.8, 2.5, seed=2, tplot=10)
+        out["cpu_baseline"] = cpu_baseline(p2, r2, v2, a.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,168 @@
+"""Parameters and synthetic initial states for the force / neighbour / integration path.
+
+The derived quantities follow the reference's setup (paths relative to the reference's root):
+  globaldomain, initialunitsize   src/setup_set_parameters.f90:1543-1569
+  FCC lattice positions           src/setup_initialise_microstate.f90:244-304
+  Mie constant C and potshift     src/setup_set_parameters.f90:934-946
+  MOP plane spacing / count       src/setup_set_parameters.f90:2298-2340
+Velocities are this package's own seeded generator (Gaussian, zero net momentum, rescaled to the
+target temperature); the reference's gfortran random_number stream is not reproduced.
+"""
+import math
+
+import numpy as np
+
+from ._abi import (FM_MAX_TYPES, FM_PREC_FP64, FM_REBUILD_DISPLACEMENT, FM_ENSEMBLE_NVE,
+                   FmParams)
+
+WCA_CUT = 2.0 ** (1.0 / 6.0)
+
+
+def fcc_globaldomain(nunits, density):
+    """globaldomain(ixyz) = initialnunits(ixyz)/((density/4)**(1/nd))."""
+    f = (density / 4.0) ** (1.0 / 3)
+    return np.array([n / f for n in nunits], dtype=np.float64)
+
+
+def fcc_lattice(nunits, density):
+    """Molecule positions of setup_initialise_lattice on one rank, (np, 3) float64.
+
+    Loop order nz, nx, ny; 4-site basis; c = (n - 0.75)*a; r = rc - halfdomain.
+    """
+    gd = fcc_globaldomain(nunits, density)
+    a = gd / np.array(nunits, dtype=np.float64)
+    half = 0.5 * gd
+    out = np.empty((4 * nunits[0] * nunits[1] * nunits[2], 3), dtype=np.float64)
+    n = 0
+    # vectorised over ny and the basis; loop order is preserved by construction
+    ny = np.arange(1, nunits[1] + 1, dtype=np.float64)
+    c2 = (ny - 0.75) * a[1]
+    for nz in range(1, nunits[2] + 1):
+        c3 = (nz - 0.75) * a[2]
+        for nx in range(1, nunits[0] + 1):
+            c1 = (nx - 0.75) * a[0]
+            block = np.empty((nunits[1], 4, 3))
+            block[:, :, 0] = c1
+            block[:, :, 1] = c2[:, None]
+            block[:, :, 2] = c3
+            block[:, 1, 0] = c1 + 0.5 * a[0]
+            block[:, 1, 2] = c3 + 0.5 * a[2]
+            block[:, 2, 1] = c2 + 0.5 * a[1]
+            block[:, 2, 2] = c3 + 0.5 * a[2]
+            block[:, 3, 0] = c1 + 0.5 * a[0]
+            block[:, 3, 1] = c2 + 0.5 * a[1]
+            block -= half
+            m = 4 * nunits[1]
+            out[n:n + m] = block.reshape(m, 3)
+            n += m
+    return out
+
+
+def seeded_velocities(np_, temperature, seed, mass=None):
+    """Gaussian velocities from a seeded PCG64 stream, zero net momentum, scaled so that
+    sum(m v^2)/(3N) == temperature."""
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((np_, 3))
+    m = np.ones(np_) if mass is None else np.asarray(mass, dtype=np.float64)
+    p = (m[:, None] * v).sum(axis=0) / m.sum()
+    v -= p
+    mv2 = float((m[:, None] * v * v).sum())
+    v *= math.sqrt(temperature * 3 * np_ / mv2)
+    return np.ascontiguousarray(v)
+
+
+def mie_C(lr, la):
+    """C_lookup = (lr/(lr-la)) * (lr/la)**(la/(lr-la))."""
+    return (lr / (lr - la)) * (lr / la) ** (la / (lr - la))
+
+
+def mie_energy(invrij2, C, eps, lr, la, alpha=1.0, potshift=0.0):
+    return C * eps * (invrij2 ** (0.5 * lr) - alpha * invrij2 ** (0.5 * la)) - potshift
+
+
+def make_params(domain, density, rcutoff, delta=0.3, dt=0.005, tplot=10, teval=None,
+                rebuild_criteria=FM_REBUILD_DISPLACEMENT, fixed_rebuild=20, precision=FM_PREC_FP64,
+                potential_flag=0, k_c=30.0, R_0=1.5, ensemble=FM_ENSEMBLE_NVE,
+                vflux_outflag=0, nplanes=None, planespacing=None, pressure_outflag=0,
+                max_neighbours=0):
+    p = FmParams()
+    for d in range(3):
+        p.domain[d] = float(domain[d])
+        p.delta_rneighbr[d] = float(delta)
+    p.density = float(density)
+    p.rcutoff = float(rcutoff)
+    p.delta_t = float(dt)
+    p.tplot = int(tplot)
+    p.teval = int(tplot if teval is None else teval)
+    p.rebuild_criteria = int(rebuild_criteria)
+    p.fixed_rebuild = int(fixed_rebuild)
+    p.potential_flag = int(potential_flag)
+    p.mie_potential = 0
+    p.ntypes = 1
+    p.precision = int(precision)
+    for t in range(FM_MAX_TYPES):
+        p.mass_lookup[t] = 1.0
+    p.k_c = float(k_c)
+    p.R_0 = float(R_0)
+    p.ensemble = int(ensemble)
+    p.vflux_outflag = int(vflux_outflag)
+    p.pressure_outflag = int(pressure_outflag)
+    if 1 <= vflux_outflag <= 3:
+        pn = vflux_outflag - 1
+        if planespacing is None:
+            # reference: planespacing = cellsidelength(2), nplanes = floor(domain/planespacing)
+            ncy = math.floor(domain[1] / (rcutoff + delta))
+            planespacing = domain[1] / ncy
+        if nplanes is None:
+            nplanes = math.floor(domain[pn] / planespacing)
+        p.planespacing = float(planespacing)
+        p.nplanes = int(nplanes)
+    p.max_neighbours = int(max_neighbours)
+    return p
+
+
+def set_mie_types(p, types, rcutoff=None):
+    """Fill the Mie lookup tables from {(ti, tj): dict(eps, lr, la, alpha, rc)} and masses.
+
+    potshift_lookup(i,j) = get_energy(1/rc**2, i, j) with the pair's own cutoff.
+    """
+    p.mie_potential = 1
+    ntypes = 1 + max(max(k) for k in types)
+    p.ntypes = ntypes
+    for (ti, tj), d in types.items():
+        for (a, b) in ((ti, tj), (tj, ti)):
+            k = a * FM_MAX_TYPES + b
+            lr, la = float(d["lr"]), float(d["la"])
+            eps = float(d.get("eps", 1.0))
+            alpha = float(d.get("alpha", 1.0))
+            C = mie_C(lr, la)
+            rc = float(d.get("rc", rcutoff if rcutoff is not None else p.rcutoff))
+            p.epsilon_lookup[k] = eps
+            p.lambdar_lookup[k] = lr
+            p.lambdaa_lookup[k] = la
+            p.C_lookup[k] = C
+            p.alpha_lookup[k] = alpha
+            p.potshift_lookup[k] = mie_energy(1.0 / rc ** 2, C, eps, lr, la, alpha)
+            p.rcutoff2_lookup[k] = rc * rc
+    return p
+
+
+def config_fcc(nunits, density, rcutoff, seed, temperature=1.0, **kw):
+    """FCC lattice + seeded velocities + parameters for one periodic domain."""
+    domain = fcc_globaldomain(nunits, density)
+    r = fcc_lattice(nunits, density)
+    v = seeded_velocities(r.shape[0], temperature, seed)
+    p = make_params(domain, density, rcutoff, **kw)
+    return p, r, v
+
+
+def config_c1(**kw):
+    """2048-molecule WCA NVE case (FCC 8^3, density 0.8, rc 2^(1/6), delta 0.3, dt 0.005)."""
+    kw.setdefault("tplot", 100)
+    return config_fcc((8, 8, 8), 0.8, WCA_CUT, seed=1, **kw)
+
+
+def config_c2(nunits=64, **kw):
+    """1,048,576-molecule LJ NVE case (FCC 64^3, density 0.8, rc 2.5, delta 0.3, dt 0.005)."""
+    kw.setdefault("tplot", 10)
+    return config_fcc((nunits, nunits, nunits), 0.8, 2.5, seed=2, **kw)

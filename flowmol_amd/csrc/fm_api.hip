@@ -1,0 +1,808 @@
+// fm_api.hip -- C ABI of libflowmol_hip: context lifecycle, host<->device state and the
+// per-routine entry points that replace the reference's force / neighbour / integration bodies.
+#include "fm_internal.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+static thread_local char g_err[1024];
+
+int fm_set_error(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+extern "C" const char* fm_last_error(void) { return g_err; }
+extern "C" int fm_abi_version(void) { return FM_ABI_VERSION; }
+
+// ---- kernel timing ----------------------------------------------------------------------------
+const char* const kt_names[KT_NCLASS] = {"vv_pass1", "checkrebuild", "sendmols", "assign_to_cell",
+                                         "neighbourlist", "forces", "vv_pass2", "macros"};
+
+struct KTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<int, size_t>> rec;  // (class, index of start event)
+    int open_cls = -1;
+    size_t open_idx = 0;
+    double total_ms[KT_NCLASS] = {0};
+    long long count[KT_NCLASS] = {0};
+};
+
+static hipEvent_t kt_event(KTimer* t) {
+    if (t->used == t->pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        t->pool.push_back(e);
+    }
+    return t->pool[t->used++];
+}
+
+static void kt_collect(fm_ctx* c) {
+    KTimer* t = c->timer;
+    if (!t || t->rec.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (auto& r : t->rec) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t->pool[r.second], t->pool[r.second + 1]) == hipSuccess) {
+            t->total_ms[r.first] += ms;
+            t->count[r.first] += 1;
+        }
+    }
+    t->rec.clear();
+    t->used = 0;
+}
+
+void ktimer_begin(fm_ctx* c, int cls) {
+    KTimer* t = c->timer;
+    if (!t || !t->on) return;
+    if (t->used + 2 > 4096) kt_collect(c);
+    hipEvent_t a = kt_event(t);
+    hipEvent_t b = kt_event(t);
+    if (!a || !b) return;
+    t->open_cls = cls;
+    t->open_idx = t->used - 2;
+    hipEventRecord(a, c->stream);
+}
+
+void ktimer_end(fm_ctx* c, int cls) {
+    KTimer* t = c->timer;
+    if (!t || !t->on || t->open_cls != cls) return;
+    hipEventRecord(t->pool[t->open_idx + 1], c->stream);
+    t->rec.push_back({cls, t->open_idx});
+    t->open_cls = -1;
+}
+
+// ---- helpers ----------------------------------------------------------------------------------
+#define CHECK_CTX(c) \
+    if (!(c)) return fm_set_error(FM_EINVAL, "null context")
+#define TRY(x)              \
+    do {                    \
+        int _rc = (x);      \
+        if (_rc) return _rc; \
+    } while (0)
+
+template <typename T>
+static int dalloc(T** p, size_t n) {
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) return fm_set_error(FM_ENOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+    return 0;
+}
+
+// read the device flag words; report the first device-side error
+static int sync_flags(fm_ctx* c) {
+    FM_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, FLAG_COUNT * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    int code = c->h_flags[FLAG_ERROR];
+    if (code) {
+        int mol = c->h_flags[FLAG_ERROR_MOL];
+        // clear so that the context can be inspected afterwards
+        launch_fill_i32(c, c->d_flags + FLAG_ERROR, 0, 2);
+        switch (code) {
+        case FM_EESCAPE:
+            return fm_set_error(code, "Aborting due to escaped mol. (molecule %d outside the domain and its halo)", mol + 1);
+        case FM_EOVERFLOW:
+            return fm_set_error(code, "capacity exceeded (neighbour list or cell) at molecule/cell %d", mol);
+        case FM_EBOND:
+            return fm_set_error(code, "polymer bond broken or too many bonds for molno %d", mol + 1);
+        default:
+            return fm_set_error(code, "device error %d at %d", code, mol);
+        }
+    }
+    return 0;
+}
+
+static void free_ctx(fm_ctx* c) {
+    if (!c) return;
+    if (c->dev >= 0) hipSetDevice(c->dev);
+    void* ptrs[] = {c->st[0], c->st[1], c->rdisp, c->ext[0], c->ext[1], c->slot_of, c->moltype[0], c->moltype[1],
+                    c->tag[0], c->tag[1], c->teth[0], c->teth[1], c->mono[0], c->mono[1], c->cell_of,
+                    c->cell_count, c->cell_start, c->cell_cursor, c->perm, c->nbl, c->nbn, c->bond, c->bondn,
+                    c->pe_lj, c->pe_poly, c->vir, c->rf, c->pxy, c->planes, c->out_flag, c->out_vals,
+                    c->out_slot, c->out_count, c->red, c->d_macros, c->d_mie, c->d_vmax, c->d_rmax,
+                    c->d_flags, c->stage, c->cub_tmp};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (c->h_flags) hipHostFree(c->h_flags);
+    if (c->timer) {
+        for (auto e : c->timer->pool) hipEventDestroy(e);
+        delete c->timer;
+    }
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+static int alloc_nlist(fm_ctx* c, int maxnb) {
+    if (c->nbl) hipFree(c->nbl);
+    c->nbl = nullptr;
+    c->maxnb = maxnb;
+    return dalloc(&c->nbl, (size_t)maxnb * c->cap);
+}
+
+// ---- lifecycle --------------------------------------------------------------------------------
+extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx** out) {
+    if (!p || !out) return fm_set_error(FM_EINVAL, "null argument");
+    *out = nullptr;
+    if (np < 0) return fm_set_error(FM_EINVAL, "np < 0");
+    for (int d = 0; d < 3; ++d)
+        if (!(p->domain[d] > 0.0)) return fm_set_error(FM_EINVAL, "domain must be positive");
+    if (!(p->rcutoff > 0.0) || !(p->delta_t > 0.0)) return fm_set_error(FM_EINVAL, "rcutoff and delta_t must be positive");
+    if (p->tplot < 1) return fm_set_error(FM_EINVAL, "tplot must be >= 1");
+    if (p->rebuild_criteria < 0 || p->rebuild_criteria > 2) return fm_set_error(FM_EINVAL, "rebuild_criteria must be 0, 1 or 2");
+    if (p->rebuild_criteria == FM_REBUILD_FIXED && p->fixed_rebuild < 1) return fm_set_error(FM_EINVAL, "fixed_rebuild must be >= 1");
+    if (p->ensemble != FM_ENSEMBLE_NVE && p->ensemble != FM_ENSEMBLE_TAG_MOVE)
+        return fm_set_error(FM_EINVAL, "ensemble must be NVE (0) or tag_move (6)");
+    if (p->precision != FM_PREC_FP64 && p->precision != FM_PREC_MIXED) return fm_set_error(FM_EINVAL, "bad precision");
+    if (p->precision == FM_PREC_MIXED && (p->mie_potential || p->potential_flag))
+        return fm_set_error(FM_EINVAL, "mixed precision supports the LJ pair law without bonds");
+    if (p->vflux_outflag == 4) return fm_set_error(FM_EINVAL, "vflux_outflag 4 (CV stress) is not on this path");
+    if (p->vflux_outflag >= 1 && p->vflux_outflag <= 3 &&
+        (p->nplanes < 1 || p->nplanes > FM_MAX_PLANES || !(p->planespacing > 0.0)))
+        return fm_set_error(FM_EINVAL, "MOP needs 1 <= nplanes <= %d and planespacing > 0", FM_MAX_PLANES);
+    if (p->mie_potential == 1 && (p->ntypes < 1 || p->ntypes > FM_MAX_TYPES))
+        return fm_set_error(FM_EINVAL, "ntypes must be 1..%d", FM_MAX_TYPES);
+
+    fm_ctx* c = new fm_ctx();
+    c->p = *p;
+    c->dev = -1;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        delete c;
+        return fm_set_error(FM_EHIP, "no HIP device");
+    }
+    c->dev = dev;
+    c->np = np;
+    c->cap = std::max(std::max(np, capacity), 1);
+    if (c->cap >= (1 << 27)) {
+        free_ctx(c);
+        return fm_set_error(FM_EINVAL, "capacity %d exceeds 2^27 molecules per context", c->cap);
+    }
+
+    // set_parameters_cells (src/setup_set_parameters.f90:1848-1903)
+    for (int d = 0; d < 3; ++d) c->half[d] = 0.5 * p->domain[d];
+    double rneighbr = p->rcutoff + (p->delta_rneighbr[0] + p->delta_rneighbr[1] + p->delta_rneighbr[2]) / 3.0;
+    c->rn2 = rneighbr * rneighbr;
+    if (p->potential_flag == 1 && rneighbr < p->R_0) {
+        rneighbr = p->R_0;
+        c->rn2 = p->R_0 * p->R_0;
+    }
+    for (int d = 0; d < 3; ++d) c->nc[d] = (int)floor(p->domain[d] / (p->rcutoff + p->delta_rneighbr[d]));
+    if (c->nc[0] < 3 || c->nc[1] < 3 || c->nc[2] < 3) {
+        free_ctx(c);
+        return fm_set_error(FM_EINVAL, "ERROR - DOMAIN SHOULD HAVE AT LEAST 3 CELLS, IN X, Y AND Z - INCREASE NUMBER OF UNITS IN INPUT");
+    }
+    for (int d = 0; d < 3; ++d) c->cs[d] = p->domain[d] / c->nc[d];
+    c->ncell = c->nc[0] * c->nc[1] * c->nc[2];
+    c->rc2 = p->rcutoff * p->rcutoff;
+    {
+        double rc = p->rcutoff, x3 = rc * (rc * rc), x6 = x3 * x3, x12 = x6 * x6;
+        c->potshift = 4.0 * (1.0 / x12 - 1.0 / x6);
+    }
+    c->dmin = std::min(p->delta_rneighbr[0], std::min(p->delta_rneighbr[1], p->delta_rneighbr[2]));
+
+    // neighbour capacity: 1.5x the expected count at the mean density, plus margin
+    int maxnb = p->max_neighbours;
+    if (maxnb <= 0) {
+        double vol = p->domain[0] * p->domain[1] * p->domain[2];
+        double dens = std::max(p->density, np / vol);
+        double expect = 4.0 / 3.0 * M_PI * rneighbr * rneighbr * rneighbr * dens;
+        maxnb = (int)(1.5 * expect) + 32;
+        maxnb = (maxnb + 3) & ~3;
+    }
+
+    int rc = 0;
+    hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        free_ctx(c);
+        return fm_set_error(FM_EHIP, "hipStreamCreate: %s", hipGetErrorString(he));
+    }
+    const size_t cap = c->cap;
+    const int T2 = FM_MAX_TYPES * FM_MAX_TYPES;
+    c->nred_blocks = (int)((cap + FM_BLOCK - 1) / FM_BLOCK) + 1;
+    if ((rc = dalloc(&c->st[0], 9 * cap)) || (rc = dalloc(&c->st[1], 9 * cap)) || (rc = dalloc(&c->rdisp, 3 * cap)) ||
+        (rc = dalloc(&c->ext[0], cap)) || (rc = dalloc(&c->ext[1], cap)) || (rc = dalloc(&c->slot_of, cap)) ||
+        (rc = dalloc(&c->cell_of, cap)) || (rc = dalloc(&c->cell_count, (size_t)c->ncell + 1)) ||
+        (rc = dalloc(&c->cell_start, (size_t)c->ncell + 1)) || (rc = dalloc(&c->cell_cursor, (size_t)c->ncell)) ||
+        (rc = dalloc(&c->perm, cap)) || (rc = dalloc(&c->nbn, cap)) || (rc = dalloc(&c->pe_lj, cap)) ||
+        (rc = dalloc(&c->pe_poly, cap)) || (rc = dalloc(&c->vir, cap)) || (rc = dalloc(&c->out_flag, cap)) ||
+        (rc = dalloc(&c->out_vals, cap)) || (rc = dalloc(&c->out_slot, cap)) || (rc = dalloc(&c->out_count, 4)) ||
+        (rc = dalloc(&c->red, (size_t)c->nred_blocks * 8)) || (rc = dalloc(&c->d_macros, 16)) ||
+        (rc = dalloc(&c->d_mie, 7 * (size_t)T2 + FM_MAX_TYPES)) || (rc = dalloc(&c->d_vmax, 1)) ||
+        (rc = dalloc(&c->d_rmax, 1)) || (rc = dalloc(&c->d_flags, FLAG_COUNT)) || (rc = dalloc(&c->stage, 9 * cap)) ||
+        (rc = alloc_nlist(c, maxnb))) {
+        free_ctx(c);
+        return rc;
+    }
+    if (p->potential_flag == 1) {
+        if ((rc = dalloc(&c->bond, FM_MAX_FUNCY * cap)) || (rc = dalloc(&c->bondn, cap)) ||
+            (rc = dalloc(&c->mono[0], 7 * cap)) || (rc = dalloc(&c->mono[1], 7 * cap))) {
+            free_ctx(c);
+            return rc;
+        }
+    }
+    if (p->mie_potential == 1) {
+        if ((rc = dalloc(&c->moltype[0], cap)) || (rc = dalloc(&c->moltype[1], cap))) {
+            free_ctx(c);
+            return rc;
+        }
+    }
+    if (p->ensemble == FM_ENSEMBLE_TAG_MOVE) {
+        if ((rc = dalloc(&c->tag[0], cap)) || (rc = dalloc(&c->tag[1], cap)) || (rc = dalloc(&c->teth[0], 6 * cap)) ||
+            (rc = dalloc(&c->teth[1], 6 * cap))) {
+            free_ctx(c);
+            return rc;
+        }
+    }
+    if (p->pressure_outflag == 1 && (rc = dalloc(&c->rf, 9 * cap))) {
+        free_ctx(c);
+        return rc;
+    }
+    if (p->vflux_outflag >= 1 && p->vflux_outflag <= 3) {
+        if ((rc = dalloc(&c->pxy, 3 * (size_t)p->nplanes)) || (rc = dalloc(&c->planes, (size_t)p->nplanes))) {
+            free_ctx(c);
+            return rc;
+        }
+        // MOP planes, src/setup_set_parameters.f90:2298-2340
+        int pn = p->vflux_outflag - 1;
+        double shift = 0.5 * (p->domain[pn] - p->planespacing * (p->nplanes - 1));
+        c->mop_shift = shift;
+        std::vector<double> pl(p->nplanes);
+        for (int n = 1; n <= p->nplanes; ++n) pl[n - 1] = p->planespacing * (n - 1) + shift - c->half[pn];
+        if (hipMemcpy(c->planes, pl.data(), pl.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            free_ctx(c);
+            return fm_set_error(FM_EHIP, "planes upload failed");
+        }
+    }
+    {
+        std::vector<double> mie(7 * T2 + FM_MAX_TYPES, 0.0);
+        for (int t = 0; t < T2; ++t) {
+            mie[t] = p->epsilon_lookup[t];
+            mie[T2 + t] = p->lambdar_lookup[t];
+            mie[2 * T2 + t] = p->lambdaa_lookup[t];
+            mie[3 * T2 + t] = p->C_lookup[t];
+            mie[4 * T2 + t] = p->alpha_lookup[t];
+            mie[5 * T2 + t] = p->potshift_lookup[t];
+            mie[6 * T2 + t] = p->rcutoff2_lookup[t];
+        }
+        for (int t = 0; t < FM_MAX_TYPES; ++t) mie[7 * T2 + t] = p->mie_potential == 1 ? p->mass_lookup[t] : 1.0;
+        if (hipMemcpy(c->d_mie, mie.data(), mie.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            free_ctx(c);
+            return fm_set_error(FM_EHIP, "table upload failed");
+        }
+    }
+    c->cub_bytes = cub_temp_bytes((int)std::max<size_t>(cap, (size_t)c->ncell + 1));
+    if ((rc = dalloc((char**)&c->cub_tmp, c->cub_bytes))) {
+        free_ctx(c);
+        return rc;
+    }
+    he = hipHostMalloc((void**)&c->h_flags, FLAG_COUNT * sizeof(int), hipHostMallocDefault);
+    if (he != hipSuccess) {
+        free_ctx(c);
+        return fm_set_error(FM_EHIP, "hipHostMalloc: %s", hipGetErrorString(he));
+    }
+    c->timer = new KTimer();
+    if ((rc = launch_fill_i32(c, c->d_flags, 0, FLAG_COUNT)) || (rc = launch_fill_f64(c, c->d_rmax, 0.0, 1)) ||
+        (rc = launch_fill_f64(c, c->st[0], 0.0, 9 * cap)) || (rc = launch_fill_f64(c, c->pe_lj, 0.0, cap)) ||
+        (rc = launch_fill_f64(c, c->pe_poly, 0.0, cap)) || (rc = launch_fill_f64(c, c->vir, 0.0, cap))) {
+        free_ctx(c);
+        return rc;
+    }
+    if (c->pxy && (rc = launch_fill_f64(c, c->pxy, 0.0, 3 * (size_t)p->nplanes))) {
+        free_ctx(c);
+        return rc;
+    }
+    if (launch_fill_f64(c, (double*)c->d_vmax, 0.0, 1) || hipStreamSynchronize(c->stream) != hipSuccess) {
+        free_ctx(c);
+        return fm_set_error(FM_EHIP, "initialisation failed");
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" void fm_finalize(fm_ctx* c) {
+    if (c) hipStreamSynchronize(c->stream);
+    free_ctx(c);
+}
+
+extern "C" int fm_get_geometry(const fm_ctx* c, int32_t ncells[3], double cellsidelength[3], double* rneighbr2,
+                               double* rcutoff2, double* potshift) {
+    CHECK_CTX(c);
+    for (int d = 0; d < 3; ++d) {
+        if (ncells) ncells[d] = c->nc[d];
+        if (cellsidelength) cellsidelength[d] = c->cs[d];
+    }
+    if (rneighbr2) *rneighbr2 = c->rn2;
+    if (rcutoff2) *rcutoff2 = c->rc2;
+    if (potshift) *potshift = c->potshift;
+    return 0;
+}
+
+extern "C" int fm_get_np(const fm_ctx* c, int32_t* np) {
+    CHECK_CTX(c);
+    *np = c->np;
+    return 0;
+}
+
+extern "C" double fm_simtime(const fm_ctx* c) { return c ? c->simtime : 0.0; }
+
+// ---- state transfer ---------------------------------------------------------------------------
+extern "C" int fm_upload_state(fm_ctx* c, int32_t np, const double* r, const double* v, const double* a) {
+    CHECK_CTX(c);
+    if (np < 0 || np > c->cap) return fm_set_error(FM_EINVAL, "np %d exceeds capacity %d", np, c->cap);
+    if (!r || !v) return fm_set_error(FM_EINVAL, "r and v are required");
+    c->np = np;
+    const size_t n3 = 3 * (size_t)np;
+    FM_HIP(hipMemcpyAsync(c->stage, r, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    FM_HIP(hipMemcpyAsync(c->stage + n3, v, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (a) FM_HIP(hipMemcpyAsync(c->stage + 2 * n3, a, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    else TRY(launch_fill_f64(c, c->stage + 2 * n3, 0.0, n3));
+    TRY(launch_from_host(c, c->stage));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    c->cells_valid = c->nlist_valid = c->rdisp_valid = false;
+    c->pe_valid = c->vir_valid = false;
+    // per-molecule properties are in the previous slot order: they must be uploaded again
+    c->has_moltype = c->has_tags = c->has_mono = false;
+    return 0;
+}
+
+static int upload_i32(fm_ctx* c, int* dst, const int32_t* src, int nf) {
+    const size_t n = (size_t)nf * c->np;
+    int* tmp = (int*)c->stage;  // staging holds 9*cap doubles >= 7*cap ints
+    FM_HIP(hipMemcpyAsync(tmp, src, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    TRY(launch_ext_to_slot_i32(c, dst, tmp, nf));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int fm_upload_moltype(fm_ctx* c, const int32_t* moltype) {
+    CHECK_CTX(c);
+    if (!c->moltype[0]) return fm_set_error(FM_EINVAL, "moltype needs mie_potential = 1");
+    for (int n = 0; n < c->np; ++n)
+        if (moltype[n] < 0 || moltype[n] >= c->p.ntypes) return fm_set_error(FM_EINVAL, "moltype out of range at %d", n);
+    TRY(upload_i32(c, c->moltype[c->cur], moltype, 1));
+    c->has_moltype = true;
+    return 0;
+}
+
+extern "C" int fm_upload_tags(fm_ctx* c, const int32_t* tag, const double* rtether) {
+    CHECK_CTX(c);
+    if (!c->tag[0]) return fm_set_error(FM_EINVAL, "tags need ensemble = tag_move");
+    for (int n = 0; n < c->np; ++n)
+        if (tag[n] != FM_TAG_FREE && tag[n] != FM_TAG_TETH && tag[n] != FM_TAG_TETH_SLIDE)
+            return fm_set_error(FM_EINVAL, "unsupported tag %d at molecule %d", tag[n], n + 1);
+    TRY(upload_i32(c, c->tag[c->cur], tag, 1));
+    // rtether (AoS) and slidev from read_tag (src/molecular_properties.f90:764-830)
+    std::vector<double> h(6 * (size_t)c->np, 0.0), r(3 * (size_t)c->np);
+    FM_HIP(hipStreamSynchronize(c->stream));
+    if (fm_download_state(c, r.data(), nullptr, nullptr)) return FM_EHIP;
+    for (int n = 0; n < c->np; ++n) {
+        for (int d = 0; d < 3; ++d) h[6 * (size_t)n + d] = rtether ? rtether[3 * (size_t)n + d] : r[3 * (size_t)n + d];
+        double sg = tag[n] == FM_TAG_TETH_SLIDE ? copysign(1.0, r[3 * (size_t)n + 1]) : 0.0;
+        for (int d = 0; d < 3; ++d) h[6 * (size_t)n + 3 + d] = tag[n] == FM_TAG_TETH_SLIDE ? c->p.wallslidev[d] * sg : 0.0;
+    }
+    FM_HIP(hipMemcpyAsync(c->stage, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TRY(launch_ext_to_slot_f64(c, c->teth[c->cur], c->stage, 6));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    c->has_tags = true;
+    return 0;
+}
+
+extern "C" int fm_upload_monomers(fm_ctx* c, const int32_t* chainID, const int32_t* subchainID, const int32_t* funcy,
+                                  const int32_t* bin_bflag) {
+    CHECK_CTX(c);
+    if (!c->mono[0]) return fm_set_error(FM_EINVAL, "monomers need potential_flag = 1");
+    const size_t np = c->np;
+    std::vector<int32_t> h(7 * np);
+    for (size_t n = 0; n < np; ++n) {
+        if (funcy[n] < 0 || funcy[n] > FM_MAX_FUNCY) return fm_set_error(FM_EINVAL, "funcy out of range at %zu", n);
+        h[7 * n] = chainID[n];
+        h[7 * n + 1] = subchainID[n];
+        h[7 * n + 2] = funcy[n];
+        for (int g = 0; g < 4; ++g) h[7 * n + 3 + g] = bin_bflag[4 * n + g];
+    }
+    TRY(upload_i32(c, c->mono[c->cur], h.data(), 7));
+    c->has_mono = true;
+    return 0;
+}
+
+extern "C" int fm_download_state(fm_ctx* c, double* r, double* v, double* a) {
+    CHECK_CTX(c);
+    const size_t n3 = 3 * (size_t)c->np;
+    TRY(launch_scatter_state_to_host_order(c, c->stage, nullptr));
+    if (r) FM_HIP(hipMemcpyAsync(r, c->stage, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (v) FM_HIP(hipMemcpyAsync(v, c->stage + n3, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (a) FM_HIP(hipMemcpyAsync(a, c->stage + 2 * n3, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int fm_download_tags(fm_ctx* c, double* rtether) {
+    CHECK_CTX(c);
+    if (!c->has_tags) return fm_set_error(FM_EINVAL, "no tags uploaded");
+    TRY(launch_slot_to_ext_f64(c, c->stage, c->teth[c->cur], 3));
+    FM_HIP(hipMemcpyAsync(rtether, c->stage, 3 * (size_t)c->np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int fm_download_molecular(fm_ctx* c, double* potenergymol, double* virialmol) {
+    CHECK_CTX(c);
+    const size_t np = c->np;
+    std::vector<double> pl(np), pp(np, 0.0);
+    if (potenergymol) {
+        if (c->pe_valid) {
+            TRY(launch_slot_to_ext_f64(c, c->stage, c->pe_lj, 1));
+            FM_HIP(hipMemcpyAsync(pl.data(), c->stage, np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            FM_HIP(hipStreamSynchronize(c->stream));
+            if (c->p.potential_flag == 1 && c->has_mono) {
+                TRY(launch_slot_to_ext_f64(c, c->stage, c->pe_poly, 1));
+                FM_HIP(hipMemcpyAsync(pp.data(), c->stage, np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                FM_HIP(hipStreamSynchronize(c->stream));
+            }
+            for (size_t n = 0; n < np; ++n) potenergymol[n] = pl[n] + pp[n];
+        } else {
+            for (size_t n = 0; n < np; ++n) potenergymol[n] = 0.0;
+        }
+    }
+    if (virialmol) {
+        if (c->vir_valid) {
+            TRY(launch_slot_to_ext_f64(c, c->stage, c->vir, 1));
+            FM_HIP(hipMemcpyAsync(virialmol, c->stage, np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            FM_HIP(hipStreamSynchronize(c->stream));
+        } else {
+            for (size_t n = 0; n < np; ++n) virialmol[n] = 0.0;
+        }
+    }
+    return 0;
+}
+
+extern "C" int fm_download_rfmol(fm_ctx* c, double* rfmol) {
+    CHECK_CTX(c);
+    if (!c->rf) return fm_set_error(FM_EINVAL, "rfmol needs pressure_outflag = 1");
+    TRY(launch_slot_to_ext_f64(c, c->stage, c->rf, 9));
+    FM_HIP(hipMemcpyAsync(rfmol, c->stage, 9 * (size_t)c->np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---- the routines -----------------------------------------------------------------------------
+extern "C" int fm_move_particles_vv(fm_ctx* c, int32_t pass_num) {
+    CHECK_CTX(c);
+    if (pass_num == 1) {
+        TRY(launch_pass1(c));
+    } else if (pass_num == 2) {
+        TRY(launch_pass2(c));
+        c->simtime = c->simtime + c->p.delta_t;
+    } else {
+        return fm_set_error(FM_EINVAL, "pass_num must be 1 or 2");
+    }
+    return 0;
+}
+
+// queue the rebuild check of step iter on the device; the decision lands in d_flags[FLAG_REBUILD]
+// except for the host-known cases returned in *host_rb
+static int queue_checkrebuild(fm_ctx* c, int iter, int* host_rb) {
+    const int forced = ((iter + 1) % c->p.tplot) == 0;
+    *host_rb = forced;
+    switch (c->p.rebuild_criteria) {
+    case FM_REBUILD_VMAX:
+        TRY(launch_check_vmax(c));
+        if (forced) {  // rmax is reset on any rebuild
+            TRY(launch_fill_f64(c, c->d_rmax, 0.0, 1));
+        }
+        break;
+    case FM_REBUILD_FIXED:
+        if (iter % c->p.fixed_rebuild == 0) *host_rb = 1;
+        break;
+    case FM_REBUILD_DISPLACEMENT:
+        // on a forced rebuild the displacement sums are discarded, so the update is skipped
+        if (!forced) TRY(launch_check_disp(c));
+        break;
+    }
+    return 0;
+}
+
+extern "C" int fm_checkrebuild(fm_ctx* c, int32_t iter, int32_t* rebuild) {
+    CHECK_CTX(c);
+    if (!rebuild) return fm_set_error(FM_EINVAL, "null rebuild");
+    int host_rb = 0;
+    TRY(queue_checkrebuild(c, iter, &host_rb));
+    TRY(sync_flags(c));
+    int rb = host_rb || c->h_flags[FLAG_REBUILD];
+    if (c->h_flags[FLAG_REBUILD]) TRY(launch_fill_i32(c, c->d_flags + FLAG_REBUILD, 0, 1));
+    if (rb) {
+        c->rdisp_valid = false;
+        if (c->p.rebuild_criteria == FM_REBUILD_VMAX) TRY(launch_fill_f64(c, c->d_rmax, 0.0, 1));
+    }
+    *rebuild = rb;
+    return 0;
+}
+
+extern "C" int fm_linklist_deallocateall(fm_ctx* c) {
+    CHECK_CTX(c);
+    c->cells_valid = false;
+    c->nlist_valid = false;
+    return 0;
+}
+
+extern "C" int fm_sendmols(fm_ctx* c) {
+    CHECK_CTX(c);
+    TRY(launch_sendmols(c));
+    c->cells_valid = c->nlist_valid = false;
+    return 0;
+}
+
+extern "C" int fm_sort_mols(fm_ctx* c) {
+    CHECK_CTX(c);
+    return 0;  // SORT_FLAG 0: molecule numbering unchanged (the slot order is internal)
+}
+
+extern "C" int fm_assign_to_cell(fm_ctx* c) {
+    CHECK_CTX(c);
+    TRY(launch_fill_i32(c, c->d_flags + FLAG_MAX_CELL, 0, 1));
+    TRY(launch_bin(c));
+    TRY(sync_flags(c));
+    c->cells_valid = true;
+    c->nlist_valid = false;
+    return 0;
+}
+
+extern "C" int fm_updateborders(fm_ctx* c, int32_t) {
+    CHECK_CTX(c);
+    return 0;
+}
+
+extern "C" int fm_assign_to_neighbourlist(fm_ctx* c) {
+    CHECK_CTX(c);
+    if (!c->cells_valid) return fm_set_error(FM_ESTATE, "assign_to_neighbourlist before assign_to_cell");
+    if (c->p.potential_flag == 1 && !c->has_mono) return fm_set_error(FM_ESTATE, "potential_flag 1 needs monomer data (fm_upload_monomers)");
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        TRY(launch_fill_i32(c, c->d_flags + FLAG_MAX_NB, 0, 1));
+        TRY(launch_nlist(c));
+        FM_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, FLAG_COUNT * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        FM_HIP(hipStreamSynchronize(c->stream));
+        int need = c->h_flags[FLAG_MAX_NB];
+        if (need <= c->maxnb || c->h_flags[FLAG_ERROR] != FM_EOVERFLOW) break;
+        // grow the list and rebuild
+        TRY(launch_fill_i32(c, c->d_flags + FLAG_ERROR, 0, 2));
+        TRY(alloc_nlist(c, ((need + need / 4 + 8) + 3) & ~3));
+    }
+    TRY(sync_flags(c));
+    c->nlist_valid = true;
+    return 0;
+}
+
+extern "C" int fm_compute_forces(fm_ctx* c, int32_t iter) {
+    CHECK_CTX(c);
+    if (!c->nlist_valid) return fm_set_error(FM_ESTATE, "simulation_compute_forces before assign_to_neighbourlist");
+    if (c->p.mie_potential == 1 && !c->has_moltype) return fm_set_error(FM_ESTATE, "Mie potential needs moltype (fm_upload_moltype)");
+    if (c->p.potential_flag == 1 && !c->has_mono) return fm_set_error(FM_ESTATE, "potential_flag 1 needs monomer data (fm_upload_monomers)");
+    if (c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && !c->has_tags) return fm_set_error(FM_ESTATE, "tag_move needs tags (fm_upload_tags)");
+    TRY(launch_forces(c, iter));
+    const bool rec = (iter % c->p.tplot) == 0;
+    const bool ev = c->p.teval > 0 && (iter % c->p.teval) == 0;
+    c->pe_valid = rec;
+    c->vir_valid = rec || ev;
+    if (c->p.ensemble == FM_ENSEMBLE_TAG_MOVE || c->p.potential_flag == 1) TRY(sync_flags(c));
+    return 0;
+}
+
+static void derive_macros(const fm_ctx* c, const double* s, fm_macros* o) {
+    memset(o, 0, sizeof *o);
+    const double np = (double)c->np;
+    o->potenergysum_LJ = s[0];
+    o->potenergysum_POLY = s[1];
+    o->potenergysum = (c->p.potential_flag == 1) ? s[2] : s[0];
+    o->virial = s[3];
+    o->mv2sum = s[4];
+    o->vsum = s[5];
+    o->msum = s[6];
+    o->max_potenergymol = s[7];
+    o->kinenergy = (0.5 * o->mv2sum) / np;
+    o->potenergy = (0.5 * o->potenergysum) / np;
+    o->totenergy = o->kinenergy + o->potenergy;
+    o->temperature = o->mv2sum / (double)(3 * c->np);
+    o->pressure = (c->p.density / (double)(c->np * 3)) * (o->mv2sum + o->virial / 2.0);
+}
+
+extern "C" int fm_evaluate_macroscopic_properties(fm_ctx* c, fm_macros* out) {
+    CHECK_CTX(c);
+    if (!out) return fm_set_error(FM_EINVAL, "null output");
+    TRY(launch_macros(c));
+    double s[8];
+    FM_HIP(hipMemcpyAsync(s, c->d_macros, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    derive_macros(c, s, out);
+    return 0;
+}
+
+extern "C" int fm_get_mop(fm_ctx* c, double* pxy, int32_t nplanes) {
+    CHECK_CTX(c);
+    if (!c->pxy) return fm_set_error(FM_EINVAL, "MOP needs vflux_outflag 1..3");
+    if (nplanes != c->p.nplanes) return fm_set_error(FM_EINVAL, "nplanes mismatch");
+    FM_HIP(hipMemcpyAsync(pxy, c->pxy, 3 * (size_t)nplanes * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int fm_reset_mop(fm_ctx* c) {
+    CHECK_CTX(c);
+    if (!c->pxy) return 0;
+    TRY(launch_fill_f64(c, c->pxy, 0.0, 3 * (size_t)c->p.nplanes));
+    return 0;
+}
+
+// ---- fused md_advance_vv loop (src/md.f90:143-163) ---------------------------------------------
+extern "C" int fm_run_vv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* macros, int32_t n_macros,
+                         int32_t* n_rebuilds) {
+    CHECK_CTX(c);
+    if (!c->nlist_valid) return fm_set_error(FM_ESTATE, "run before the initial neighbour list");
+    int nrb = 0, nm = 0;
+    std::vector<int> rec_iters;
+    for (int iter = iter0; iter < iter0 + nsteps; ++iter) {
+        TRY(fm_move_particles_vv(c, 1));
+        int rb = 0;
+        TRY(fm_checkrebuild(c, iter, &rb));
+        if (rb) {
+            TRY(fm_linklist_deallocateall(c));
+            TRY(fm_sendmols(c));
+            TRY(fm_sort_mols(c));
+            TRY(fm_assign_to_cell(c));
+            TRY(fm_updateborders(c, 1));
+            TRY(fm_assign_to_neighbourlist(c));
+            ++nrb;
+        }
+        TRY(fm_compute_forces(c, iter));
+        TRY(fm_move_particles_vv(c, 2));
+        if (iter % c->p.tplot == 0 && macros && nm < n_macros) {
+            TRY(fm_evaluate_macroscopic_properties(c, &macros[nm]));
+            ++nm;
+        }
+    }
+    TRY(sync_flags(c));
+    if (n_rebuilds) *n_rebuilds = nrb;
+    return 0;
+}
+
+// ---- inspection -------------------------------------------------------------------------------
+extern "C" int fm_download_cells(fm_ctx* c, int32_t* cell_of_mol) {
+    CHECK_CTX(c);
+    if (!c->cells_valid) return fm_set_error(FM_ESTATE, "no cell lists");
+    std::vector<int> cs(c->np);
+    TRY(launch_slot_to_ext_i32(c, (int*)c->stage, c->cell_of, 1));
+    FM_HIP(hipMemcpyAsync(cs.data(), c->stage, c->np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    const int nx = c->nc[0], ny = c->nc[1];
+    for (int n = 0; n < c->np; ++n) {
+        int ci = cs[n];
+        int cx = ci % nx, cy = (ci / nx) % ny, cz = ci / (nx * ny);
+        cell_of_mol[n] = ((cz + 1) * (ny + 2) + (cy + 1)) * (nx + 2) + (cx + 1);
+    }
+    return 0;
+}
+
+extern "C" int fm_download_neighbourlist(fm_ctx* c, int32_t* mol_i, int32_t* mol_j, int32_t* shift, int64_t capacity,
+                                         int64_t* n_entries) {
+    CHECK_CTX(c);
+    if (!c->nlist_valid) return fm_set_error(FM_ESTATE, "no neighbour list");
+    const size_t cap = c->cap, np = c->np;
+    std::vector<int> nbn(np), ext(np);
+    FM_HIP(hipMemcpyAsync(nbn.data(), c->nbn, np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipMemcpyAsync(ext.data(), c->ext[c->cur], np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    int64_t total = 0;
+    for (size_t i = 0; i < np; ++i) total += nbn[i];
+    if (n_entries) *n_entries = total;
+    if (!mol_i) return 0;
+    if (capacity < total) return fm_set_error(FM_EINVAL, "capacity %lld < %lld entries", (long long)capacity, (long long)total);
+    std::vector<uint32_t> row(np);
+    std::vector<int64_t> off(np + 1, 0);
+    for (size_t i = 0; i < np; ++i) off[i + 1] = off[i] + nbn[i];
+    for (int k = 0; k < c->maxnb; ++k) {
+        FM_HIP(hipMemcpy(row.data(), c->nbl + (size_t)k * cap, np * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        bool any = false;
+        for (size_t i = 0; i < np; ++i) {
+            if (k >= nbn[i]) continue;
+            any = true;
+            int64_t q = off[i] + k;
+            mol_i[q] = ext[i];
+            mol_j[q] = ext[row[i] >> 5];
+            shift[q] = (int)(row[i] & 31u);
+        }
+        if (!any) break;
+    }
+    return 0;
+}
+
+extern "C" int fm_download_bonds(fm_ctx* c, int32_t* bond, int32_t* bond_shift) {
+    CHECK_CTX(c);
+    if (!c->bond || !c->nlist_valid) return fm_set_error(FM_ESTATE, "no bond lists");
+    const size_t cap = c->cap, np = c->np;
+    std::vector<int> bn(np), ext(np);
+    std::vector<uint32_t> rows(FM_MAX_FUNCY * cap);
+    FM_HIP(hipMemcpyAsync(bn.data(), c->bondn, np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipMemcpyAsync(ext.data(), c->ext[c->cur], np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipMemcpyAsync(rows.data(), c->bond, FM_MAX_FUNCY * cap * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < np; ++i) {
+        size_t e = ext[i];
+        for (int b = 0; b < FM_MAX_FUNCY; ++b) {
+            if (b < bn[i]) {
+                uint32_t w = rows[(size_t)b * cap + i];
+                bond[FM_MAX_FUNCY * e + b] = ext[w >> 5];
+                bond_shift[FM_MAX_FUNCY * e + b] = (int)(w & 31u);
+            } else {
+                bond[FM_MAX_FUNCY * e + b] = -1;
+                bond_shift[FM_MAX_FUNCY * e + b] = 13;
+            }
+        }
+    }
+    return 0;
+}
+
+extern "C" int fm_synchronize(fm_ctx* c) {
+    CHECK_CTX(c);
+    FM_HIP(hipStreamSynchronize(c->stream));
+    return sync_flags(c);
+}
+
+extern "C" int fm_kernel_timing(fm_ctx* c, int32_t enable) {
+    CHECK_CTX(c);
+    kt_collect(c);
+    c->timer->on = enable != 0;
+    for (int k = 0; k < KT_NCLASS; ++k) {
+        c->timer->total_ms[k] = 0.0;
+        c->timer->count[k] = 0;
+    }
+    return 0;
+}
+
+extern "C" int fm_get_kernel_timing(fm_ctx* c, char* names, int32_t names_cap, double* avg_ms, int64_t* counts,
+                                    int32_t max_classes, int32_t* n_classes) {
+    CHECK_CTX(c);
+    kt_collect(c);
+    std::string s;
+    int n = std::min<int>(KT_NCLASS, max_classes);
+    for (int k = 0; k < n; ++k) {
+        s += kt_names[k];
+        s += '\n';
+        if (avg_ms) avg_ms[k] = c->timer->count[k] ? c->timer->total_ms[k] / c->timer->count[k] : 0.0;
+        if (counts) counts[k] = c->timer->count[k];
+    }
+    if (names && names_cap > 0) {
+        strncpy(names, s.c_str(), names_cap - 1);
+        names[names_cap - 1] = 0;
+    }
+    if (n_classes) *n_classes = n;
+    return 0;
+}

@@ -1,0 +1,154 @@
+// fm_internal.h -- device context and kernel launchers of libflowmol_hip (gfx950).
+//
+// Data layout in HBM (one context = one periodic domain):
+//   * molecule state in "slot" order, structure of arrays: x,y,z, vx,vy,vz, ax,ay,az (FP64).
+//     Slots are re-ordered by cell at every assign_to_cell (counting sort), so a cell's molecules
+//     are contiguous; inside a cell they are ordered by decreasing molecule number, the order of
+//     the reference's head-inserted cell lists.  ext[slot] is the host-visible molecule number
+//     (0-based), slot_of[molno] its inverse.
+//   * cells: interior cells only (ncx*ncy*ncz), x fastest; cell_start[c]..cell_start[c+1].
+//   * neighbour list: ELL, entry k of slot i at nbl[k*cap + i] (coalesced across a wavefront),
+//     word = (j_slot << 5) | shift_code, shift_code = (sx+1)*9+(sy+1)*3+(sz+1): the image of j is
+//     r_j + s*domain, formed component by component exactly as the reference's halo copy.
+//   * bonds: bond[b*cap + i] same encoding, FM_MAX_FUNCY per slot.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/flowmol_hip.h"
+
+#define FM_BLOCK 256
+
+// device flag words (d_flags)
+enum {
+    FLAG_REBUILD = 0,     // displacement / vmax criterion fired
+    FLAG_ERROR = 1,       // first FM_E* error code
+    FLAG_ERROR_MOL = 2,   // molecule number of the first error
+    FLAG_MAX_CELL = 3,    // largest cell occupancy seen at the last binning
+    FLAG_MAX_NB = 4,      // largest neighbour count at the last build
+    FLAG_NOUT = 5,        // molecules outside the box at sendmols
+    FLAG_EDGE = 6,        // molecules exactly on the lower domain face (clamped into cell 1)
+    FLAG_COUNT = 8
+};
+
+struct KTimer;
+
+struct fm_ctx {
+    fm_params p;
+    int np = 0, cap = 0;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+
+    // geometry (set_parameters_cells)
+    int nc[3] = {0, 0, 0};
+    int ncell = 0;
+    double half[3], cs[3], rn2, rc2, potshift, dmin;
+
+    // state, slot order (two copies for the permutation at rebuild)
+    double* st[2] = {nullptr, nullptr};   // each: 9*cap doubles: x y z vx vy vz ax ay az
+    int cur = 0;
+    double* rdisp = nullptr;              // 3*cap
+    int* ext[2] = {nullptr, nullptr};
+    int* slot_of = nullptr;
+    int* moltype[2] = {nullptr, nullptr};
+    int* tag[2] = {nullptr, nullptr};
+    double* teth[2] = {nullptr, nullptr}; // 6*cap: rtether xyz, slidev xyz
+    int* mono[2] = {nullptr, nullptr};    // 7*cap: chainID subchainID funcy bflag[4]
+    bool has_moltype = false, has_tags = false, has_mono = false;
+
+    // cells
+    int* cell_of = nullptr;               // per slot: interior cell index
+    int* cell_count = nullptr;            // ncell
+    int* cell_start = nullptr;            // ncell+1
+    int* cell_cursor = nullptr;           // ncell
+    int* perm = nullptr;                  // cap
+
+    // neighbour list
+    int maxnb = 0;
+    uint32_t* nbl = nullptr;
+    int* nbn = nullptr;
+    uint32_t* bond = nullptr;             // FM_MAX_FUNCY*cap
+    int* bondn = nullptr;
+
+    // per-molecule tallies
+    double* pe_lj = nullptr;
+    double* pe_poly = nullptr;
+    double* vir = nullptr;
+    double* rf = nullptr;                 // 9*cap
+    double* pxy = nullptr;                // 3*nplanes
+    double* planes = nullptr;
+    double mop_shift = 0.0;
+
+    // sendmols scratch
+    int* out_flag = nullptr;              // cap
+    int* out_vals = nullptr;              // cap
+    int* out_slot = nullptr;              // cap
+    int* out_count = nullptr;             // 1
+
+    // reductions
+    double* red = nullptr;                // partial sums
+    int nred_blocks = 0;
+    double* d_macros = nullptr;           // 16 doubles
+    double* d_mie = nullptr;              // eps, lr, la, C, alpha, potshift, rc2 tables + mass[4]
+    unsigned long long* d_vmax = nullptr; // bits of max |v|^2
+    double* d_rmax = nullptr;
+
+    int* d_flags = nullptr;
+    int* h_flags = nullptr;               // pinned
+    double* stage = nullptr;              // 9*cap doubles: host-order staging
+    bool pe_valid = false, vir_valid = false;
+
+    void* cub_tmp = nullptr;
+    size_t cub_bytes = 0;
+
+    bool cells_valid = false, nlist_valid = false, rdisp_valid = false;
+    bool forces_valid = false;
+    double simtime = 0.0;
+    int last_record_iter = -1;
+
+    KTimer* timer = nullptr;
+
+    double* x() const { return st[cur]; }
+};
+
+// ---- error handling -----------------------------------------------------------------------------
+int fm_set_error(int code, const char* fmt, ...);
+#define FM_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return fm_set_error(FM_EHIP, "%s failed: %s (%s:%d)", #call,               \
+                                hipGetErrorString(_e), __FILE__, __LINE__);            \
+    } while (0)
+
+// ---- kernel timing (HIP events on the context stream) -------------------------------------------
+void ktimer_begin(fm_ctx* c, int cls);
+void ktimer_end(fm_ctx* c, int cls);
+enum { KT_PASS1 = 0, KT_CHECK, KT_SENDMOLS, KT_BIN, KT_NLIST, KT_FORCES, KT_PASS2, KT_MACROS, KT_NCLASS };
+extern const char* const kt_names[KT_NCLASS];
+
+// ---- launchers (fm_kernels.hip) -----------------------------------------------------------------
+int launch_fill_i32(fm_ctx* c, int* p, int v, size_t n);
+int launch_fill_f64(fm_ctx* c, double* p, double v, size_t n);
+int launch_pass1(fm_ctx* c);
+int launch_pass2(fm_ctx* c);
+int launch_check_disp(fm_ctx* c);
+int launch_check_vmax(fm_ctx* c);
+int launch_sendmols(fm_ctx* c);
+int launch_bin(fm_ctx* c);
+int launch_nlist(fm_ctx* c);
+int launch_forces(fm_ctx* c, int iter);
+int launch_macros(fm_ctx* c);
+int launch_scatter_state_to_host_order(fm_ctx* c, double* d_out9, double* d_tally);
+int launch_from_host(fm_ctx* c, const double* d_in9);
+int launch_ext_to_slot_i32(fm_ctx* c, int* dst, const int* src, int nf);
+int launch_ext_to_slot_f64(fm_ctx* c, double* dst, const double* src, int nf);
+int launch_slot_to_ext_f64(fm_ctx* c, double* dst, const double* src, int nf);
+int launch_slot_to_ext_i32(fm_ctx* c, int* dst, const int* src, int nf);
+
+// hipcub wrappers (fm_cub.hip)
+int cub_exclusive_scan(fm_ctx* c, const int* in, int* out, int n);
+int cub_select_flagged_index(fm_ctx* c, const int* flags, int* out, int* count, int n);
+size_t cub_temp_bytes(int n_items);

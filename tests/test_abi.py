@@ -1,0 +1,71 @@
+"""The C-ABI library loads and exports every symbol include/flowmol_hip.h declares; the ctypes
+struct mirrors match the C layout.  No compute calls (runs without a GPU)."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from flowmol_amd import _abi, _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "flowmol_hip.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fm_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_parses():
+    names = declared_functions()
+    assert "fm_init" in names and "fm_compute_forces" in names and len(names) >= 30
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libflowmol_hip.so not built (python -m flowmol_amd.build)")
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (fm_[a-z0-9_]+)$", out, flags=re.M))
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, missing
+    # the ctypes table covers exactly the declared functions
+    assert sorted(_lib.SIGNATURES) == declared_functions()
+
+
+def test_library_loads_and_reports_version():
+    L = _lib.load()
+    assert L.fm_abi_version() == 1
+
+
+def _c_layout():
+    src = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "flowmol_hip.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(fm_params), sizeof(fm_macros),
+         offsetof(fm_params, k_c), offsetof(fm_params, planespacing),
+         offsetof(fm_params, max_cell_np), offsetof(fm_macros, max_potenergymol));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        exe = os.path.join(d, "l")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        return [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+
+
+def test_struct_layout_matches_header():
+    sp, sm, okc, ops, omc, omax = _c_layout()
+    assert C.sizeof(_abi.FmParams) == sp
+    assert C.sizeof(_abi.FmMacros) == sm
+    assert _abi.FmParams.k_c.offset == okc
+    assert _abi.FmParams.planespacing.offset == ops
+    assert _abi.FmParams.max_cell_np.offset == omc
+    assert _abi.FmMacros.max_potenergymol.offset == omax

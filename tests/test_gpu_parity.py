@@ -1,0 +1,188 @@
+"""Parity of the HIP path (through the C-ABI) against the CPU restatement of the reference.
+
+Bars: cell membership and neighbour sets bit-exact; forces, per-molecule potential energy and
+virial within 1e-12 relative (scale-aware for net forces: |da_i|_inf / sum_j |f_ij|); positions
+and velocities of the routine-by-routine drop-in sequence bit-exact where the arithmetic is
+exact (wrap, renumbering) and within 1e-10 absolute over short trajectories.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from flowmol_amd import config
+from flowmol_amd.md import MDContext
+from oracle.oracle import Oracle
+from tests.helpers import canonical_pairs, equilibrated, full_from_half, rel_force_err
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-12
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def c1_state():
+    return equilibrated(config.config_c1, 300)
+
+
+def _pair(p, r, v, a=None):
+    o = Oracle(p, r, v, a)
+    g = MDContext(p, r, v, a)
+    return o, g
+
+
+def test_cells_and_neighbourlist_c1(c1_state):
+    p, r, v, a = c1_state
+    say("case: cells+nlist C1 2048 WCA")
+    o, g = _pair(p, r, v, a)
+    o.assign_to_cell()
+    g.assign_to_cell()
+    assert np.array_equal(g.cells(), o.cells())
+    o.updateborders(1)
+    o.assign_to_neighbourlist()
+    g.messenger_updateborders(1)
+    g.assign_to_neighbourlist()
+    oi, oj, os_ = o.neighbourlist()
+    gi, gj, gs = g.neighbourlist()
+    assert np.array_equal(canonical_pairs(oi, oj, os_), canonical_pairs(gi, gj, gs))
+    # the GPU full list is exactly the reference half list with unshifted entries mirrored
+    fi, fj, fs = full_from_half(oi, oj, os_)
+    ko = np.sort((fi.astype(np.int64) << 32) | (fj.astype(np.int64) << 5) | fs)
+    kg = np.sort((gi.astype(np.int64) << 32) | (gj.astype(np.int64) << 5) | gs)
+    assert np.array_equal(ko, kg)
+    g.close()
+
+
+def test_forces_c1(c1_state):
+    p, r, v, a = c1_state
+    say("case: forces C1 2048 WCA")
+    o, g = _pair(p, r, v, a)
+    o.setup()
+    g.setup(0)
+    o.compute_forces(0)
+    _, _, ao = o.state()
+    _, _, ag = g.state()
+    scale = o.force_scale()
+    assert rel_force_err(ag, ao, scale) < FORCE_TOL
+    peo, viro = o.molecular()
+    peg, virg = g.molecular()
+    np.testing.assert_allclose(peg, peo, rtol=FORCE_TOL, atol=1e-14)
+    np.testing.assert_allclose(virg, viro, rtol=FORCE_TOL, atol=1e-12)
+    mo = o.macros()
+    mg = g.evaluate_macroscopic_properties()
+    for k in ("potenergy", "kinenergy", "pressure", "virial", "temperature"):
+        assert abs(mg[k] - mo[k]) <= 1e-12 * max(1.0, abs(mo[k])), (k, mg[k], mo[k])
+    g.close()
+
+
+def _lj_liquid(nunits, seed, steps=200, **kw):
+    p, r, v = config.config_fcc(nunits, 0.8, 2.5, seed=seed, **kw)
+    o = Oracle(p, r, v)
+    o.setup()
+    o.compute_forces(0)
+    o.run_vv(1, steps)
+    o.rebuild()
+    r2, v2, a2 = o.state()
+    return p, r2, v2, a2
+
+
+def test_forces_lj_rc25():
+    p, r, v, a = _lj_liquid((6, 6, 6), seed=7, tplot=10)
+    say("case: forces LJ rc2.5 864")
+    o, g = _pair(p, r, v, a)
+    o.setup()
+    g.setup(0)
+    o.compute_forces(0)
+    _, _, ao = o.state()
+    _, _, ag = g.state()
+    assert rel_force_err(ag, ao, o.force_scale()) < FORCE_TOL
+    # independent check: the reference's all-pairs minimum-image routine
+    o.compute_forces_allpairs(0)
+    _, _, aap = o.state()
+    assert rel_force_err(ag, aap, o.force_scale()) < FORCE_TOL
+    peo, viro = o.molecular()
+    peg, virg = g.molecular()
+    np.testing.assert_allclose(peg, peo, rtol=FORCE_TOL, atol=1e-13)
+    np.testing.assert_allclose(virg, viro, rtol=FORCE_TOL, atol=1e-11)
+    g.close()
+
+
+def test_sendmols_renumbering_bit_exact():
+    """Molecules pushed outside the box are wrapped and renumbered exactly as the single-rank
+    sendmols does (leavers of each axis/direction pass get their numbers reversed)."""
+    p, r, v = config.config_fcc((5, 5, 5), 0.8, 2.5, seed=3)
+    rng = np.random.default_rng(11)
+    L = np.array(p.domain[:])
+    r = r + rng.uniform(-0.15, 0.15, r.shape)
+    # push a few molecules out through faces, edges and corners
+    idx = rng.choice(r.shape[0], 60, replace=False)
+    r[idx] += rng.choice([-1, 0, 1], (60, 3)) * (0.5 * L + 0.01)
+    say("case: sendmols renumbering 500")
+    o, g = _pair(p, r, v)
+    o.sendmols()
+    g.sendmols()
+    ro, vo, _ = o.state()
+    rg, vg, _ = g.state()
+    assert np.array_equal(ro, rg)
+    assert np.array_equal(vo, vg)
+    assert (np.abs(rg) <= 0.5 * L).all()
+    g.close()
+
+
+def test_drop_in_sequence_short_trajectory(c1_state):
+    """md_advance_vv routine by routine on both sides: identical rebuild steps, states close."""
+    p, r, v, a = c1_state
+    say("case: 100-step drop-in trajectory C1")
+    o, g = _pair(p, r, v, a)
+    o.setup()
+    o.compute_forces(0)
+    g.setup(0)
+    for it in range(1, 101):
+        o.move_particles_vv(1)
+        o.updateborders(0)
+        rbo = o.checkrebuild(it)
+        if rbo:
+            o.rebuild()
+        o.compute_forces(it)
+        o.move_particles_vv(2)
+        rbg = g.md_advance_vv(it)
+        assert rbo == rbg, f"rebuild decision differs at iter {it}"
+    ro, vo, ao = o.state()
+    rg, vg, ag = g.state()
+    assert np.abs(rg - ro).max() < 1e-10
+    assert np.abs(vg - vo).max() < 1e-9
+    g.close()
+
+
+def test_fused_loop_equals_routine_sequence(c1_state):
+    p, r, v, a = c1_state
+    say("case: fused run_vv vs routine sequence C1")
+    g1 = MDContext(p, r, v, a)
+    g2 = MDContext(p, r, v, a)
+    g1.setup(0)
+    g2.setup(0)
+    for it in range(1, 51):
+        g1.md_advance_vv(it)
+    g2.run_vv(1, 50)
+    r1, v1, a1 = g1.state()
+    r2, v2, a2 = g2.state()
+    assert np.array_equal(r1, r2) and np.array_equal(v1, v2) and np.array_equal(a1, a2)
+    g1.close()
+    g2.close()
+
+
+def test_energy_conservation_c1():
+    p, r, v = config.config_c1()
+    say("case: NVE drift C1 1000 steps")
+    g = MDContext(p, r, v)
+    g.setup(0)
+    ms, nrb = g.run_vv(1, 1000, n_macros=10)
+    e = np.array([m["totenergy"] for m in ms])
+    assert nrb > 0
+    assert np.abs(e - e[0]).max() < 1e-3 * abs(e[0])
+    g.close()
