@@ -62,6 +62,8 @@ class FmParams(C.Structure):
         ("planespacing", C.c_double),
         ("max_neighbours", C.c_int32),
         ("max_cell_np", C.c_int32),
+        ("brick", C.c_int32 * 3),
+        ("_pad2", C.c_int32),
     ]
 
 

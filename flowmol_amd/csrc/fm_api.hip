@@ -145,8 +145,29 @@ static void free_ctx(fm_ctx* c) {
 static int alloc_nlist(fm_ctx* c, int maxnb) {
     if (c->nbl) hipFree(c->nbl);
     c->nbl = nullptr;
-    c->maxnb = maxnb;
-    return dalloc(&c->nbl, (size_t)maxnb * c->cap);
+    c->maxnb = (maxnb + 3) & ~3;
+    return dalloc(&c->nbl, (size_t)(c->maxnb / 4) * c->cap);
+}
+
+// Brick shape: the largest number of cells whose expected molecule count fits one pass of the
+// workgroup (240 of 256 threads), preferring the shape with the least staged halo per cell.
+static void choose_brick(const int nc[3], double occ, int out[3]) {
+    const double target = 240.0;
+    double best = -1.0;
+    out[0] = out[1] = out[2] = 1;
+    for (int bx = 1; bx <= std::min(nc[0], 12); ++bx)
+        for (int by = 1; by <= std::min(nc[1], 12); ++by)
+            for (int bz = 1; bz <= std::min(nc[2], 12); ++bz) {
+                const int P = bx * by * bz;
+                if (P > 1 && P * occ > target) continue;
+                const double eff = (double)P / ((bx + 2) * (by + 2) * (bz + 2));
+                if (eff > best + 1e-12) {
+                    best = eff;
+                    out[0] = bx;
+                    out[1] = by;
+                    out[2] = bz;
+                }
+            }
 }
 
 // ---- lifecycle --------------------------------------------------------------------------------
@@ -209,6 +230,15 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
         c->potshift = 4.0 * (1.0 / x12 - 1.0 / x6);
     }
     c->dmin = std::min(p->delta_rneighbr[0], std::min(p->delta_rneighbr[1], p->delta_rneighbr[2]));
+    if (p->brick[0] > 0 && p->brick[1] > 0 && p->brick[2] > 0) {
+        for (int d = 0; d < 3; ++d) c->brick[d] = std::min(p->brick[d], c->nc[d]);
+    } else {
+        choose_brick(c->nc, std::max(1.0, (double)np / c->ncell), c->brick);
+    }
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) c->lds_limit = (size_t)prop.sharedMemPerBlock;
+    }
 
     // neighbour capacity: 1.5x the expected count at the mean density, plus margin
     int maxnb = p->max_neighbours;
@@ -370,6 +400,7 @@ extern "C" int fm_upload_state(fm_ctx* c, int32_t np, const double* r, const dou
     FM_HIP(hipStreamSynchronize(c->stream));
     c->cells_valid = c->nlist_valid = c->rdisp_valid = false;
     c->pe_valid = c->vir_valid = false;
+    c->moved_since_bin = true;
     // per-molecule properties are in the previous slot order: they must be uploaded again
     c->has_moltype = c->has_tags = c->has_mono = false;
     return 0;
@@ -499,6 +530,7 @@ extern "C" int fm_download_rfmol(fm_ctx* c, double* rfmol) {
 extern "C" int fm_move_particles_vv(fm_ctx* c, int32_t pass_num) {
     CHECK_CTX(c);
     if (pass_num == 1) {
+        c->moved_since_bin = true;
         TRY(launch_pass1(c));
     } else if (pass_num == 2) {
         TRY(launch_pass2(c));
@@ -567,11 +599,28 @@ extern "C" int fm_sort_mols(fm_ctx* c) {
     return 0;  // SORT_FLAG 0: molecule numbering unchanged (the slot order is internal)
 }
 
+// largest staged region after binning: sizes the LDS of the brick kernels
+static int size_regions(fm_ctx* c) {
+    const int maxreg = c->h_flags[FLAG_MAX_REGION];
+    if (maxreg > 65535)
+        return fm_set_error(FM_EOVERFLOW, "brick region holds %d molecules (16-bit list indices); use a smaller brick", maxreg);
+    c->maxreg = std::max(maxreg, 1);
+    size_t need = std::max(brick_lds_bytes(c, c->maxreg, 0), brick_lds_bytes(c, c->maxreg, 2));
+    if (need > c->lds_limit)
+        return fm_set_error(FM_EOVERFLOW, "brick region needs %zu bytes of LDS (limit %zu); use a smaller brick", need,
+                            c->lds_limit);
+    return 0;
+}
+
 extern "C" int fm_assign_to_cell(fm_ctx* c) {
     CHECK_CTX(c);
     TRY(launch_fill_i32(c, c->d_flags + FLAG_MAX_CELL, 0, 1));
+    TRY(launch_fill_i32(c, c->d_flags + FLAG_MAX_REGION, 0, 1));
     TRY(launch_bin(c));
+    TRY(launch_region_max(c));
     TRY(sync_flags(c));
+    TRY(size_regions(c));
+    c->moved_since_bin = false;
     c->cells_valid = true;
     c->nlist_valid = false;
     return 0;
@@ -663,28 +712,74 @@ extern "C" int fm_reset_mop(fm_ctx* c) {
 }
 
 // ---- fused md_advance_vv loop (src/md.f90:143-163) ---------------------------------------------
+// Same operations as the call-per-routine sequence; VV pass 1 and the displacement update share
+// one kernel, and VV pass 2 is applied by the force kernel as it writes a.
+static int rebuild_lists(fm_ctx* c) {
+    c->cells_valid = c->nlist_valid = false;
+    TRY(launch_sendmols(c));
+    TRY(fm_assign_to_cell(c));
+    return fm_assign_to_neighbourlist(c);
+}
+
 extern "C" int fm_run_vv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* macros, int32_t n_macros,
                          int32_t* n_rebuilds) {
     CHECK_CTX(c);
     if (!c->nlist_valid) return fm_set_error(FM_ESTATE, "run before the initial neighbour list");
+    if (c->p.mie_potential == 1 && !c->has_moltype) return fm_set_error(FM_ESTATE, "Mie potential needs moltype");
+    if (c->p.potential_flag == 1 && !c->has_mono) return fm_set_error(FM_ESTATE, "potential_flag 1 needs monomer data");
+    if (c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && !c->has_tags) return fm_set_error(FM_ESTATE, "tag_move needs tags");
+    const bool sync_each_force = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE || c->p.potential_flag == 1;
     int nrb = 0, nm = 0;
-    std::vector<int> rec_iters;
     for (int iter = iter0; iter < iter0 + nsteps; ++iter) {
-        TRY(fm_move_particles_vv(c, 1));
-        int rb = 0;
-        TRY(fm_checkrebuild(c, iter, &rb));
+        const int forced = ((iter + 1) % c->p.tplot) == 0;
+        int rb = forced;
+        bool need_flag = false;
+        switch (c->p.rebuild_criteria) {
+        case FM_REBUILD_DISPLACEMENT:
+            c->moved_since_bin = true;
+            if (forced) {
+                TRY(launch_pass1(c));
+            } else {
+                TRY(launch_pass1_check_disp(c));
+                need_flag = true;
+            }
+            break;
+        case FM_REBUILD_VMAX:
+            c->moved_since_bin = true;
+            TRY(launch_pass1(c));
+            TRY(launch_check_vmax(c));
+            need_flag = !forced;
+            break;
+        default:
+            c->moved_since_bin = true;
+            TRY(launch_pass1(c));
+            if (iter % c->p.fixed_rebuild == 0) rb = 1;
+            break;
+        }
+        if (need_flag) {
+            TRY(sync_flags(c));
+            if (c->h_flags[FLAG_REBUILD]) {
+                rb = 1;
+                TRY(launch_fill_i32(c, c->d_flags + FLAG_REBUILD, 0, 1));
+            }
+        }
         if (rb) {
-            TRY(fm_linklist_deallocateall(c));
-            TRY(fm_sendmols(c));
-            TRY(fm_sort_mols(c));
-            TRY(fm_assign_to_cell(c));
-            TRY(fm_updateborders(c, 1));
-            TRY(fm_assign_to_neighbourlist(c));
+            c->rdisp_valid = false;
+            if (c->p.rebuild_criteria == FM_REBUILD_VMAX) {
+                TRY(launch_fill_f64(c, c->d_rmax, 0.0, 1));
+                TRY(launch_fill_i32(c, c->d_flags + FLAG_REBUILD, 0, 1));
+            }
+            TRY(rebuild_lists(c));
             ++nrb;
         }
-        TRY(fm_compute_forces(c, iter));
-        TRY(fm_move_particles_vv(c, 2));
-        if (iter % c->p.tplot == 0 && macros && nm < n_macros) {
+        TRY(launch_forces(c, iter, true));
+        const bool rec = (iter % c->p.tplot) == 0;
+        const bool ev = c->p.teval > 0 && (iter % c->p.teval) == 0;
+        c->pe_valid = rec;
+        c->vir_valid = rec || ev;
+        c->simtime = c->simtime + c->p.delta_t;
+        if (sync_each_force) TRY(sync_flags(c));
+        if (rec && macros && nm < n_macros) {
             TRY(fm_evaluate_macroscopic_properties(c, &macros[nm]));
             ++nm;
         }
@@ -728,8 +823,16 @@ extern "C" int fm_download_neighbourlist(fm_ctx* c, int32_t* mol_i, int32_t* mol
     std::vector<uint32_t> row(np);
     std::vector<int64_t> off(np + 1, 0);
     for (size_t i = 0; i < np; ++i) off[i + 1] = off[i] + nbn[i];
+    uint32_t* dec = nullptr;
+    FM_HIP(hipMalloc((void**)&dec, (size_t)c->maxnb * cap * sizeof(uint32_t)));
+    struct Free {
+        uint32_t* p;
+        ~Free() { hipFree(p); }
+    } guard{dec};
+    TRY(launch_decode_nlist(c, dec));
+    FM_HIP(hipStreamSynchronize(c->stream));
     for (int k = 0; k < c->maxnb; ++k) {
-        FM_HIP(hipMemcpy(row.data(), c->nbl + (size_t)k * cap, np * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        FM_HIP(hipMemcpy(row.data(), dec + (size_t)k * cap, np * sizeof(uint32_t), hipMemcpyDeviceToHost));
         bool any = false;
         for (size_t i = 0; i < np; ++i) {
             if (k >= nbn[i]) continue;

@@ -1,0 +1,878 @@
+// fm_brick.hip -- LDS-staged cell-brick kernels: neighbour-list build, pair/bond forces with the
+// energy/virial/MOP tallies, and list decoding for inspection.
+//
+// A brick is Bx*By*Bz interior cells; its region is the brick plus one cell on every side
+// ((Bx+2)*(By+2)*(Bz+2) cells).  One workgroup stages the region's molecule positions into LDS,
+// cell by cell in region order, with each cell's periodic image shift applied while staging
+// (x + s*L, one add per shifted component, as the reference's halo copy forms its images).
+// Molecules are contiguous per cell in slot order, so staging reads are coalesced.
+//
+// The neighbour list of molecule i holds 16-bit indices into its brick's LDS region layout.
+// Between rebuilds the cell contents and the slot order do not change, so the force kernel
+// re-stages the identical layout and the indices stay valid.
+#include "fm_internal.h"
+
+#include <limits.h>
+#include <math.h>
+
+#pragma clang fp contract(off)
+
+namespace {
+
+struct St {
+    double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az;
+};
+
+__host__ __device__ inline St state_of(double* base, int cap) {
+    St s;
+    s.x = base;
+    s.y = base + (size_t)cap;
+    s.z = base + 2 * (size_t)cap;
+    s.vx = base + 3 * (size_t)cap;
+    s.vy = base + 4 * (size_t)cap;
+    s.vz = base + 5 * (size_t)cap;
+    s.ax = base + 6 * (size_t)cap;
+    s.ay = base + 7 * (size_t)cap;
+    s.az = base + 8 * (size_t)cap;
+    return s;
+}
+
+__device__ inline void set_err(int* flags, int code, int mol) {
+    if (atomicCAS(&flags[FLAG_ERROR], 0, code) == 0) flags[FLAG_ERROR_MOL] = mol;
+}
+
+// one atomicMax per wavefront instead of one per lane
+__device__ inline void wave_atomic_max(int* addr, int v, bool active) {
+    int m = active ? v : INT_MIN;
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0 && m != INT_MIN && m > __atomic_load_n(addr, __ATOMIC_RELAXED)) atomicMax(addr, m);
+}
+
+// 1/x from v_rcp_f64 refined by two Newton steps (within an ulp of the IEEE quotient; the
+// pair law needs 1e-12 relative, not the reference's last bit)
+__device__ inline double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+__device__ inline double fmaf64(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+__device__ inline double p3(double x) { return x * (x * x); }
+__device__ inline double p4(double x) { double x2 = x * x; return x2 * x2; }
+__device__ inline double p6(double x) { double x3 = x * (x * x); return x3 * x3; }
+__device__ inline double p7(double x) { double x2 = x * x; return (x * x2) * (x2 * x2); }
+
+__device__ inline int shift_code(int sx, int sy, int sz) { return (sx + 1) * 9 + (sy + 1) * 3 + (sz + 1); }
+__device__ inline void decode_shift(int code, int& sx, int& sy, int& sz) {
+    sx = code / 9 - 1;
+    sy = (code / 3) % 3 - 1;
+    sz = code % 3 - 1;
+}
+__device__ inline double img(double r, int s, double L) { return s == 0 ? r : r + (double)s * L; }
+
+// ---- region metadata and staging ---------------------------------------------------------------
+struct Region {
+    // LDS arrays
+    int* off;      // nreg+1: prefix of molecule counts (region layout)
+    int* gc;       // nreg: global cell
+    int* sh;       // nreg: shift code of the region cell
+    int* ic_off;   // nic+1: prefix over interior (i-) cells
+    int* ic_rc;    // nic: region cell of each i-cell
+    double *lx, *ly, *lz;
+    int* lslot;    // optional: slot of each staged molecule
+    int nic;
+    int total;     // staged molecules
+    int ni;        // i-molecules
+};
+
+struct BG {  // brick geometry
+    int nc[3], B[3], nb[3], R[3], nreg;
+    double L[3];
+};
+
+// Carve the LDS buffer (8-byte aligned pieces first).
+__device__ inline void carve(Region& r, char* smem, const BG& g, int maxreg, bool with_slot) {
+    double* d = (double*)smem;
+    r.lx = d;
+    r.ly = d + maxreg;
+    r.lz = d + 2 * maxreg;
+    int* ip = (int*)(d + 3 * maxreg);
+    r.lslot = nullptr;
+    if (with_slot) {
+        r.lslot = ip;
+        ip += maxreg;
+    }
+    r.off = ip;
+    ip += g.nreg + 1;
+    r.gc = ip;
+    ip += g.nreg;
+    r.sh = ip;
+    ip += g.nreg;
+    const int nbmax = g.B[0] * g.B[1] * g.B[2];
+    r.ic_off = ip;
+    ip += nbmax + 1;
+    r.ic_rc = ip;
+}
+
+__host__ __device__ inline size_t region_lds_bytes(const BG& g, int maxreg, bool with_slot) {
+    const int nbmax = g.B[0] * g.B[1] * g.B[2];
+    return (size_t)maxreg * (3 * sizeof(double) + (with_slot ? sizeof(int) : 0)) +
+           (size_t)(3 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int) + 16;
+}
+
+// exclusive scan of cnt[0..n) into off[0..n] by the first wavefront
+__device__ inline void wave0_scan(int* off, int n) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int carry = 0;
+        for (int base = 0; base < n; base += 64) {
+            int q = base + lane;
+            int v = q < n ? off[q] : 0;
+            int x = v;
+            for (int d = 1; d < 64; d <<= 1) {
+                int y = __shfl_up(x, d, 64);
+                if (lane >= d) x += y;
+            }
+            if (q < n) off[q] = carry + x - v;
+            carry += __shfl(x, 63, 64);
+        }
+        if (lane == 0) off[n] = carry;
+    }
+}
+
+// Region metadata + interior-cell list + staged positions.  Ends with __syncthreads().
+__device__ void stage_region(Region& r, const BG& g, int brick, const int* start, const St& s, bool stage_pos,
+                             const int* mono_src, int cap, int* lchain, int* lsub) {
+    const int bx = brick % g.nb[0], by = (brick / g.nb[0]) % g.nb[1], bz = brick / (g.nb[0] * g.nb[1]);
+    const int org[3] = {bx * g.B[0], by * g.B[1], bz * g.B[2]};
+    for (int t = threadIdx.x; t < g.nreg; t += blockDim.x) {
+        const int rr[3] = {t % g.R[0], (t / g.R[0]) % g.R[1], t / (g.R[0] * g.R[1])};
+        int gg[3], ss[3];
+        for (int d = 0; d < 3; ++d) {
+            int v = org[d] + rr[d] - 1, sv = 0;
+            if (v < 0) {
+                v += g.nc[d];
+                sv = -1;
+            } else if (v >= g.nc[d]) {
+                v -= g.nc[d];
+                sv = 1;
+            }
+            gg[d] = v;
+            ss[d] = sv;
+        }
+        const int c = gg[0] + g.nc[0] * (gg[1] + g.nc[1] * gg[2]);
+        r.gc[t] = c;
+        r.sh[t] = shift_code(ss[0], ss[1], ss[2]);
+        r.off[t] = start[c + 1] - start[c];
+    }
+    __syncthreads();
+    wave0_scan(r.off, g.nreg);
+    if (threadIdx.x == 0) {  // interior cells of this brick (partial bricks at the far faces)
+        int nic = 0;
+        const int ex = min(g.B[0], g.nc[0] - org[0]), ey = min(g.B[1], g.nc[1] - org[1]),
+                  ez = min(g.B[2], g.nc[2] - org[2]);
+        int acc = 0;
+        for (int z = 1; z <= ez; ++z)
+            for (int y = 1; y <= ey; ++y)
+                for (int x = 1; x <= ex; ++x) {
+                    const int rc = x + g.R[0] * (y + g.R[1] * z);
+                    r.ic_rc[nic] = rc;
+                    r.ic_off[nic] = acc;
+                    acc += r.off[rc + 1] - r.off[rc];
+                    ++nic;
+                }
+        r.ic_off[nic] = acc;
+    }
+    __syncthreads();
+    r.total = r.off[g.nreg];
+    r.nic = min(g.B[0], g.nc[0] - org[0]) * min(g.B[1], g.nc[1] - org[1]) * min(g.B[2], g.nc[2] - org[2]);
+    r.ni = r.ic_off[r.nic];
+    if (stage_pos || r.lslot || lchain) {
+        for (int q = threadIdx.x; q < r.total; q += blockDim.x) {
+            int lo = 0, hi = g.nreg - 1;  // last rc with off[rc] <= q
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (r.off[mid] <= q) lo = mid;
+                else hi = mid - 1;
+            }
+            const int rc = lo;
+            const int slot = start[r.gc[rc]] + (q - r.off[rc]);
+            if (stage_pos) {
+                int sx, sy, sz;
+                decode_shift(r.sh[rc], sx, sy, sz);
+                r.lx[q] = img(s.x[slot], sx, g.L[0]);
+                r.ly[q] = img(s.y[slot], sy, g.L[1]);
+                r.lz[q] = img(s.z[slot], sz, g.L[2]);
+            }
+            if (r.lslot) r.lslot[q] = slot;
+            if (lchain) {
+                lchain[q] = mono_src[slot];
+                lsub[q] = mono_src[slot + (size_t)cap];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// i-molecule q of the brick -> (region cell, LDS index, slot)
+__device__ inline void imol(const Region& r, const int* start, int q, int& rc, int& li, int& slot) {
+    int lo = 0, hi = r.nic - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (r.ic_off[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    rc = r.ic_rc[lo];
+    const int k = q - r.ic_off[lo];
+    li = r.off[rc] + k;
+    slot = start[r.gc[rc]] + k;
+}
+
+__device__ inline int region_cell_of(const Region& r, int nreg, int q) {
+    int lo = 0, hi = nreg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (r.off[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---- neighbour list: src/linklist.f90:660-804 (set semantics), bonds 2542-2584 -----------------
+struct NlbArgs {
+    St s;
+    BG g;
+    const int* start;
+    const int* ext;
+    const int* mono;  // slot order: chainID, subchainID, funcy, bflag[4] (stride cap) or null
+    uint64_t* nbl;   // entries 4c..4c+3 of slot i in word nbl[c*cap + i] (16 bits each)
+    int* nbn;
+    uint32_t* bond;
+    int* bondn;
+    int* flags;
+    int cap, maxnb, maxreg;
+    double rn2;
+    int xsorted;     // cells are sorted by x (positions unchanged since binning)
+    double xwin;     // window half-width: above sqrt(rn2), so no pair inside rn2 is skipped
+};
+
+__device__ inline int get_bondflag(const int* mono, int cap, int i, int scID) {
+    // get_bondflag, src/modules.f90:597-610 (intbits = 31)
+    int group = (int)ceilf((float)scID / 31.0f);
+    int expo = scID % 31 - 1;
+    if (expo == -1) expo = 30;
+    if (group < 1 || group > 4) return 0;
+    int w = mono[i + (size_t)(2 + group) * cap];
+    return (w >> expo) & 1;
+}
+
+template <bool POLY>
+__global__ __launch_bounds__(FM_BLOCK) void k_nlist_brick(NlbArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Region r;
+    carve(r, smem, a.g, a.maxreg, POLY);
+    int* lchain = nullptr;
+    int* lsub = nullptr;
+    if (POLY) {  // chain ids of the staged molecules after the brick tables
+        lchain = r.ic_rc + a.g.B[0] * a.g.B[1] * a.g.B[2];
+        lsub = lchain + a.maxreg;
+    }
+    stage_region(r, a.g, blockIdx.x, a.start, a.s, true, a.mono, a.cap, lchain, lsub);
+    if (r.total > a.maxreg) return;  // cannot happen: maxreg is the measured maximum
+    const size_t cap = a.cap;
+    const int R0 = a.g.R[0], R1 = a.g.R[1];
+    const int nrounds = (r.ni + blockDim.x - 1) / blockDim.x;
+    for (int round = 0; round < nrounds; ++round) {
+        const int q = round * blockDim.x + threadIdx.x;
+        const bool active = q < r.ni;
+        int cnt = 0;
+        if (active) {
+            int rc, li, slot;
+            imol(r, a.start, q, rc, li, slot);
+            const int rx = rc % R0, ry = (rc / R0) % R1, rz = rc / (R0 * R1);
+            const double xi = r.lx[li], yi = r.ly[li], zi = r.lz[li];
+            int nb = 0, chain = 0, funcy = 0;
+            if (POLY) {
+                chain = lchain[li];
+                funcy = a.mono[slot + 2 * cap];
+            }
+            uint64_t word = 0;
+            // the 27 neighbour cells as 9 runs of three x-consecutive region cells, which are
+            // contiguous in the LDS layout
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const int c0 = (rx - 1) + R0 * ((ry + dy) + R1 * (rz + dz));
+                    int jb = r.off[c0], je = r.off[c0 + 3];
+                    if (!POLY && a.xsorted) {
+                        // the run is sorted by x: keep only xi - xwin <= x <= xi + xwin
+                        const double xlo = xi - a.xwin, xhi = xi + a.xwin;
+                        int lo = jb, hi = je;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (r.lx[mid] < xlo) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        jb = lo;
+                        lo = jb;
+                        hi = je;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (r.lx[mid] <= xhi) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        je = lo;
+                    }
+                    for (int j = jb; j < je; ++j) {
+                        const double ddx = xi - r.lx[j], ddy = yi - r.ly[j], ddz = zi - r.lz[j];
+                        const double r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                        const bool hit = (r2 < a.rn2) && (j != li);
+                        const uint32_t sh = (uint32_t)(cnt & 3) << 4;
+                        word |= hit ? ((uint64_t)(uint32_t)j << sh) : 0ull;
+                        if (hit && (cnt & 3) == 3) {
+                            if (cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word;
+                            word = 0;
+                        }
+                        cnt += hit ? 1 : 0;
+                        if (POLY && j != li && lchain[j] == chain && get_bondflag(a.mono, a.cap, slot, lsub[j])) {
+                            const int c2 = region_cell_of(r, a.g.nreg, j);
+                            if (nb < funcy && nb < FM_MAX_FUNCY)
+                                a.bond[(size_t)nb * cap + slot] = ((uint32_t)r.lslot[j] << 5) | (uint32_t)r.sh[c2];
+                            ++nb;
+                        }
+                    }
+                }
+            if ((cnt & 3) != 0 && cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word;
+            a.nbn[slot] = cnt < a.maxnb ? cnt : a.maxnb;
+            if (cnt > a.maxnb) set_err(a.flags, FM_EOVERFLOW, a.ext[slot]);
+            if (POLY) {
+                a.bondn[slot] = nb < funcy ? nb : funcy;
+                if (nb > funcy) set_err(a.flags, FM_EBOND, a.ext[slot]);
+            }
+        }
+        wave_atomic_max(&a.flags[FLAG_MAX_NB], cnt, active);
+    }
+}
+
+// ---- forces: src/simulation_compute_forces.f90:628-726 (pairs), 731-789 (bonds) ----------------
+struct FbArgs {
+    St s;
+    BG g;
+    const int* start;
+    const uint64_t* nbl;
+    const int* nbn;
+    const uint32_t* bond;
+    const int* bondn;
+    const int* ext;
+    const int* moltype;    // null unless Mie
+    const int* tag;        // null unless tag_move
+    double* teth;          // rtether xyz, slidev xyz
+    double* pe_lj;
+    double* pe_poly;
+    double* vir;
+    double* rf;            // 9*cap or null
+    double* pxy;           // 3*nplanes or null
+    const double* planes;
+    int* flags;
+    int cap, maxreg;
+    int rec, ev;
+    int kick;              // fused simulation_move_particles_vv(2): v += (0.5*dt)*a
+    double hdt;
+    double rc2, potshift;
+    double k_c, R0, R02, dt;
+    double teth_k2, teth_k4, teth_k6;
+    double hx, hz;
+    int nplanes;
+    double hp, spacing, shift;
+    const double* mie;     // [7][FM_MAX_TYPES^2] + mass[FM_MAX_TYPES]
+};
+
+// forward cell offsets of the reference's half-list scan (src/linklist.f90:686-688)
+__device__ inline bool is_forward(int dx, int dy, int dz) {
+    if (dz == 1) return true;
+    if (dz == -1) return false;
+    if (dy == 1) return true;
+    if (dy == -1) return dx == 1;
+    return dx == 1;
+}
+
+__device__ inline double dp_heaviside(double x) { return ceil(copysign(0.5, x)); }
+
+// pressure_tensor_forces_MOP (src/simulation_record.f90:7068-7112) for one list entry
+__device__ void mop_entry(const FbArgs& a, const double ri[3], const double rj[3], const double rij[3],
+                          double acc, double* spxy) {
+    int pi = (int)llround((ri[1] + a.hp - a.shift) / a.spacing) + 1;
+    pi = pi < 1 ? 1 : (pi > a.nplanes ? a.nplanes : pi);
+    int pj = (int)llround((rj[1] + a.hp - a.shift) / a.spacing) + 1;
+    pj = pj < 1 ? 1 : (pj > a.nplanes ? a.nplanes : pj);
+    const int step = (pj - pi) >= 0 ? 1 : -1;
+    for (int n = pi;; n += step) {
+        const double plane = a.planes[n - 1];
+        const double pyb0 = ri[0] + (rij[0] / rij[1]) * (plane - ri[1]);
+        const double pyb2 = ri[2] + (rij[2] / rij[1]) * (plane - ri[1]);
+        const double fx = copysign(1.0, ri[1] - plane) - copysign(1.0, rj[1] - plane);
+        const double hxx = dp_heaviside(a.hx - pyb0) - dp_heaviside(-a.hx - pyb0);
+        const double hzz = dp_heaviside(a.hz - pyb2) - dp_heaviside(-a.hz - pyb2);
+        if (fx != 0.0 && hxx != 0.0 && hzz != 0.0)
+            for (int d = 0; d < 3; ++d) atomicAdd(&spxy[3 * (n - 1) + d], acc * rij[d] * fx * hxx * hzz);
+        if (n == pj) break;
+    }
+}
+
+// Does the reference visit this entry in its MOP tally?  Shifted entries: always (one per end).
+// Unshifted pairs: once, from the molecule whose cell scans the other's as a forward cell, or in
+// the same cell from the higher molecule number (head-inserted list order).
+__device__ inline bool mop_owner(const Region& r, const BG& g, int rci, int rcj, int exti, int extj) {
+    const int shj = r.sh[rcj];
+    if (shj != 13 || r.sh[rci] != 13) return true;
+    if (rcj == rci) return exti > extj;
+    const int dx = rcj % g.R[0] - rci % g.R[0];
+    const int dy = (rcj / g.R[0]) % g.R[1] - (rci / g.R[0]) % g.R[1];
+    const int dz = rcj / (g.R[0] * g.R[1]) - rci / (g.R[0] * g.R[1]);
+    return is_forward(dx, dy, dz);
+}
+
+template <bool MIE, bool RECORD, bool EXTRA>
+__global__ __launch_bounds__(FM_BLOCK) void k_forces_brick(FbArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Region r;
+    const bool mop = RECORD && a.pxy != nullptr;
+    carve(r, smem, a.g, a.maxreg, mop || MIE);
+    double* spxy = nullptr;
+    if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
+    stage_region(r, a.g, blockIdx.x, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr);
+    if (mop) {
+        for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
+        __syncthreads();
+    }
+    const size_t cap = a.cap;
+    const int T2 = FM_MAX_TYPES * FM_MAX_TYPES;
+    for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
+        int rci, li, i;
+        imol(r, a.start, q, rci, li, i);
+        const double xi = r.lx[li], yi = r.ly[li], zi = r.lz[li];
+        const int n = a.nbn[i];
+        double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, vir = 0.0;
+        double rf[9];
+        const bool do_rf = RECORD && a.rf != nullptr;
+        if (do_rf)
+            for (int k = 0; k < 9; ++k) rf[k] = 0.0;
+        int ti = 0;
+        double mi = 1.0;
+        if (MIE) {
+            ti = a.moltype[i];
+            mi = a.mie[7 * T2 + ti];
+        }
+        const int exti = mop ? a.ext[i] : 0;
+        const uint64_t* row = a.nbl + i;
+        uint64_t word = n > 0 ? row[0] : 0;
+        for (int k = 0; k < n; ++k) {
+            if ((k & 3) == 0 && k > 0) word = row[(size_t)(k >> 2) * cap];
+            const int j = (int)((word >> (16 * (k & 3))) & 0xffffu);
+            const double rx = xi - r.lx[j], ry = yi - r.ly[j], rz = zi - r.lz[j];
+            const double r2 = (rx * rx + ry * ry) + rz * rz;
+            double rc2 = a.rc2;
+            int t = 0;
+            if (MIE) {
+                t = ti * FM_MAX_TYPES + a.moltype[r.lslot[j]];
+                const double c = a.mie[6 * T2 + t];
+                if (c > 0.0) rc2 = c;
+            }
+            if (!(r2 < rc2)) continue;
+            const double inv = recip(r2);
+            double acc, e = 0.0;
+            if (!MIE) {
+                acc = 48.0 * (p7(inv) - 0.5 * p4(inv));
+                if (RECORD) e = 4.0 * (p6(inv) - p3(inv)) - a.potshift;
+            } else {
+                const double eps = a.mie[t], lr = a.mie[T2 + t], la = a.mie[2 * T2 + t], C = a.mie[3 * T2 + t],
+                             al = a.mie[4 * T2 + t];
+                acc = C * eps * (lr * pow(inv, 0.5 * lr + 1) - al * la * pow(inv, 0.5 * la + 1));
+                if (RECORD) e = C * eps * (pow(inv, 0.5 * lr) - al * pow(inv, 0.5 * la)) - a.mie[5 * T2 + t];
+            }
+            if (MIE) {
+                fx = fx + acc * rx / mi;
+                fy = fy + acc * ry / mi;
+                fz = fz + acc * rz / mi;
+            } else {  // LJ mass is 1
+                fx = fmaf64(acc, rx, fx);
+                fy = fmaf64(acc, ry, fy);
+                fz = fmaf64(acc, rz, fz);
+            }
+            if (RECORD || a.ev) vir = vir + acc * r2;
+            if (RECORD) {
+                pe = pe + e;
+                if (do_rf) {
+                    const double rr[3] = {rx, ry, rz};
+                    for (int p = 0; p < 3; ++p)
+                        for (int qq = 0; qq < 3; ++qq) rf[3 * p + qq] = rf[3 * p + qq] + acc * rr[p] * rr[qq];
+                }
+                if (mop) {
+                    const int rcj = region_cell_of(r, a.g.nreg, j);
+                    if (mop_owner(r, a.g, rci, rcj, exti, a.ext[r.lslot[j]])) {
+                        const double ri[3] = {xi, yi, zi}, rj[3] = {r.lx[j], r.ly[j], r.lz[j]}, rij[3] = {rx, ry, rz};
+                        mop_entry(a, ri, rj, rij, acc, spxy);
+                    }
+                }
+            }
+        }
+        if (EXTRA && a.bondn) {  // FENE, simulation_compute_forces_poly
+            const int nb = a.bondn[i];
+            double pp = 0.0;
+            for (int b = 0; b < nb; ++b) {
+                const uint32_t w = a.bond[(size_t)b * cap + i];
+                const int j = (int)(w >> 5);
+                int sx, sy, sz;
+                decode_shift((int)(w & 31u), sx, sy, sz);
+                const double rx = xi - img(a.s.x[j], sx, a.g.L[0]), ry = yi - img(a.s.y[j], sy, a.g.L[1]),
+                             rz = zi - img(a.s.z[j], sz, a.g.L[2]);
+                const double r2 = (rx * rx + ry * ry) + rz * rz;
+                if (r2 >= a.R02) {
+                    set_err(a.flags, FM_EBOND, a.ext[i]);
+                    continue;
+                }
+                const double acc = -a.k_c / (1.0 - (r2 / a.R02));
+                fx = fx + acc * rx / mi;
+                fy = fy + acc * ry / mi;
+                fz = fz + acc * rz / mi;
+                if (RECORD || a.ev) vir = vir + acc * r2;
+                if (RECORD) {
+                    pp = pp + 0.5 * a.k_c * a.R0 * a.R0 * log(1.0 - (r2 / a.R02));
+                    if (do_rf) {
+                        const double rr[3] = {rx, ry, rz};
+                        for (int p = 0; p < 3; ++p)
+                            for (int qq = 0; qq < 3; ++qq) rf[3 * p + qq] = rf[3 * p + qq] + acc * rr[p] * rr[qq];
+                    }
+                }
+            }
+            if (RECORD) a.pe_poly[i] = pp;
+        }
+        if (EXTRA && a.tag) {  // tether_force, src/molecular_properties.f90:833-915
+            const int tg = a.tag[i];
+            if (tg == FM_TAG_TETH || tg == FM_TAG_TETH_SLIDE) {
+                double* T = a.teth;
+                const double rio[3] = {xi - T[i], yi - T[i + cap], zi - T[i + 2 * cap]};
+                const double mag = sqrt((rio[0] * rio[0] + rio[1] * rio[1]) + rio[2] * rio[2]);
+                const double am = -2.0 * a.teth_k2 * mag - 4.0 * a.teth_k4 * (mag * mag) -
+                                  6.0 * a.teth_k6 * pow(mag, 4.0);
+                fx = fx + am * rio[0];
+                fy = fy + am * rio[1];
+                fz = fz + am * rio[2];
+                for (int d = 0; d < 3; ++d) T[i + d * cap] = T[i + d * cap] + T[i + (3 + d) * cap] * a.dt;
+            }
+        }
+        a.s.ax[i] = fx;
+        a.s.ay[i] = fy;
+        a.s.az[i] = fz;
+        if (a.kick) {
+            a.s.vx[i] = a.s.vx[i] + a.hdt * fx;
+            a.s.vy[i] = a.s.vy[i] + a.hdt * fy;
+            a.s.vz[i] = a.s.vz[i] + a.hdt * fz;
+        }
+        if (RECORD || a.ev) a.vir[i] = vir;
+        if (RECORD) {
+            a.pe_lj[i] = pe;
+            if (do_rf)
+                for (int k = 0; k < 9; ++k) a.rf[i + k * cap] = rf[k];
+        }
+    }
+    if (mop) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x)
+            if (spxy[t] != 0.0) atomicAdd(&a.pxy[t], spxy[t]);
+    }
+}
+
+// Mixed precision: image coordinates and differences in FP64, the pair law and the per-molecule
+// force sum in FP32; energy, virial and the tallies accumulated in FP64.
+template <bool RECORD, bool EXTRA>
+__global__ __launch_bounds__(FM_BLOCK) void k_forces_brick_mixed(FbArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Region r;
+    const bool mop = RECORD && a.pxy != nullptr;
+    carve(r, smem, a.g, a.maxreg, mop);
+    double* spxy = nullptr;
+    if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
+    stage_region(r, a.g, blockIdx.x, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr);
+    if (mop) {
+        for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
+        __syncthreads();
+    }
+    const size_t cap = a.cap;
+    const float rc2 = (float)a.rc2;
+    for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
+        int rci, li, i;
+        imol(r, a.start, q, rci, li, i);
+        const double xi = r.lx[li], yi = r.ly[li], zi = r.lz[li];
+        const int n = a.nbn[i];
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        double pe = 0.0, vir = 0.0;
+        const int exti = mop ? a.ext[i] : 0;
+        const uint64_t* row = a.nbl + i;
+        uint64_t word = n > 0 ? row[0] : 0;
+        for (int k = 0; k < n; ++k) {
+            if ((k & 3) == 0 && k > 0) word = row[(size_t)(k >> 2) * cap];
+            const int j = (int)((word >> (16 * (k & 3))) & 0xffffu);
+            const double rxd = xi - r.lx[j], ryd = yi - r.ly[j], rzd = zi - r.lz[j];
+            const float rx = (float)rxd, ry = (float)ryd, rz = (float)rzd;
+            float acc, inv;
+            {
+#pragma clang fp contract(fast)
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                if (!(r2 < rc2)) continue;
+                inv = __builtin_amdgcn_rcpf(r2);
+                const float i2 = inv * inv, i4 = i2 * i2, i3 = i2 * inv;
+                acc = 48.f * (i4 * i3 - 0.5f * i4);
+                fx += acc * rx;
+                fy += acc * ry;
+                fz += acc * rz;
+            }
+            if (RECORD || a.ev) vir = vir + (double)acc * ((rxd * rxd + ryd * ryd) + rzd * rzd);
+            if (RECORD) {
+                const double id = (double)inv;
+                pe = pe + (4.0 * (p6(id) - p3(id)) - a.potshift);
+                if (mop) {
+                    const int rcj = region_cell_of(r, a.g.nreg, j);
+                    if (mop_owner(r, a.g, rci, rcj, exti, a.ext[r.lslot[j]])) {
+                        const double ri[3] = {xi, yi, zi}, rj[3] = {r.lx[j], r.ly[j], r.lz[j]},
+                                     rij[3] = {rxd, ryd, rzd};
+                        mop_entry(a, ri, rj, rij, (double)acc, spxy);
+                    }
+                }
+            }
+        }
+        double dfx = fx, dfy = fy, dfz = fz;
+        if (EXTRA && a.tag) {
+            const int tg = a.tag[i];
+            if (tg == FM_TAG_TETH || tg == FM_TAG_TETH_SLIDE) {
+                double* T = a.teth;
+                const double rio[3] = {xi - T[i], yi - T[i + cap], zi - T[i + 2 * cap]};
+                const double mag = sqrt((rio[0] * rio[0] + rio[1] * rio[1]) + rio[2] * rio[2]);
+                const double am = -2.0 * a.teth_k2 * mag - 4.0 * a.teth_k4 * (mag * mag) -
+                                  6.0 * a.teth_k6 * pow(mag, 4.0);
+                dfx = dfx + am * rio[0];
+                dfy = dfy + am * rio[1];
+                dfz = dfz + am * rio[2];
+                for (int d = 0; d < 3; ++d) T[i + d * cap] = T[i + d * cap] + T[i + (3 + d) * cap] * a.dt;
+            }
+        }
+        a.s.ax[i] = dfx;
+        a.s.ay[i] = dfy;
+        a.s.az[i] = dfz;
+        if (a.kick) {
+            a.s.vx[i] = a.s.vx[i] + a.hdt * dfx;
+            a.s.vy[i] = a.s.vy[i] + a.hdt * dfy;
+            a.s.vz[i] = a.s.vz[i] + a.hdt * dfz;
+        }
+        if (RECORD || a.ev) a.vir[i] = vir;
+        if (RECORD) a.pe_lj[i] = pe;
+    }
+    if (mop) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x)
+            if (spxy[t] != 0.0) atomicAdd(&a.pxy[t], spxy[t]);
+    }
+}
+
+// ---- decode the LDS-indexed list into (slot << 5 | shift code) for inspection ------------------
+__global__ __launch_bounds__(FM_BLOCK) void k_decode_brick(St s, BG g, const int* start, const uint64_t* nbl,
+                                                           const int* nbn, int cap, int maxreg, uint32_t* out) {
+    extern __shared__ __align__(16) char smem[];
+    Region r;
+    carve(r, smem, g, maxreg, true);
+    stage_region(r, g, blockIdx.x, start, s, false, nullptr, cap, nullptr, nullptr);
+    for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
+        int rci, li, i;
+        imol(r, start, q, rci, li, i);
+        const int n = nbn[i];
+        for (int k = 0; k < n; ++k) {
+            const int j = (int)((nbl[(size_t)(k >> 2) * cap + i] >> (16 * (k & 3))) & 0xffffu);
+            const int rcj = region_cell_of(r, g.nreg, j);
+            out[(size_t)k * cap + i] = ((uint32_t)r.lslot[j] << 5) | (uint32_t)r.sh[rcj];
+        }
+    }
+}
+
+// largest staged region (LDS sizing) and largest brick interior
+__global__ __launch_bounds__(FM_BLOCK) void k_region_max(BG g, const int* start, int nbrick, int* out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbrick) return;
+    const int bx = b % g.nb[0], by = (b / g.nb[0]) % g.nb[1], bz = b / (g.nb[0] * g.nb[1]);
+    const int org[3] = {bx * g.B[0], by * g.B[1], bz * g.B[2]};
+    int tot = 0;
+    for (int rz = 0; rz < g.R[2]; ++rz)
+        for (int ry = 0; ry < g.R[1]; ++ry)
+            for (int rx = 0; rx < g.R[0]; ++rx) {
+                int v[3] = {org[0] + rx - 1, org[1] + ry - 1, org[2] + rz - 1};
+                for (int d = 0; d < 3; ++d) {
+                    if (v[d] < 0) v[d] += g.nc[d];
+                    else if (v[d] >= g.nc[d]) v[d] -= g.nc[d];
+                }
+                const int c = v[0] + g.nc[0] * (v[1] + g.nc[1] * v[2]);
+                tot += start[c + 1] - start[c];
+            }
+    if (tot > __atomic_load_n(out, __ATOMIC_RELAXED)) atomicMax(out, tot);
+}
+
+inline int nblk(size_t n) { return (int)((n + FM_BLOCK - 1) / FM_BLOCK); }
+
+BG make_bg(const fm_ctx* c) {
+    BG g;
+    for (int d = 0; d < 3; ++d) {
+        g.nc[d] = c->nc[d];
+        g.B[d] = c->brick[d];
+        g.nb[d] = (c->nc[d] + c->brick[d] - 1) / c->brick[d];
+        g.R[d] = c->brick[d] + 2;
+        g.L[d] = c->p.domain[d];
+    }
+    g.nreg = g.R[0] * g.R[1] * g.R[2];
+    return g;
+}
+
+}  // namespace
+
+#define LAUNCH_CHECK()                                                                       \
+    do {                                                                                     \
+        hipError_t _e = hipGetLastError();                                                   \
+        if (_e != hipSuccess) return fm_set_error(FM_EHIP, "kernel launch: %s (%s:%d)",       \
+                                                  hipGetErrorString(_e), __FILE__, __LINE__); \
+    } while (0)
+
+int brick_count(const fm_ctx* c) {
+    BG g = make_bg(c);
+    return g.nb[0] * g.nb[1] * g.nb[2];
+}
+
+size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind) {
+    BG g = make_bg(c);
+    // kind 0: list build (+ slots and chain ids with bonds); 1: forces; 2: forces with MOP
+    if (kind == 0) {
+        bool poly = c->p.potential_flag == 1;
+        return region_lds_bytes(g, maxreg, poly) + (poly ? 2 * (size_t)maxreg * sizeof(int) : 0);
+    }
+    if (kind == 1) return region_lds_bytes(g, maxreg, c->p.mie_potential == 1);
+    return region_lds_bytes(g, maxreg, true) + 3 * (size_t)c->p.nplanes * sizeof(double);
+}
+
+// measure the largest region (after binning); result in d_flags[FLAG_MAX_REGION]
+int launch_region_max(fm_ctx* c) {
+    BG g = make_bg(c);
+    int nbrick = g.nb[0] * g.nb[1] * g.nb[2];
+    k_region_max<<<nblk(nbrick), FM_BLOCK, 0, c->stream>>>(g, c->cell_start, nbrick, c->d_flags + FLAG_MAX_REGION);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_nlist(fm_ctx* c) {
+    if (c->np == 0) return 0;
+    BG g = make_bg(c);
+    NlbArgs a;
+    a.s = state_of(c->x(), c->cap);
+    a.g = g;
+    a.start = c->cell_start;
+    a.ext = c->ext[c->cur];
+    a.mono = (c->p.potential_flag == 1 && c->has_mono) ? c->mono[c->cur] : nullptr;
+    a.nbl = c->nbl;
+    a.nbn = c->nbn;
+    a.bond = c->bond;
+    a.bondn = c->bondn;
+    a.flags = c->d_flags;
+    a.cap = c->cap;
+    a.maxnb = c->maxnb;
+    a.maxreg = c->maxreg;
+    a.rn2 = c->rn2;
+    a.xsorted = c->moved_since_bin ? 0 : 1;
+    a.xwin = sqrt(c->rn2) * (1.0 + 1e-9) + 1e-12;
+    const size_t lds = brick_lds_bytes(c, c->maxreg, 0);
+    ktimer_begin(c, KT_NLIST);
+    if (a.mono) k_nlist_brick<true><<<brick_count(c), FM_BLOCK, lds, c->stream>>>(a);
+    else k_nlist_brick<false><<<brick_count(c), FM_BLOCK, lds, c->stream>>>(a);
+    ktimer_end(c, KT_NLIST);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_forces(fm_ctx* c, int iter, bool kick) {
+    if (c->np == 0) return 0;
+    BG g = make_bg(c);
+    FbArgs a;
+    const int cur = c->cur;
+    a.s = state_of(c->x(), c->cap);
+    a.g = g;
+    a.start = c->cell_start;
+    a.nbl = c->nbl;
+    a.nbn = c->nbn;
+    const bool poly = c->p.potential_flag == 1 && c->has_mono;
+    a.bond = poly ? c->bond : nullptr;
+    a.bondn = poly ? c->bondn : nullptr;
+    a.ext = c->ext[cur];
+    const bool mie = c->p.mie_potential == 1;
+    a.moltype = mie ? c->moltype[cur] : nullptr;
+    const bool tm = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && c->has_tags;
+    a.tag = tm ? c->tag[cur] : nullptr;
+    a.teth = tm ? c->teth[cur] : nullptr;
+    a.pe_lj = c->pe_lj;
+    a.pe_poly = c->pe_poly;
+    a.vir = c->vir;
+    a.flags = c->d_flags;
+    a.cap = c->cap;
+    a.maxreg = c->maxreg;
+    const int tplot = c->p.tplot, teval = c->p.teval;
+    a.rec = (iter % tplot) == 0;
+    a.ev = teval > 0 && (iter % teval) == 0;
+    a.rf = (a.rec && c->p.pressure_outflag == 1) ? c->rf : nullptr;
+    const bool mop = a.rec && c->p.vflux_outflag >= 1 && c->p.vflux_outflag <= 3 && c->pxy;
+    a.pxy = mop ? c->pxy : nullptr;
+    a.planes = c->planes;
+    a.rc2 = c->rc2;
+    a.potshift = c->potshift;
+    a.k_c = c->p.k_c;
+    a.R0 = c->p.R_0;
+    a.R02 = c->p.R_0 * c->p.R_0;
+    a.dt = c->p.delta_t;
+    a.teth_k2 = c->p.teth_k2;
+    a.teth_k4 = c->p.teth_k4;
+    a.teth_k6 = c->p.teth_k6;
+    a.hx = c->half[0];
+    a.hz = c->half[2];
+    a.nplanes = c->p.nplanes;
+    const int pn = c->p.vflux_outflag - 1;
+    a.hp = (pn >= 0 && pn < 3) ? c->half[pn] : 0.0;
+    a.spacing = c->p.planespacing;
+    a.shift = c->mop_shift;
+    a.mie = mie ? c->d_mie : nullptr;
+    a.kick = kick ? 1 : 0;
+    a.hdt = 0.5 * c->p.delta_t;
+    const size_t lds = brick_lds_bytes(c, c->maxreg, mop ? 2 : 1);
+    ktimer_begin(c, KT_FORCES);
+    dim3 grid(brick_count(c)), blk(FM_BLOCK);
+    const bool extra = poly || tm;
+#define FB(M, R, E) k_forces_brick<M, R, E><<<grid, blk, lds, c->stream>>>(a)
+#define FM(R, E) k_forces_brick_mixed<R, E><<<grid, blk, lds, c->stream>>>(a)
+    if (c->p.precision == FM_PREC_MIXED && !mie && !poly) {
+        if (extra) { if (a.rec) FM(true, true); else FM(false, true); }
+        else { if (a.rec) FM(true, false); else FM(false, false); }
+    } else if (mie) {
+        if (extra) { if (a.rec) FB(true, true, true); else FB(true, false, true); }
+        else { if (a.rec) FB(true, true, false); else FB(true, false, false); }
+    } else {
+        if (extra) { if (a.rec) FB(false, true, true); else FB(false, false, true); }
+        else { if (a.rec) FB(false, true, false); else FB(false, false, false); }
+    }
+#undef FB
+#undef FM
+    ktimer_end(c, KT_FORCES);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_decode_nlist(fm_ctx* c, uint32_t* out) {
+    if (c->np == 0) return 0;
+    BG g = make_bg(c);
+    const size_t lds = brick_lds_bytes(c, c->maxreg, 2) ;
+    k_decode_brick<<<brick_count(c), FM_BLOCK, lds, c->stream>>>(state_of(c->x(), c->cap), g, c->cell_start, c->nbl,
+                                                                 c->nbn, c->cap, c->maxreg, out);
+    LAUNCH_CHECK();
+    return 0;
+}

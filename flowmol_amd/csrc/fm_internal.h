@@ -30,6 +30,7 @@ enum {
     FLAG_MAX_NB = 4,      // largest neighbour count at the last build
     FLAG_NOUT = 5,        // molecules outside the box at sendmols
     FLAG_EDGE = 6,        // molecules exactly on the lower domain face (clamped into cell 1)
+    FLAG_MAX_REGION = 7,  // largest LDS region (brick + one-cell halo) in molecules
     FLAG_COUNT = 8
 };
 
@@ -65,9 +66,15 @@ struct fm_ctx {
     int* cell_cursor = nullptr;           // ncell
     int* perm = nullptr;                  // cap
 
-    // neighbour list
+    // LDS bricks
+    int brick[3] = {1, 1, 1};
+    int maxreg = 0;                       // largest staged region (molecules) since the last binning
+    size_t lds_limit = 65536;             // dynamic LDS per workgroup
+
+    // neighbour list: 16-bit indices into the brick's LDS region layout, four per 64-bit word:
+    // entries 4c..4c+3 of slot i in nbl[c*cap + i]
     int maxnb = 0;
-    uint32_t* nbl = nullptr;
+    uint64_t* nbl = nullptr;
     int* nbn = nullptr;
     uint32_t* bond = nullptr;             // FM_MAX_FUNCY*cap
     int* bondn = nullptr;
@@ -105,6 +112,7 @@ struct fm_ctx {
 
     bool cells_valid = false, nlist_valid = false, rdisp_valid = false;
     bool forces_valid = false;
+    bool moved_since_bin = true;          // cells are x-sorted only right after binning
     double simtime = 0.0;
     int last_record_iter = -1;
 
@@ -129,7 +137,7 @@ void ktimer_end(fm_ctx* c, int cls);
 enum { KT_PASS1 = 0, KT_CHECK, KT_SENDMOLS, KT_BIN, KT_NLIST, KT_FORCES, KT_PASS2, KT_MACROS, KT_NCLASS };
 extern const char* const kt_names[KT_NCLASS];
 
-// ---- launchers (fm_kernels.hip) -----------------------------------------------------------------
+// ---- launchers (fm_kernels.hip, fm_brick.hip) ----------------------------------------------------
 int launch_fill_i32(fm_ctx* c, int* p, int v, size_t n);
 int launch_fill_f64(fm_ctx* c, double* p, double v, size_t n);
 int launch_pass1(fm_ctx* c);
@@ -139,7 +147,8 @@ int launch_check_vmax(fm_ctx* c);
 int launch_sendmols(fm_ctx* c);
 int launch_bin(fm_ctx* c);
 int launch_nlist(fm_ctx* c);
-int launch_forces(fm_ctx* c, int iter);
+int launch_forces(fm_ctx* c, int iter, bool kick = false);
+int launch_pass1_check_disp(fm_ctx* c);
 int launch_macros(fm_ctx* c);
 int launch_scatter_state_to_host_order(fm_ctx* c, double* d_out9, double* d_tally);
 int launch_from_host(fm_ctx* c, const double* d_in9);
@@ -147,6 +156,12 @@ int launch_ext_to_slot_i32(fm_ctx* c, int* dst, const int* src, int nf);
 int launch_ext_to_slot_f64(fm_ctx* c, double* dst, const double* src, int nf);
 int launch_slot_to_ext_f64(fm_ctx* c, double* dst, const double* src, int nf);
 int launch_slot_to_ext_i32(fm_ctx* c, int* dst, const int* src, int nf);
+
+// LDS-brick kernels (fm_brick.hip)
+int brick_count(const fm_ctx* c);
+size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind);
+int launch_region_max(fm_ctx* c);
+int launch_decode_nlist(fm_ctx* c, uint32_t* out);
 
 // hipcub wrappers (fm_cub.hip)
 int cub_exclusive_scan(fm_ctx* c, const int* in, int* out, int n);

@@ -6,6 +6,7 @@
 // The mixed-precision pair law re-enables contraction locally.
 #include "fm_internal.h"
 
+#include <limits.h>
 #include <math.h>
 
 #pragma clang fp contract(off)
@@ -99,7 +100,44 @@ __global__ __launch_bounds__(FM_BLOCK) void k_check_disp(St s, double* rd, int n
     rd[i + 2 * (size_t)cap] = d2;
     bool hit = fabs(d0) > thresh || fabs(d1) > thresh || fabs(d2) > thresh;
     unsigned long long m = __ballot(hit);
-    if (m != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicOr(&flags[FLAG_REBUILD], 1);
+    if (m != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1 &&
+        __atomic_load_n(&flags[FLAG_REBUILD], __ATOMIC_RELAXED) == 0)
+        atomicOr(&flags[FLAG_REBUILD], 1);
+}
+
+// simulation_move_particles_vv(1) followed by the displacement update of simulation_checkrebuild,
+// in one pass over the state (same operations as k_pass1 then k_check_disp)
+__global__ __launch_bounds__(FM_BLOCK) void k_pass1_disp(St s, double* rd, int np, int cap, double dt, double hdt2,
+                                                          double hdt, double thresh, int fresh, int* flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    if (i < np) {
+        const double ax = s.ax[i], ay = s.ay[i], az = s.az[i];
+        double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+        s.x[i] = s.x[i] + dt * vx + hdt2 * ax;
+        s.y[i] = s.y[i] + dt * vy + hdt2 * ay;
+        s.z[i] = s.z[i] + dt * vz + hdt2 * az;
+        vx = vx + hdt * ax;
+        vy = vy + hdt * ay;
+        vz = vz + hdt * az;
+        s.vx[i] = vx;
+        s.vy[i] = vy;
+        s.vz[i] = vz;
+        double d0 = fresh ? 0.0 : rd[i];
+        double d1 = fresh ? 0.0 : rd[i + cap];
+        double d2 = fresh ? 0.0 : rd[i + 2 * (size_t)cap];
+        d0 = d0 + vx * dt;
+        d1 = d1 + vy * dt;
+        d2 = d2 + vz * dt;
+        rd[i] = d0;
+        rd[i + cap] = d1;
+        rd[i + 2 * (size_t)cap] = d2;
+        hit = fabs(d0) > thresh || fabs(d1) > thresh || fabs(d2) > thresh;
+    }
+    unsigned long long m = __ballot(hit);
+    if (m != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1 &&
+        __atomic_load_n(&flags[FLAG_REBUILD], __ATOMIC_RELAXED) == 0)
+        atomicOr(&flags[FLAG_REBUILD], 1);
 }
 
 // ---- rebuild criterion 0 (Rapaport vmax), src/simulation_checkrebuild.f90:84-99 --------------
@@ -118,7 +156,8 @@ __global__ __launch_bounds__(FM_BLOCK) void k_vmax(St s, int np, unsigned long l
         double b = red[0];
         for (int w = 1; w < FM_BLOCK / 64; ++w) b = fmax(b, red[w]);
         // non-negative doubles order like their bit patterns
-        atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(b));
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(b);
+        if (bits > __atomic_load_n(vmax_bits, __ATOMIC_RELAXED)) atomicMax(vmax_bits, bits);
     }
 }
 
@@ -274,22 +313,53 @@ __global__ __launch_bounds__(FM_BLOCK) void k_bin_scatter(const int* cell_of, co
     perm[pos] = i;
 }
 
-// order each cell by decreasing molecule number (the reference's head-inserted cell list order)
-__global__ __launch_bounds__(FM_BLOCK) void k_bin_sort(const int* start, int* perm, const int* ext, int ncell,
-                                                       int maxcell, int* flags) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    int b = start[c], e = start[c + 1];
-    if (e - b > maxcell) set_err(flags, FM_EOVERFLOW, c);
-    atomicMax(&flags[FLAG_MAX_CELL], e - b);
-    for (int k = b + 1; k < e; ++k) {
-        int v = perm[k], key = ext[v];
-        int m = k - 1;
-        while (m >= b && ext[perm[m]] < key) {
-            perm[m + 1] = perm[m];
-            --m;
+// order each cell by increasing x (ties: molecule number), which lets the list build window
+// its candidate runs by x: one wavefront per cell, bitonic sort in registers; cells above 64
+// molecules fall back to an insertion sort by lane 0
+__device__ inline bool xkey_less(double xa, int ea, double xb, int eb) { return xa < xb || (xa == xb && ea < eb); }
+
+__global__ __launch_bounds__(FM_BLOCK) void k_bin_sort(const int* start, int* perm, const int* ext, const double* x,
+                                                       int ncell, int maxcell, int* flags) {
+    const int lane = threadIdx.x & 63;
+    const int c = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (c >= ncell) return;  // whole wavefront
+    const int b = start[c], e = start[c + 1], n = e - b;
+    if (n > maxcell && lane == 0) set_err(flags, FM_EOVERFLOW, c);
+    // read before the atomic: the maximum settles after a few cells, so few atomics are issued
+    if (lane == 0 && n > __atomic_load_n(&flags[FLAG_MAX_CELL], __ATOMIC_RELAXED)) atomicMax(&flags[FLAG_MAX_CELL], n);
+    if (n <= 1) return;
+    if (n <= 64) {
+        int v = lane < n ? perm[b + lane] : -1;
+        double kx = lane < n ? x[v] : INFINITY;  // padding sorts last
+        int ke = lane < n ? ext[v] : INT_MAX;
+        for (int k = 2; k <= 64; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const double okx = __shfl_xor(kx, j, 64);
+                const int oke = __shfl_xor(ke, j, 64);
+                const int ov = __shfl_xor(v, j, 64);
+                const bool asc = (lane & k) == 0;
+                const bool lower = (lane & j) == 0;
+                const bool o_less = xkey_less(okx, oke, kx, ke), o_more = xkey_less(kx, ke, okx, oke);
+                const bool take = lower ? (asc ? o_less : o_more) : (asc ? o_more : o_less);
+                if (take) {
+                    kx = okx;
+                    ke = oke;
+                    v = ov;
+                }
+            }
+        if (lane < n) perm[b + lane] = v;
+    } else if (lane == 0) {
+        for (int k = b + 1; k < e; ++k) {
+            const int v = perm[k];
+            const double kx = x[v];
+            const int ke = ext[v];
+            int m = k - 1;
+            while (m >= b && xkey_less(kx, ke, x[perm[m]], ext[perm[m]])) {
+                perm[m + 1] = perm[m];
+                --m;
+            }
+            perm[m + 1] = v;
         }
-        perm[m + 1] = v;
     }
 }
 
@@ -323,398 +393,6 @@ __global__ __launch_bounds__(FM_BLOCK) void k_permute(PermArgs a) {
     }
     if (a.mono_dst)
         for (int f = 0; f < 7; ++f) a.mono_dst[k + f * cap] = a.mono_src[o + f * cap];
-}
-
-// ---- neighbour list, src/linklist.f90:660-804 (set semantics), bonds 2542-2584 -----------------
-struct NlArgs {
-    St s;
-    const int* cell_of;
-    const int* start;
-    const int* ext;
-    const int* mono;      // chainID, subchainID, funcy, bflag[4] (slot order) or null
-    uint32_t* nbl;
-    int* nbn;
-    uint32_t* bond;
-    int* bondn;
-    int* flags;
-    int np, cap, maxnb;
-    int nx, ny, nz;
-    double Lx, Ly, Lz, rn2;
-};
-
-__device__ inline int get_bondflag(const int* mono, int cap, int i, int scID) {
-    // get_bondflag, src/modules.f90:597-610 (intbits = 31)
-    int group = (int)ceilf((float)scID / 31.0f);
-    int expo = scID % 31 - 1;
-    if (expo == -1) expo = 30;
-    if (group < 1 || group > 4) return 0;
-    int w = mono[i + (size_t)(2 + group) * cap];
-    return (w >> expo) & 1;
-}
-
-__global__ __launch_bounds__(FM_BLOCK) void k_nlist(NlArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.np) return;
-    const int ci = a.cell_of[i];
-    const int cx = ci % a.nx, cy = (ci / a.nx) % a.ny, cz = ci / (a.nx * a.ny);
-    const double xi = a.s.x[i], yi = a.s.y[i], zi = a.s.z[i];
-    const size_t cap = a.cap;
-    int cnt = 0, nb = 0;
-    int chain = 0, funcy = 0;
-    if (a.mono) {
-        chain = a.mono[i];
-        funcy = a.mono[i + 2 * cap];
-    }
-    for (int dz = -1; dz <= 1; ++dz) {
-        int zz = cz + dz, sz = 0;
-        if (zz < 0) { zz += a.nz; sz = -1; } else if (zz >= a.nz) { zz -= a.nz; sz = 1; }
-        for (int dy = -1; dy <= 1; ++dy) {
-            int yy = cy + dy, sy = 0;
-            if (yy < 0) { yy += a.ny; sy = -1; } else if (yy >= a.ny) { yy -= a.ny; sy = 1; }
-            for (int dx = -1; dx <= 1; ++dx) {
-                int xx = cx + dx, sx = 0;
-                if (xx < 0) { xx += a.nx; sx = -1; } else if (xx >= a.nx) { xx -= a.nx; sx = 1; }
-                const int c = xx + a.nx * (yy + a.ny * zz);
-                const uint32_t code = (uint32_t)shift_code(sx, sy, sz);
-                const int jb = a.start[c], je = a.start[c + 1];
-                for (int j = jb; j < je; ++j) {
-                    if (j == i) continue;
-                    double rx = xi - img(a.s.x[j], sx, a.Lx);
-                    double ry = yi - img(a.s.y[j], sy, a.Ly);
-                    double rz = zi - img(a.s.z[j], sz, a.Lz);
-                    double r2 = (rx * rx + ry * ry) + rz * rz;
-                    if (r2 < a.rn2) {
-                        if (cnt < a.maxnb) a.nbl[(size_t)cnt * cap + i] = ((uint32_t)j << 5) | code;
-                        ++cnt;
-                    }
-                    if (a.mono && a.mono[j] == chain) {
-                        if (get_bondflag(a.mono, a.cap, i, a.mono[j + cap])) {
-                            if (nb < funcy && nb < FM_MAX_FUNCY)
-                                a.bond[(size_t)nb * cap + i] = ((uint32_t)j << 5) | code;
-                            ++nb;
-                        }
-                    }
-                }
-            }
-        }
-    }
-    a.nbn[i] = cnt < a.maxnb ? cnt : a.maxnb;
-    atomicMax(&a.flags[FLAG_MAX_NB], cnt);
-    if (cnt > a.maxnb) set_err(a.flags, FM_EOVERFLOW, a.ext[i]);
-    if (a.mono) {
-        a.bondn[i] = nb < funcy ? nb : funcy;
-        if (nb > funcy) set_err(a.flags, FM_EBOND, a.ext[i]);
-    }
-}
-
-// ---- forces, src/simulation_compute_forces.f90:628-726 (pairs), 731-789 (bonds) ---------------
-struct FArgs {
-    St s;
-    const uint32_t* nbl;
-    const int* nbn;
-    const uint32_t* bond;
-    const int* bondn;
-    const int* ext;
-    const int* cell_of;
-    const int* moltype;    // null unless Mie
-    const int* tag;        // null unless tag_move
-    double* teth;          // rtether xyz, slidev xyz
-    double* pe_lj;
-    double* pe_poly;
-    double* vir;
-    double* rf;            // 9*cap or null
-    double* pxy;           // 3*nplanes or null
-    const double* planes;
-    int* flags;
-    int np, cap, nx, ny;
-    int rec, ev;           // record step (PE, virial, tallies) / virial-only step
-    double Lx, Ly, Lz, hx, hz, rc2, potshift;
-    double k_c, R0, R02, dt;
-    double teth_k2, teth_k4, teth_k6;
-    // MOP
-    int nplanes, pn;
-    double hp, spacing, shift;
-    // Mie tables
-    const double* mie;     // [7][FM_MAX_TYPES^2]: eps, lr, la, C, alpha, potshift, rc2 ; + mass[4]
-};
-
-// forward cell offsets of the reference's half-list scan (src/linklist.f90:686-688)
-__device__ inline bool is_forward(int dx, int dy, int dz) {
-    if (dz == 1) return true;
-    if (dz == -1) return false;
-    if (dy == 1) return true;
-    if (dy == -1) return dx == 1;
-    return dx == 1;
-}
-
-__device__ inline double dp_heaviside(double x) { return ceil(copysign(0.5, x)); }
-
-// pressure_tensor_forces_MOP (src/simulation_record.f90:7068-7112) for one list entry
-__device__ void mop_entry(const FArgs& a, const double ri[3], const double rj[3], const double rij[3],
-                          double acc, double* spxy) {
-    int pi = (int)llround((ri[1] + a.hp - a.shift) / a.spacing) + 1;
-    pi = pi < 1 ? 1 : (pi > a.nplanes ? a.nplanes : pi);
-    int pj = (int)llround((rj[1] + a.hp - a.shift) / a.spacing) + 1;
-    pj = pj < 1 ? 1 : (pj > a.nplanes ? a.nplanes : pj);
-    int step = (pj - pi) >= 0 ? 1 : -1;
-    for (int n = pi;; n += step) {
-        double plane = a.planes[n - 1];
-        double pyb0 = ri[0] + (rij[0] / rij[1]) * (plane - ri[1]);
-        double pyb2 = ri[2] + (rij[2] / rij[1]) * (plane - ri[1]);
-        double fx = copysign(1.0, ri[1] - plane) - copysign(1.0, rj[1] - plane);
-        double hxx = dp_heaviside(a.hx - pyb0) - dp_heaviside(-a.hx - pyb0);
-        double hzz = dp_heaviside(a.hz - pyb2) - dp_heaviside(-a.hz - pyb2);
-        if (fx != 0.0 && hxx != 0.0 && hzz != 0.0) {
-            for (int d = 0; d < 3; ++d) atomicAdd(&spxy[3 * (n - 1) + d], acc * rij[d] * fx * hxx * hzz);
-        }
-        if (n == pj) break;
-    }
-}
-
-template <bool MIE, bool RECORD>
-__global__ __launch_bounds__(FM_BLOCK) void k_forces_fp64(FArgs a) {
-    extern __shared__ double spxy[];
-    const bool mop = RECORD && a.pxy != nullptr;
-    if (mop) {
-        for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
-        __syncthreads();
-    }
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < a.np) {
-        const size_t cap = a.cap;
-        const double xi = a.s.x[i], yi = a.s.y[i], zi = a.s.z[i];
-        const int n = a.nbn[i];
-        double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, vir = 0.0;
-        double rf[9];
-        const bool do_rf = RECORD && a.rf != nullptr;
-        if (do_rf)
-            for (int q = 0; q < 9; ++q) rf[q] = 0.0;
-        int ti = 0;
-        double mi = 1.0;
-        if (MIE) {
-            ti = a.moltype[i];
-            mi = a.mie[7 * FM_MAX_TYPES * FM_MAX_TYPES + ti];
-        }
-        const int exti = mop ? a.ext[i] : 0;
-        const int ci = mop ? a.cell_of[i] : 0;
-        for (int k = 0; k < n; ++k) {
-            const uint32_t w = a.nbl[(size_t)k * cap + i];
-            const int j = (int)(w >> 5);
-            int sx, sy, sz;
-            decode_shift((int)(w & 31u), sx, sy, sz);
-            const double rj[3] = {img(a.s.x[j], sx, a.Lx), img(a.s.y[j], sy, a.Ly), img(a.s.z[j], sz, a.Lz)};
-            const double rx = xi - rj[0], ry = yi - rj[1], rz = zi - rj[2];
-            const double r2 = (rx * rx + ry * ry) + rz * rz;
-            double rc2 = a.rc2;
-            int t = 0;
-            if (MIE) {
-                t = ti * FM_MAX_TYPES + a.moltype[j];
-                double c = a.mie[6 * FM_MAX_TYPES * FM_MAX_TYPES + t];
-                if (c > 0.0) rc2 = c;
-            }
-            if (!(r2 < rc2)) continue;
-            const double inv = 1.0 / r2;
-            double acc, e = 0.0;
-            if (!MIE) {
-                acc = 48.0 * (p7(inv) - 0.5 * p4(inv));
-                if (RECORD) e = 4.0 * (p6(inv) - p3(inv)) - a.potshift;
-            } else {
-                const int T2 = FM_MAX_TYPES * FM_MAX_TYPES;
-                double eps = a.mie[t], lr = a.mie[T2 + t], la = a.mie[2 * T2 + t], C = a.mie[3 * T2 + t],
-                       al = a.mie[4 * T2 + t];
-                acc = C * eps * (lr * pow(inv, 0.5 * lr + 1) - al * la * pow(inv, 0.5 * la + 1));
-                if (RECORD) e = C * eps * (pow(inv, 0.5 * lr) - al * pow(inv, 0.5 * la)) - a.mie[5 * T2 + t];
-            }
-            if (MIE) {
-                fx = fx + acc * rx / mi;
-                fy = fy + acc * ry / mi;
-                fz = fz + acc * rz / mi;
-            } else {  // mass 1: x/1.0 == x
-                fx = fx + acc * rx;
-                fy = fy + acc * ry;
-                fz = fz + acc * rz;
-            }
-            if (RECORD || a.ev) vir = vir + acc * r2;
-            if (RECORD) {
-                pe = pe + e;
-                if (do_rf) {
-                    const double rr[3] = {rx, ry, rz};
-                    for (int p = 0; p < 3; ++p)
-                        for (int q = 0; q < 3; ++q) rf[3 * p + q] = rf[3 * p + q] + acc * rr[p] * rr[q];
-                }
-                if (mop) {
-                    // the reference visits each unshifted pair once, from the list of the molecule
-                    // whose cell scans the other's as a forward cell (same cell: the higher molno)
-                    bool own = true;
-                    if (sx == 0 && sy == 0 && sz == 0) {
-                        const int cj = a.cell_of[j];
-                        if (cj == ci) own = exti > a.ext[j];
-                        else {
-                            int ddx = cj % a.nx - ci % a.nx, ddy = (cj / a.nx) % a.ny - (ci / a.nx) % a.ny,
-                                ddz = cj / (a.nx * a.ny) - ci / (a.nx * a.ny);
-                            own = is_forward(ddx, ddy, ddz);
-                        }
-                    }
-                    if (own) {
-                        const double ri[3] = {xi, yi, zi}, rij[3] = {rx, ry, rz};
-                        mop_entry(a, ri, rj, rij, acc, spxy);
-                    }
-                }
-            }
-        }
-        if (a.bondn) {  // FENE, simulation_compute_forces_poly
-            const int nb = a.bondn[i];
-            double pp = 0.0;
-            for (int b = 0; b < nb; ++b) {
-                const uint32_t w = a.bond[(size_t)b * cap + i];
-                const int j = (int)(w >> 5);
-                int sx, sy, sz;
-                decode_shift((int)(w & 31u), sx, sy, sz);
-                const double rx = xi - img(a.s.x[j], sx, a.Lx), ry = yi - img(a.s.y[j], sy, a.Ly),
-                             rz = zi - img(a.s.z[j], sz, a.Lz);
-                const double r2 = (rx * rx + ry * ry) + rz * rz;
-                if (r2 >= a.R02) {
-                    set_err(a.flags, FM_EBOND, a.ext[i]);
-                    continue;
-                }
-                const double acc = -a.k_c / (1.0 - (r2 / a.R02));
-                fx = fx + acc * rx / mi;
-                fy = fy + acc * ry / mi;
-                fz = fz + acc * rz / mi;
-                if (RECORD || a.ev) vir = vir + acc * r2;
-                if (RECORD) {
-                    pp = pp + 0.5 * a.k_c * a.R0 * a.R0 * log(1.0 - (r2 / a.R02));
-                    if (do_rf) {
-                        const double rr[3] = {rx, ry, rz};
-                        for (int p = 0; p < 3; ++p)
-                            for (int q = 0; q < 3; ++q) rf[3 * p + q] = rf[3 * p + q] + acc * rr[p] * rr[q];
-                    }
-                }
-            }
-            if (RECORD) a.pe_poly[i] = pp;
-        }
-        if (a.tag) {  // tether_force, src/molecular_properties.f90:833-915
-            const int tg = a.tag[i];
-            if (tg == FM_TAG_TETH || tg == FM_TAG_TETH_SLIDE) {
-                double* T = a.teth;
-                const double rio[3] = {xi - T[i], yi - T[i + cap], zi - T[i + 2 * cap]};
-                const double mag = sqrt((rio[0] * rio[0] + rio[1] * rio[1]) + rio[2] * rio[2]);
-                const double am = -2.0 * a.teth_k2 * mag - 4.0 * a.teth_k4 * (mag * mag) -
-                                  6.0 * a.teth_k6 * pow(mag, 4.0);
-                fx = fx + am * rio[0];
-                fy = fy + am * rio[1];
-                fz = fz + am * rio[2];
-                for (int d = 0; d < 3; ++d) T[i + d * cap] = T[i + d * cap] + T[i + (3 + d) * cap] * a.dt;
-            }
-        }
-        a.s.ax[i] = fx;
-        a.s.ay[i] = fy;
-        a.s.az[i] = fz;
-        if (RECORD || a.ev) a.vir[i] = vir;
-        if (RECORD) {
-            a.pe_lj[i] = pe;
-            if (do_rf)
-                for (int q = 0; q < 9; ++q) a.rf[i + q * cap] = rf[q];
-        }
-    }
-    if (mop) {
-        __syncthreads();
-        for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x)
-            if (spxy[t] != 0.0) atomicAdd(&a.pxy[t], spxy[t]);
-    }
-}
-
-// Mixed precision: image coordinates and differences in FP64, the pair law in FP32, per-molecule
-// force summed in FP32; energy, virial and tallies accumulated in FP64.
-template <bool RECORD>
-__global__ __launch_bounds__(FM_BLOCK) void k_forces_mixed(FArgs a) {
-    extern __shared__ double spxy[];
-    const bool mop = RECORD && a.pxy != nullptr;
-    if (mop) {
-        for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
-        __syncthreads();
-    }
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < a.np) {
-        const size_t cap = a.cap;
-        const double xi = a.s.x[i], yi = a.s.y[i], zi = a.s.z[i];
-        const int n = a.nbn[i];
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        double pe = 0.0, vir = 0.0;
-        const float rc2 = (float)a.rc2;
-        const int exti = mop ? a.ext[i] : 0;
-        const int ci = mop ? a.cell_of[i] : 0;
-        for (int k = 0; k < n; ++k) {
-            const uint32_t w = a.nbl[(size_t)k * cap + i];
-            const int j = (int)(w >> 5);
-            int sx, sy, sz;
-            decode_shift((int)(w & 31u), sx, sy, sz);
-            const double rj[3] = {img(a.s.x[j], sx, a.Lx), img(a.s.y[j], sy, a.Ly), img(a.s.z[j], sz, a.Lz)};
-            const double rxd = xi - rj[0], ryd = yi - rj[1], rzd = zi - rj[2];
-            const float rx = (float)rxd, ry = (float)ryd, rz = (float)rzd;
-            float acc, inv;
-            {
-#pragma clang fp contract(fast)
-                const float r2 = rx * rx + ry * ry + rz * rz;
-                if (!(r2 < rc2)) continue;
-                inv = __builtin_amdgcn_rcpf(r2);
-                const float i2 = inv * inv, i4 = i2 * i2, i3 = i2 * inv;
-                acc = 48.f * (i4 * i3 - 0.5f * i4);
-                fx += acc * rx;
-                fy += acc * ry;
-                fz += acc * rz;
-            }
-            if (RECORD || a.ev) {
-                const double r2d = (rxd * rxd + ryd * ryd) + rzd * rzd;
-                vir = vir + (double)acc * r2d;
-            }
-            if (RECORD) {
-                const double id = (double)inv;
-                pe = pe + (4.0 * (p6(id) - p3(id)) - a.potshift);
-                if (mop) {
-                    bool own = true;
-                    if (sx == 0 && sy == 0 && sz == 0) {
-                        const int cj = a.cell_of[j];
-                        if (cj == ci) own = exti > a.ext[j];
-                        else {
-                            int ddx = cj % a.nx - ci % a.nx, ddy = (cj / a.nx) % a.ny - (ci / a.nx) % a.ny,
-                                ddz = cj / (a.nx * a.ny) - ci / (a.nx * a.ny);
-                            own = is_forward(ddx, ddy, ddz);
-                        }
-                    }
-                    if (own) {
-                        const double ri[3] = {xi, yi, zi}, rij[3] = {rxd, ryd, rzd};
-                        mop_entry(a, ri, rj, rij, (double)acc, spxy);
-                    }
-                }
-            }
-        }
-        double dfx = fx, dfy = fy, dfz = fz;
-        if (a.tag) {
-            const int tg = a.tag[i];
-            if (tg == FM_TAG_TETH || tg == FM_TAG_TETH_SLIDE) {
-                double* T = a.teth;
-                const double rio[3] = {xi - T[i], yi - T[i + cap], zi - T[i + 2 * cap]};
-                const double mag = sqrt((rio[0] * rio[0] + rio[1] * rio[1]) + rio[2] * rio[2]);
-                const double am = -2.0 * a.teth_k2 * mag - 4.0 * a.teth_k4 * (mag * mag) -
-                                  6.0 * a.teth_k6 * pow(mag, 4.0);
-                dfx = dfx + am * rio[0];
-                dfy = dfy + am * rio[1];
-                dfz = dfz + am * rio[2];
-                for (int d = 0; d < 3; ++d) T[i + d * cap] = T[i + d * cap] + T[i + (3 + d) * cap] * a.dt;
-            }
-        }
-        a.s.ax[i] = dfx;
-        a.s.ay[i] = dfy;
-        a.s.az[i] = dfz;
-        if (RECORD || a.ev) a.vir[i] = vir;
-        if (RECORD) a.pe_lj[i] = pe;
-    }
-    if (mop) {
-        __syncthreads();
-        for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x)
-            if (spxy[t] != 0.0) atomicAdd(&a.pxy[t], spxy[t]);
-    }
 }
 
 // ---- macroscopic sums, src/simulation_record.f90:1205-1314 ------------------------------------
@@ -920,8 +598,8 @@ int launch_bin(fm_ctx* c) {
                                                                c->np);
     LAUNCH_CHECK();
     int maxcell = c->p.max_cell_np > 0 ? c->p.max_cell_np : (1 << 26);
-    k_bin_sort<<<nblocks(c->ncell), FM_BLOCK, 0, c->stream>>>(c->cell_start, c->perm, c->ext[c->cur], c->ncell,
-                                                               maxcell, c->d_flags);
+    k_bin_sort<<<nblocks((size_t)c->ncell * 64), FM_BLOCK, 0, c->stream>>>(c->cell_start, c->perm, c->ext[c->cur],
+                                                                            s.x, c->ncell, maxcell, c->d_flags);
     LAUNCH_CHECK();
     PermArgs a;
     a.src = c->st[c->cur];
@@ -946,106 +624,6 @@ int launch_bin(fm_ctx* c) {
     ktimer_end(c, KT_BIN);
     LAUNCH_CHECK();
     c->cur = nxt;
-    return 0;
-}
-
-int launch_nlist(fm_ctx* c) {
-    if (c->np == 0) return 0;
-    ktimer_begin(c, KT_NLIST);
-    NlArgs a;
-    a.s = state_of(c->x(), c->cap);
-    a.cell_of = c->cell_of;
-    a.start = c->cell_start;
-    a.ext = c->ext[c->cur];
-    a.mono = (c->p.potential_flag == 1 && c->has_mono) ? c->mono[c->cur] : nullptr;
-    a.nbl = c->nbl;
-    a.nbn = c->nbn;
-    a.bond = c->bond;
-    a.bondn = c->bondn;
-    a.flags = c->d_flags;
-    a.np = c->np;
-    a.cap = c->cap;
-    a.maxnb = c->maxnb;
-    a.nx = c->nc[0];
-    a.ny = c->nc[1];
-    a.nz = c->nc[2];
-    a.Lx = c->p.domain[0];
-    a.Ly = c->p.domain[1];
-    a.Lz = c->p.domain[2];
-    a.rn2 = c->rn2;
-    k_nlist<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(a);
-    ktimer_end(c, KT_NLIST);
-    LAUNCH_CHECK();
-    return 0;
-}
-
-int launch_forces(fm_ctx* c, int iter) {
-    if (c->np == 0) return 0;
-    FArgs a;
-    const int cur = c->cur;
-    a.s = state_of(c->x(), c->cap);
-    a.nbl = c->nbl;
-    a.nbn = c->nbn;
-    const bool poly = c->p.potential_flag == 1 && c->has_mono;
-    a.bond = poly ? c->bond : nullptr;
-    a.bondn = poly ? c->bondn : nullptr;
-    a.ext = c->ext[cur];
-    a.cell_of = c->cell_of;
-    const bool mie = c->p.mie_potential == 1;
-    a.moltype = mie ? c->moltype[cur] : nullptr;
-    const bool tm = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && c->has_tags;
-    a.tag = tm ? c->tag[cur] : nullptr;
-    a.teth = tm ? c->teth[cur] : nullptr;
-    a.pe_lj = c->pe_lj;
-    a.pe_poly = c->pe_poly;
-    a.vir = c->vir;
-    a.flags = c->d_flags;
-    a.np = c->np;
-    a.cap = c->cap;
-    a.nx = c->nc[0];
-    a.ny = c->nc[1];
-    const int tplot = c->p.tplot, teval = c->p.teval;
-    a.rec = (iter % tplot) == 0;
-    a.ev = teval > 0 && (iter % teval) == 0;
-    a.rf = (a.rec && c->p.pressure_outflag == 1) ? c->rf : nullptr;
-    const bool mop = a.rec && c->p.vflux_outflag >= 1 && c->p.vflux_outflag <= 3 && c->pxy;
-    a.pxy = mop ? c->pxy : nullptr;
-    a.planes = c->planes;
-    a.Lx = c->p.domain[0];
-    a.Ly = c->p.domain[1];
-    a.Lz = c->p.domain[2];
-    a.hx = c->half[0];
-    a.hz = c->half[2];
-    a.rc2 = c->rc2;
-    a.potshift = c->potshift;
-    a.k_c = c->p.k_c;
-    a.R0 = c->p.R_0;
-    a.R02 = c->p.R_0 * c->p.R_0;
-    a.dt = c->p.delta_t;
-    a.teth_k2 = c->p.teth_k2;
-    a.teth_k4 = c->p.teth_k4;
-    a.teth_k6 = c->p.teth_k6;
-    a.nplanes = c->p.nplanes;
-    a.pn = c->p.vflux_outflag - 1;
-    a.hp = (a.pn >= 0 && a.pn < 3) ? c->half[a.pn] : 0.0;
-    a.spacing = c->p.planespacing;
-    a.shift = c->mop_shift;
-    a.mie = mie ? c->d_mie : nullptr;
-    size_t shm = mop ? 3 * (size_t)c->p.nplanes * sizeof(double) : 0;
-    ktimer_begin(c, KT_FORCES);
-    dim3 g(nblocks(c->np)), b(FM_BLOCK);
-    if (c->p.precision == FM_PREC_MIXED && !mie && !poly) {
-        if (a.rec) k_forces_mixed<true><<<g, b, shm, c->stream>>>(a);
-        else k_forces_mixed<false><<<g, b, shm, c->stream>>>(a);
-    } else if (mie) {
-        if (a.rec) k_forces_fp64<true, true><<<g, b, shm, c->stream>>>(a);
-        else k_forces_fp64<true, false><<<g, b, shm, c->stream>>>(a);
-    } else {
-        if (a.rec) k_forces_fp64<false, true><<<g, b, shm, c->stream>>>(a);
-        else k_forces_fp64<false, false><<<g, b, shm, c->stream>>>(a);
-    }
-    ktimer_end(c, KT_FORCES);
-    LAUNCH_CHECK();
     return 0;
 }
 
@@ -1105,5 +683,18 @@ int launch_slot_to_ext_i32(fm_ctx* c, int* dst, const int* src, int nf) {
     if (c->np == 0) return 0;
     k_slot_to_ext<int><<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(dst, src, c->cap, nf, c->ext[c->cur], c->np);
     LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_pass1_check_disp(fm_ctx* c) {
+    if (c->np == 0) return 0;
+    const double dt = c->p.delta_t;
+    ktimer_begin(c, KT_PASS1);
+    k_pass1_disp<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(state_of(c->x(), c->cap), c->rdisp, c->np, c->cap, dt,
+                                                              0.5 * (dt * dt), 0.5 * dt, 0.5 * c->dmin,
+                                                              c->rdisp_valid ? 0 : 1, c->d_flags);
+    ktimer_end(c, KT_PASS1);
+    LAUNCH_CHECK();
+    c->rdisp_valid = true;
     return 0;
 }

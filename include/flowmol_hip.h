@@ -101,9 +101,11 @@ typedef struct fm_params {
     int32_t _pad1;
     double planespacing;         /* plane positions: planespacing*(n-1)+shift-halfdomain, src/setup_set_parameters.f90:2298-2340 */
 
-    /* capacities (0 = automatic) */
+    /* capacities and launch shape (0 = automatic) */
     int32_t max_neighbours;      /* per-molecule neighbour list capacity */
     int32_t max_cell_np;         /* per-cell capacity check */
+    int32_t brick[3];            /* cells per LDS brick along x,y,z (0,0,0: chosen from the occupancy) */
+    int32_t _pad2;
 } fm_params;
 
 typedef struct fm_ctx fm_ctx;   /* opaque */
