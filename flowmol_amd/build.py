@@ -44,6 +44,38 @@ def _compile(src):
     return obj
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_BIN = os.path.join(FORTRAN_DIR, "bin")
+
+
+def amdflang():
+    for c in (os.environ.get("FC_AMD"), "/opt/rocm/bin/amdflang", shutil.which("amdflang")):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran(verbose=True):
+    """The iso_c_binding interface module and the Fortran time-loop driver (amdflang)."""
+    fc = amdflang()
+    if fc is None:
+        if verbose:
+            print("amdflang not found: Fortran interface not built")
+        return None
+    os.makedirs(FORTRAN_BIN, exist_ok=True)
+    mod_obj = os.path.join(FORTRAN_BIN, "flowmol_hip_c.o")
+    cmds = [
+        [fc, "-O2", "-c", os.path.join(FORTRAN_DIR, "flowmol_hip_c.f90"), "-J", FORTRAN_BIN, "-o", mod_obj],
+        [fc, "-O2", os.path.join(FORTRAN_DIR, "md_vv_driver.f90"), mod_obj, "-J", FORTRAN_BIN, "-L", HERE,
+         "-lflowmol_hip", "-Wl,-rpath,$ORIGIN/../..", "-o", os.path.join(FORTRAN_BIN, "md_vv_driver")],
+    ]
+    for cmd in cmds:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return os.path.join(FORTRAN_BIN, "md_vv_driver")
+
+
 def build(jobs=None, verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs or len(SOURCES)) as ex:
@@ -56,6 +88,7 @@ def build(jobs=None, verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_fortran(verbose)
     return LIB
 
 

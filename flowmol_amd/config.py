@@ -166,3 +166,78 @@ def config_c2(nunits=64, **kw):
     """1,048,576-molecule LJ NVE case (FCC 64^3, density 0.8, rc 2.5, delta 0.3, dt 0.005)."""
     kw.setdefault("tplot", 10)
     return config_fcc((nunits, nunits, nunits), 0.8, 2.5, seed=2, **kw)
+
+
+# ---- polymer melt: dense FENE chains on the FCC lattice ---------------------------------------------
+def chain_topology(nchains, nmono):
+    """polymer_info_MD monomer fields for linear chains: chainID, subchainID, funcy, bin_bflag(4).
+
+    Consecutive molecules form chains of nmono beads (setup_lattice_dense_FENE_info,
+    src/setup_initialise_microstate.f90:347-425); bead s is bonded to s-1 and s+1, recorded as
+    bits of bin_bflag (get_bondflag, src/modules.f90:597-610: group ceil(s/31), bit mod(s,31)-1).
+    """
+    n = nchains * nmono
+    chain = np.repeat(np.arange(1, nchains + 1), nmono).astype(np.int32)
+    sub = np.tile(np.arange(1, nmono + 1), nchains).astype(np.int32)
+    funcy = np.where((sub == 1) | (sub == nmono), 1, 2).astype(np.int32)
+    if nmono == 1:
+        funcy[:] = 0
+    bflag = np.zeros((n, 4), np.int64)
+    for nb in (-1, 1):
+        t = sub + nb
+        ok = (t >= 1) & (t <= nmono)
+        g = (t - 1) // 31
+        b = (t - 1) % 31
+        for grp in range(4):
+            m = ok & (g == grp)
+            bflag[m, grp] |= (1 << b[m])
+    return chain, sub, funcy, bflag.astype(np.int32)
+
+
+def config_polymer(nunits=(6, 8, 6), density=0.85, nmono=8, seed=4, k_c=30.0, R_0=1.5, dt=0.002,
+                   mie_fraction=0.5, tplot=10, **kw):
+    """FENE chains of nmono beads on the FCC lattice (4*nunits[1] divisible by nmono) with WCA
+    between all beads, and a second species (whole chains, chosen with a seeded shuffle) whose
+    mutual interaction is Mie 14-6 (eps 1, rc 2.5); species 0 pairs and mixed pairs are WCA."""
+    if (4 * nunits[1]) % nmono:
+        raise ValueError("4*nunits_y must be divisible by the chain length")
+    domain = fcc_globaldomain(nunits, density)
+    r = fcc_lattice(nunits, density)
+    n = r.shape[0]
+    nchains = n // nmono
+    v = seeded_velocities(n, 1.0, seed)
+    p = make_params(domain, density, 2.5, dt=dt, tplot=tplot, potential_flag=1, k_c=k_c, R_0=R_0, **kw)
+    set_mie_types(p, {(0, 0): dict(lr=12.0, la=6.0, eps=1.0, rc=WCA_CUT),
+                      (0, 1): dict(lr=12.0, la=6.0, eps=1.0, rc=WCA_CUT),
+                      (1, 1): dict(lr=14.0, la=6.0, eps=1.0, rc=2.5)})
+    rng = np.random.default_rng(seed)
+    chain_species = np.zeros(nchains, np.int32)
+    chain_species[rng.permutation(nchains)[: int(round(mie_fraction * nchains))]] = 1
+    moltype = np.repeat(chain_species, nmono).astype(np.int32)
+    mono = chain_topology(nchains, nmono)
+    return p, r, v, moltype, mono
+
+
+# ---- Couette flow between tethered walls -------------------------------------------------------------
+def config_couette(nunits=(8, 8, 8), density=0.8, rcutoff=2.5, wall_units=3, wallslidev=1.0, nplanes=64,
+                   seed=3, tplot=1, precision=0, **kw):
+    """Tethered FCC walls `wall_units` lattice units thick at the bottom and top of y; the top wall
+    slides at wallslidev along x (TETHERCOEFFICIENTS 0 5000 5000000 as src/default.in); MOP stress
+    on nplanes planes normal to y every record step."""
+    from ._abi import FM_ENSEMBLE_TAG_MOVE, FM_TAG_FREE, FM_TAG_TETH, FM_TAG_TETH_SLIDE
+    domain = fcc_globaldomain(nunits, density)
+    r = fcc_lattice(nunits, density)
+    n = r.shape[0]
+    a = domain[1] / nunits[1]
+    h = 0.5 * domain[1]
+    tag = np.full(n, FM_TAG_FREE, np.int32)
+    tag[r[:, 1] < -h + wall_units * a] = FM_TAG_TETH
+    tag[r[:, 1] > h - wall_units * a] = FM_TAG_TETH_SLIDE
+    v = seeded_velocities(n, 1.0, seed)
+    v[tag != FM_TAG_FREE] = 0.0
+    spacing = domain[1] / nplanes
+    p = make_params(domain, density, rcutoff, tplot=tplot, ensemble=FM_ENSEMBLE_TAG_MOVE, vflux_outflag=2,
+                    nplanes=nplanes, planespacing=spacing, precision=precision, **kw)
+    p.teth_k2, p.teth_k4, p.teth_k6 = 0.0, 5000.0, 5.0e6
+    p.wallslidev[0] = wallslidev
+    return p, r, v, tag, r.copy()

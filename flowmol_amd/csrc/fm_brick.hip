@@ -116,10 +116,13 @@ __device__ inline void carve(Region& r, char* smem, const BG& g, int maxreg, boo
     r.ic_rc = ip;
 }
 
+// Bytes of the region tables, rounded up to 16 so that an array of doubles placed after them
+// (the MOP plane tallies, updated with 64-bit LDS atomics) is naturally aligned.
 __host__ __device__ inline size_t region_lds_bytes(const BG& g, int maxreg, bool with_slot) {
     const int nbmax = g.B[0] * g.B[1] * g.B[2];
-    return (size_t)maxreg * (3 * sizeof(double) + (with_slot ? sizeof(int) : 0)) +
-           (size_t)(3 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int) + 16;
+    const size_t b = (size_t)maxreg * (3 * sizeof(double) + (with_slot ? sizeof(int) : 0)) +
+                     (size_t)(3 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int) + 16;
+    return (b + 15) & ~(size_t)15;
 }
 
 // exclusive scan of cnt[0..n) into off[0..n] by the first wavefront
@@ -845,6 +848,8 @@ int launch_forces(fm_ctx* c, int iter, bool kick) {
     a.kick = kick ? 1 : 0;
     a.hdt = 0.5 * c->p.delta_t;
     const size_t lds = brick_lds_bytes(c, c->maxreg, mop ? 2 : 1);
+    if (mop && (region_lds_bytes(g, c->maxreg, true) % sizeof(double)) != 0)
+        return fm_set_error(FM_EINVAL, "MOP tallies would be misaligned in LDS");
     ktimer_begin(c, KT_FORCES);
     dim3 grid(brick_count(c)), blk(FM_BLOCK);
     const bool extra = poly || tm;

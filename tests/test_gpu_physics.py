@@ -1,0 +1,220 @@
+"""GPU parity for the remaining physics of the path: Mie types, FENE chains and bond lists,
+MOP planes, the per-molecule virial tensor, tethered/sliding walls, the rebuild criteria, the
+reference's own restart state, and the mixed-precision pair law (against the FP64 path)."""
+import sys
+
+import numpy as np
+import pytest
+
+from flowmol_amd import config
+from flowmol_amd._abi import FM_PREC_MIXED, FM_REBUILD_FIXED, FM_REBUILD_VMAX
+from flowmol_amd.md import MDContext
+from flowmol_amd.restart import read_final_state
+from oracle.oracle import Oracle
+from tests.helpers import canonical_pairs, rel_force_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+GOLDEN = "tests/golden/intrinsic_interface_initial_state"
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def drop_in_steps(o, g, it0, n):
+    """md_advance_vv routine by routine on both sides; returns the rebuild steps of each."""
+    ro, rg = [], []
+    for it in range(it0, it0 + n):
+        o.move_particles_vv(1)
+        o.updateborders(0)
+        if o.checkrebuild(it):
+            o.rebuild()
+            ro.append(it)
+        o.compute_forces(it)
+        o.move_particles_vv(2)
+        if g.md_advance_vv(it):
+            rg.append(it)
+    return ro, rg
+
+
+@pytest.fixture(scope="module")
+def polymer_state():
+    p, r, v, mt, mono = config.config_polymer(nunits=(6, 8, 6))
+    o = Oracle(p, r, v)
+    o.set_moltype(mt)
+    o.set_monomers(*mono)
+    o.setup()
+    o.compute_forces(0)
+    o.run_vv(1, 100)
+    o.rebuild()
+    r2, v2, a2 = o.state()
+    return p, r2, v2, a2, mt, mono
+
+
+def test_polymer_mie_bonds_lists_forces(polymer_state):
+    p, r, v, a, mt, mono = polymer_state
+    say("case: FENE + WCA/Mie14-6 melt 1152 beads")
+    o = Oracle(p, r, v, a)
+    o.set_moltype(mt)
+    o.set_monomers(*mono)
+    g = MDContext(p, r, v, a)
+    g.set_moltype(mt)
+    g.set_monomers(*mono)
+    o.setup()
+    g.setup(0)
+    oi, oj, os_ = o.neighbourlist()
+    gi, gj, gs = g.neighbourlist()
+    assert np.array_equal(canonical_pairs(oi, oj, os_), canonical_pairs(gi, gj, gs))
+    ob, obs = o.bonds()
+    gb, gbs = g.bonds()
+    canon = lambda b, s: sorted((i, int(x), int(y)) for i in range(b.shape[0]) for x, y in zip(b[i], s[i]) if x >= 0)
+    assert canon(ob, obs) == canon(gb, gbs)
+    assert sum(1 for row in gb for x in row if x >= 0) == 2 * (r.shape[0] - r.shape[0] // 8)
+    o.compute_forces(0)
+    _, _, ao = o.state()
+    _, _, ag = g.state()
+    assert rel_force_err(ag, ao, o.force_scale()) < TOL
+    peo, viro = o.molecular()
+    peg, virg = g.molecular()
+    np.testing.assert_allclose(peg, peo, rtol=TOL, atol=1e-12)
+    np.testing.assert_allclose(virg, viro, rtol=TOL, atol=1e-10)
+    mo, mg = o.macros(), g.evaluate_macroscopic_properties()
+    for k in ("potenergysum_LJ", "potenergysum_POLY", "virial"):
+        assert abs(mg[k] - mo[k]) <= 1e-11 * max(1.0, abs(mo[k])), k
+    g.close()
+
+
+def test_polymer_short_trajectory(polymer_state):
+    p, r, v, a, mt, mono = polymer_state
+    say("case: polymer 40-step drop-in trajectory")
+    o = Oracle(p, r, v, a)
+    o.set_moltype(mt)
+    o.set_monomers(*mono)
+    g = MDContext(p, r, v, a)
+    g.set_moltype(mt)
+    g.set_monomers(*mono)
+    o.setup()
+    o.compute_forces(0)
+    g.setup(0)
+    ro, rg = drop_in_steps(o, g, 1, 40)
+    assert ro == rg
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-9
+    g.close()
+
+
+def _lj(nunits, seed, steps=150, **kw):
+    p, r, v = config.config_fcc(nunits, 0.8, 2.5, seed=seed, **kw)
+    r = r + np.random.default_rng(seed).uniform(-1e-3, 1e-3, r.shape)
+    o = Oracle(p, r, v)
+    o.setup()
+    o.compute_forces(0)
+    o.run_vv(1, steps)
+    o.rebuild()
+    r2, v2, a2 = o.state()
+    return p, r2, v2, a2
+
+
+def test_mop_planes_and_virial_tensor():
+    p, r, v, a = _lj((8, 8, 8), seed=21, tplot=1, vflux_outflag=2, nplanes=16,
+                     planespacing=config.fcc_globaldomain((8, 8, 8), 0.8)[1] / 16, pressure_outflag=1)
+    say("case: MOP 16 planes + rfmol, 2048 LJ")
+    o = Oracle(p, r, v, a)
+    g = MDContext(p, r, v, a)
+    o.setup()
+    g.setup(0)   # iter 0 is a record step: one MOP tally
+    o.reset_mop()
+    o.compute_forces(0)
+    Po, Pg = o.mop(), g.mop()
+    assert np.abs(Po).max() > 1.0
+    assert np.abs(Pg - Po).max() <= 1e-11 * np.abs(Po).max()
+    rfo, rfg = o.rfmol(), g.rfmol()
+    assert np.abs(rfg - rfo).max() <= 1e-11 * np.abs(rfo).max()
+    # virial tensor trace equals the per-molecule virial (both sides of each pair)
+    _, virg = g.molecular()
+    assert np.allclose(np.trace(rfg, axis1=1, axis2=2), virg, rtol=1e-12, atol=1e-10)
+    g.close()
+
+
+def test_couette_walls_short_trajectory():
+    p, r, v, tag, rt = config.config_couette(nunits=(8, 8, 8), tplot=5, nplanes=16)
+    say("case: tethered walls, sliding top, 30 steps")
+    o = Oracle(p, r, v)
+    o.set_tags(tag, rt)
+    g = MDContext(p, r, v)
+    g.set_tags(tag, rt)
+    o.setup()
+    o.compute_forces(0)
+    g.setup(0)
+    _, _, ao = o.state()
+    _, _, ag = g.state()
+    assert rel_force_err(ag, ao, o.force_scale() + 1.0) < TOL
+    ro, rg = drop_in_steps(o, g, 1, 30)
+    assert ro == rg
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-9
+    assert np.abs(g.rtether() - o.rtether()).max() < 1e-12
+    top = tag == 6
+    assert np.allclose(g.rtether()[top, 0] - rt[top, 0], 30 * p.delta_t * 1.0 + p.delta_t, atol=1e-9)
+    Po, Pg = o.mop(), g.mop()
+    assert np.abs(Pg - Po).max() <= 1e-10 * max(1.0, np.abs(Po).max())
+    g.close()
+
+
+@pytest.mark.parametrize("crit", [FM_REBUILD_VMAX, FM_REBUILD_FIXED])
+def test_rebuild_criteria_trajectory(crit):
+    p, r, v, a = _lj((6, 6, 6), seed=23, tplot=1000, rebuild_criteria=crit, fixed_rebuild=7)
+    say(f"case: rebuild criterion {crit}, 60 steps")
+    o = Oracle(p, r, v, a)
+    g = MDContext(p, r, v, a)
+    o.setup()
+    o.compute_forces(0)
+    g.setup(0)
+    ro, rg = drop_in_steps(o, g, 1, 60)
+    assert ro == rg and len(ro) > 2
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-10
+    g.close()
+
+
+def test_reference_restart_state():
+    d = read_final_state(GOLDEN)
+    h = d["header"]
+    p = config.make_params(h["globaldomain"], h["density"], h["rcutoff"], delta=h["delta_rneighbr"],
+                           dt=h["delta_t"], tplot=h["tplot"])
+    say("case: reference final_state (liquid-vapour 2635)")
+    o = Oracle(p, d["r"], d["v"])
+    g = MDContext(p, d["r"], d["v"])
+    o.assign_to_cell()
+    g.assign_to_cell()
+    assert np.array_equal(o.cells(), g.cells())
+    o.updateborders(1)
+    o.assign_to_neighbourlist()
+    g.assign_to_neighbourlist()
+    oi, oj, os_ = o.neighbourlist()
+    gi, gj, gs = g.neighbourlist()
+    assert np.array_equal(canonical_pairs(oi, oj, os_), canonical_pairs(gi, gj, gs))
+    o.compute_forces(0)
+    g.simulation_compute_forces(0)
+    assert rel_force_err(g.state()[2], o.state()[2], o.force_scale()) < TOL
+    ro, rg = drop_in_steps(o, g, 1, 50)
+    assert ro == rg
+    g.close()
+
+
+def test_mixed_precision_error_vs_fp64():
+    """FP32 pair law: force error against the FP64 path stays at FP32 level."""
+    p, r, v, a = _lj((8, 8, 8), seed=25, tplot=10)
+    say("case: mixed precision forces, 2048 LJ")
+    g64 = MDContext(p, r, v, a)
+    g64.setup(0)
+    pm, _, _ = config.config_fcc((8, 8, 8), 0.8, 2.5, seed=25, tplot=10, precision=FM_PREC_MIXED)
+    g32 = MDContext(pm, r, v, a)
+    g32.setup(0)
+    a64, a32 = g64.state()[2], g32.state()[2]
+    rms = np.sqrt(((a32 - a64) ** 2).sum() / (a64 ** 2).sum())
+    assert rms < 1e-5
+    pe64, _ = g64.molecular()
+    pe32, _ = g32.molecular()
+    assert abs(pe32.sum() - pe64.sum()) < 1e-5 * abs(pe64.sum())
+    g64.close()
+    g32.close()
