@@ -38,7 +38,81 @@ def parse():
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU sample (after its setup)")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP event timing")
+    ap.add_argument("--workload", choices=["c2", "serve"], default="c2",
+                    help="c2: 1M-molecule LJ step throughput (default); serve: replica stream latency")
+    ap.add_argument("--replicas", type=int, default=48, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=6.0, help="serve: Poisson arrival rate (1/s)")
+    ap.add_argument("--replica-steps", type=int, default=2000, help="serve: steps per replica")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run a plain device-to-device copy of this size (PMC byte calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(mb):
+    """A plain device copy of known size through the HIP runtime (its blit kernel calibrates the
+    FETCH_SIZE / WRITE_SIZE counters)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mb * 2**20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def traffic_from_profile():
+    """HBM bytes per force launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), scaled
+    by the same counters on a copy of known size; see profiles/forces_c2_traffic.json."""
+    path = os.path.join(ROOT, "profiles", "forces_c2_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("bytes_per_launch")
+
+
+def serve_main(a):
+    import numpy as np
+
+    from flowmol_amd.serve import ReplicaServer, poisson_workload
+
+    reqs = poisson_workload(a.replicas, a.rate, seed=5, nsteps=a.replica_steps)
+    # warm-up: one small replica through the same path (kernel loading, graph capture)
+    warm = poisson_workload(1, 100.0, seed=99, nmin=4096, nmax=4096, nsteps=50)
+    srv = ReplicaServer(chunk=50)
+    srv.serve(warm, realtime=False)
+    drift = []
+
+    def on_result(res):
+        drift.append(res["macros"]["totenergy"])
+
+    results, stats = srv.serve(reqs, realtime=True, on_result=on_result)
+    srv.close()
+    sizes = [q.np for q in reqs]
+    out = {
+        "metric": "ensemble replica p50/p99 latency (Couette NEMD replica stream, 1 GPU)",
+        "value": stats["p99_latency_s"],
+        "unit": "s (p99 replica latency)",
+        "higher_is_better": False,
+        "n_gpus": 1,
+        "p50_latency_s": stats["p50_latency_s"],
+        "p99_latency_s": stats["p99_latency_s"],
+        "aggregate_atom_steps_per_s": stats["atom_steps_per_s"],
+        "wall_s": stats["wall_s"],
+        "replicas": stats["replicas"],
+        "dtype": "f64",
+        "data": "synthetic",
+        "vs_baseline": None,
+        "config": {"workload": f"{a.replicas} Couette replicas, Poisson arrivals rate {a.rate}/s (seed 5), "
+                               f"sizes log-uniform 4096-262144 (FCC cubes), U in [0.5,2.0], "
+                               f"{a.replica_steps} steps each, MOP 16 planes every 10 steps, FP64",
+                   "mean_molecules": float(np.mean(sizes)), "chunk_steps": 50},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(p, r, v, steps):
@@ -63,6 +137,11 @@ def main():
         print(f"bench.py measures one GPU only (the path has no multi-GPU form here); got --gpus {a.gpus}",
               file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration_mb:
+        copy_calibration(a.copy_calibration_mb)
+    if a.workload == "serve":
+        serve_main(a)
+        return
     import numpy as np
 
     from flowmol_amd import config
@@ -107,7 +186,7 @@ def main():
         tfl = flops_per / avg_s / 1e12
         peak_tf = FP64_VALU_PEAK_TFLOPS if prec == FM_PREC_FP64 else FP32_VALU_PEAK_TFLOPS
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
                 "kernel": "forces", "avg_ms": round(kt["forces"][0], 5), "launches": kt["forces"][1],
                 "bytes_per_launch": round(bytes_per), "flops_per_launch": round(flops_per),
                 "valu": {"achieved": round(tfl, 2), "peak": peak_tf, "unit": "TFLOP/s",

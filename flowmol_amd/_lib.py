@@ -49,6 +49,15 @@ SIGNATURES = {
     "fm_download_cells": (C.c_int, [_vp, _ip]),
     "fm_download_neighbourlist": (C.c_int, [_vp, _ip, _ip, _ip, _i64, _lp]),
     "fm_download_bonds": (C.c_int, [_vp, _ip, _ip]),
+    "fm_batch_create": (C.c_int, [C.POINTER(_vp)]),
+    "fm_batch_destroy": (None, [_vp]),
+    "fm_batch_use_graphs": (C.c_int, [_vp, _i32]),
+    "fm_batch_add": (C.c_int, [_vp, _vp, _i32]),
+    "fm_batch_remove": (C.c_int, [_vp, _vp]),
+    "fm_batch_run": (C.c_int, [_vp, _i32, _ip]),
+    "fm_batch_size": (C.c_int, [_vp, _ip]),
+    "fm_batch_get_iter": (C.c_int, [_vp, _vp, _ip]),
+    "fm_memory_footprint": (C.c_int, [C.POINTER(FmParams), _i32, _lp]),
     "fm_synchronize": (C.c_int, [_vp]),
     "fm_kernel_timing": (C.c_int, [_vp, _i32]),
     "fm_get_kernel_timing": (C.c_int, [_vp, C.c_char_p, _i32, _dp, _lp, _i32, _ip]),

@@ -85,11 +85,6 @@ void ktimer_end(fm_ctx* c, int cls) {
 // ---- helpers ----------------------------------------------------------------------------------
 #define CHECK_CTX(c) \
     if (!(c)) return fm_set_error(FM_EINVAL, "null context")
-#define TRY(x)              \
-    do {                    \
-        int _rc = (x);      \
-        if (_rc) return _rc; \
-    } while (0)
 
 template <typename T>
 static int dalloc(T** p, size_t n) {
@@ -121,6 +116,8 @@ static int sync_flags(fm_ctx* c) {
     }
     return 0;
 }
+
+int ctx_sync_flags(fm_ctx* c) { return sync_flags(c); }
 
 static void free_ctx(fm_ctx* c) {
     if (!c) return;
@@ -356,6 +353,40 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
         return fm_set_error(FM_EHIP, "initialisation failed");
     }
     *out = c;
+    return 0;
+}
+
+// Device bytes fm_init allocates for these parameters (admission control by HBM footprint).
+extern "C" int fm_memory_footprint(const fm_params* p, int32_t np, int64_t* bytes) {
+    if (!p || !bytes || np < 0) return fm_set_error(FM_EINVAL, "bad argument");
+    int nc[3];
+    for (int d = 0; d < 3; ++d) {
+        nc[d] = (int)floor(p->domain[d] / (p->rcutoff + p->delta_rneighbr[d]));
+        if (nc[d] < 3) return fm_set_error(FM_EINVAL, "fewer than 3 cells");
+    }
+    const size_t ncell = (size_t)nc[0] * nc[1] * nc[2];
+    const size_t cap = std::max(np, 1);
+    double rneighbr = p->rcutoff + (p->delta_rneighbr[0] + p->delta_rneighbr[1] + p->delta_rneighbr[2]) / 3.0;
+    if (p->potential_flag == 1 && rneighbr < p->R_0) rneighbr = p->R_0;
+    int maxnb = p->max_neighbours;
+    if (maxnb <= 0) {
+        const double vol = p->domain[0] * p->domain[1] * p->domain[2];
+        const double dens = std::max(p->density, np / vol);
+        maxnb = (int)(1.5 * (4.0 / 3.0 * M_PI * rneighbr * rneighbr * rneighbr * dens)) + 32;
+    }
+    maxnb = (maxnb + 3) & ~3;
+    size_t b = 0;
+    b += 2 * 9 * cap * 8 + 3 * cap * 8 + 9 * cap * 8;          // state copies, rdisp, staging
+    b += (2 + 1 + 1 + 1 + 1 + 3) * cap * 4;                     // ext, slot_of, cell_of, perm, nbn, sendmols scratch
+    b += 3 * cap * 8;                                           // pe_lj, pe_poly, vir
+    b += (size_t)(maxnb / 4) * cap * 8;                         // neighbour words
+    b += 3 * (ncell + 1) * 4;                                   // cell counts, starts, cursors
+    b += cap * 4 + 4 * 1024 * 1024;                             // scan/select temporary, reductions
+    if (p->potential_flag == 1) b += FM_MAX_FUNCY * cap * 4 + cap * 4 + 2 * 7 * cap * 4;
+    if (p->mie_potential == 1) b += 2 * cap * 4;
+    if (p->ensemble == FM_ENSEMBLE_TAG_MOVE) b += 2 * cap * 4 + 2 * 6 * cap * 8;
+    if (p->pressure_outflag == 1) b += 9 * cap * 8;
+    *bytes = (int64_t)b;
     return 0;
 }
 
@@ -720,6 +751,8 @@ static int rebuild_lists(fm_ctx* c) {
     TRY(fm_assign_to_cell(c));
     return fm_assign_to_neighbourlist(c);
 }
+
+int rebuild_lists_internal(fm_ctx* c) { return rebuild_lists(c); }
 
 extern "C" int fm_run_vv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* macros, int32_t n_macros,
                          int32_t* n_rebuilds) {

@@ -14,6 +14,7 @@
 
 #include <limits.h>
 #include <math.h>
+#include <string.h>
 
 #pragma clang fp contract(off)
 
@@ -437,14 +438,13 @@ __device__ inline bool mop_owner(const Region& r, const BG& g, int rci, int rcj,
 }
 
 template <bool MIE, bool RECORD, bool EXTRA>
-__global__ __launch_bounds__(FM_BLOCK) void k_forces_brick(FbArgs a) {
-    extern __shared__ __align__(16) char smem[];
+__device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* smem) {
     Region r;
     const bool mop = RECORD && a.pxy != nullptr;
     carve(r, smem, a.g, a.maxreg, mop || MIE);
     double* spxy = nullptr;
     if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
-    stage_region(r, a.g, blockIdx.x, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr);
+    stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr);
     if (mop) {
         for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
         __syncthreads();
@@ -469,10 +469,14 @@ __global__ __launch_bounds__(FM_BLOCK) void k_forces_brick(FbArgs a) {
         }
         const int exti = mop ? a.ext[i] : 0;
         const uint64_t* row = a.nbl + i;
-        uint64_t word = n > 0 ? row[0] : 0;
-        for (int k = 0; k < n; ++k) {
-            if ((k & 3) == 0 && k > 0) word = row[(size_t)(k >> 2) * cap];
-            const int j = (int)((word >> (16 * (k & 3))) & 0xffffu);
+        // four 16-bit entries per word: unrolled so the shifts are constants and the four
+        // LDS reads of a word can be in flight together
+        for (int kw = 0; kw < n; kw += 4) {
+            const uint64_t word = row[(size_t)(kw >> 2) * cap];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+            if (kw + q >= n) break;
+            const int j = (int)((word >> (16 * q)) & 0xffffu);
             const double rx = xi - r.lx[j], ry = yi - r.ly[j], rz = zi - r.lz[j];
             const double r2 = (rx * rx + ry * ry) + rz * rz;
             double rc2 = a.rc2;
@@ -519,6 +523,7 @@ __global__ __launch_bounds__(FM_BLOCK) void k_forces_brick(FbArgs a) {
                     }
                 }
             }
+        }
         }
         if (EXTRA && a.bondn) {  // FENE, simulation_compute_forces_poly
             const int nb = a.bondn[i];
@@ -587,17 +592,32 @@ __global__ __launch_bounds__(FM_BLOCK) void k_forces_brick(FbArgs a) {
     }
 }
 
+template <bool MIE, bool RECORD, bool EXTRA>
+__global__ __launch_bounds__(FM_BLOCK) void k_forces_brick(FbArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    forces_body<MIE, RECORD, EXTRA>(a, blockIdx.x, smem);
+}
+
+// packed launch over many domains: block b works on brick table[b].y of domain table[b].x
+template <bool MIE, bool EXTRA>
+__global__ __launch_bounds__(FM_BLOCK) void k_forces_brick_multi(const FbArgs* descs, const int2* table) {
+    extern __shared__ __align__(16) char smem[];
+    const int2 t = table[blockIdx.x];
+    const FbArgs& a = descs[t.x];
+    if (a.rec) forces_body<MIE, true, EXTRA>(a, t.y, smem);
+    else forces_body<MIE, false, EXTRA>(a, t.y, smem);
+}
+
 // Mixed precision: image coordinates and differences in FP64, the pair law and the per-molecule
 // force sum in FP32; energy, virial and the tallies accumulated in FP64.
 template <bool RECORD, bool EXTRA>
-__global__ __launch_bounds__(FM_BLOCK) void k_forces_brick_mixed(FbArgs a) {
-    extern __shared__ __align__(16) char smem[];
+__device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, char* smem) {
     Region r;
     const bool mop = RECORD && a.pxy != nullptr;
     carve(r, smem, a.g, a.maxreg, mop);
     double* spxy = nullptr;
     if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
-    stage_region(r, a.g, blockIdx.x, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr);
+    stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr);
     if (mop) {
         for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
         __syncthreads();
@@ -613,10 +633,14 @@ __global__ __launch_bounds__(FM_BLOCK) void k_forces_brick_mixed(FbArgs a) {
         double pe = 0.0, vir = 0.0;
         const int exti = mop ? a.ext[i] : 0;
         const uint64_t* row = a.nbl + i;
-        uint64_t word = n > 0 ? row[0] : 0;
-        for (int k = 0; k < n; ++k) {
-            if ((k & 3) == 0 && k > 0) word = row[(size_t)(k >> 2) * cap];
-            const int j = (int)((word >> (16 * (k & 3))) & 0xffffu);
+        // four 16-bit entries per word: unrolled so the shifts are constants and the four
+        // LDS reads of a word can be in flight together
+        for (int kw = 0; kw < n; kw += 4) {
+            const uint64_t word = row[(size_t)(kw >> 2) * cap];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+            if (kw + q >= n) break;
+            const int j = (int)((word >> (16 * q)) & 0xffffu);
             const double rxd = xi - r.lx[j], ryd = yi - r.ly[j], rzd = zi - r.lz[j];
             const float rx = (float)rxd, ry = (float)ryd, rz = (float)rzd;
             float acc, inv;
@@ -644,6 +668,7 @@ __global__ __launch_bounds__(FM_BLOCK) void k_forces_brick_mixed(FbArgs a) {
                     }
                 }
             }
+        }
         }
         double dfx = fx, dfy = fy, dfz = fz;
         if (EXTRA && a.tag) {
@@ -678,6 +703,21 @@ __global__ __launch_bounds__(FM_BLOCK) void k_forces_brick_mixed(FbArgs a) {
     }
 }
 
+template <bool RECORD, bool EXTRA>
+__global__ __launch_bounds__(FM_BLOCK) void k_forces_brick_mixed(FbArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    forces_body_mixed<RECORD, EXTRA>(a, blockIdx.x, smem);
+}
+
+template <bool EXTRA>
+__global__ __launch_bounds__(FM_BLOCK) void k_forces_brick_mixed_multi(const FbArgs* descs, const int2* table) {
+    extern __shared__ __align__(16) char smem[];
+    const int2 t = table[blockIdx.x];
+    const FbArgs& a = descs[t.x];
+    if (a.rec) forces_body_mixed<true, EXTRA>(a, t.y, smem);
+    else forces_body_mixed<false, EXTRA>(a, t.y, smem);
+}
+
 // ---- decode the LDS-indexed list into (slot << 5 | shift code) for inspection ------------------
 __global__ __launch_bounds__(FM_BLOCK) void k_decode_brick(St s, BG g, const int* start, const uint64_t* nbl,
                                                            const int* nbn, int cap, int maxreg, uint32_t* out) {
@@ -697,25 +737,27 @@ __global__ __launch_bounds__(FM_BLOCK) void k_decode_brick(St s, BG g, const int
     }
 }
 
-// largest staged region (LDS sizing) and largest brick interior
+// largest staged region (LDS sizing): one wavefront per brick sums its region cells
 __global__ __launch_bounds__(FM_BLOCK) void k_region_max(BG g, const int* start, int nbrick, int* out) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbrick) return;
+    const int b = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (b >= nbrick) return;  // whole wavefront
     const int bx = b % g.nb[0], by = (b / g.nb[0]) % g.nb[1], bz = b / (g.nb[0] * g.nb[1]);
     const int org[3] = {bx * g.B[0], by * g.B[1], bz * g.B[2]};
     int tot = 0;
-    for (int rz = 0; rz < g.R[2]; ++rz)
-        for (int ry = 0; ry < g.R[1]; ++ry)
-            for (int rx = 0; rx < g.R[0]; ++rx) {
-                int v[3] = {org[0] + rx - 1, org[1] + ry - 1, org[2] + rz - 1};
-                for (int d = 0; d < 3; ++d) {
-                    if (v[d] < 0) v[d] += g.nc[d];
-                    else if (v[d] >= g.nc[d]) v[d] -= g.nc[d];
-                }
-                const int c = v[0] + g.nc[0] * (v[1] + g.nc[1] * v[2]);
-                tot += start[c + 1] - start[c];
-            }
-    if (tot > __atomic_load_n(out, __ATOMIC_RELAXED)) atomicMax(out, tot);
+    for (int t = lane; t < g.nreg; t += 64) {
+        const int rr[3] = {t % g.R[0], (t / g.R[0]) % g.R[1], t / (g.R[0] * g.R[1])};
+        int v[3];
+        for (int d = 0; d < 3; ++d) {
+            v[d] = org[d] + rr[d] - 1;
+            if (v[d] < 0) v[d] += g.nc[d];
+            else if (v[d] >= g.nc[d]) v[d] -= g.nc[d];
+        }
+        const int c = v[0] + g.nc[0] * (v[1] + g.nc[1] * v[2]);
+        tot += start[c + 1] - start[c];
+    }
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
+    if (lane == 0 && tot > __atomic_load_n(out, __ATOMIC_RELAXED)) atomicMax(out, tot);
 }
 
 inline int nblk(size_t n) { return (int)((n + FM_BLOCK - 1) / FM_BLOCK); }
@@ -762,7 +804,8 @@ size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind) {
 int launch_region_max(fm_ctx* c) {
     BG g = make_bg(c);
     int nbrick = g.nb[0] * g.nb[1] * g.nb[2];
-    k_region_max<<<nblk(nbrick), FM_BLOCK, 0, c->stream>>>(g, c->cell_start, nbrick, c->d_flags + FLAG_MAX_REGION);
+    k_region_max<<<nblk((size_t)nbrick * 64), FM_BLOCK, 0, c->stream>>>(g, c->cell_start, nbrick,
+                                                                          c->d_flags + FLAG_MAX_REGION);
     LAUNCH_CHECK();
     return 0;
 }
@@ -796,8 +839,11 @@ int launch_nlist(fm_ctx* c) {
     return 0;
 }
 
-int launch_forces(fm_ctx* c, int iter, bool kick) {
-    if (c->np == 0) return 0;
+size_t fargs_bytes() { return sizeof(FbArgs); }
+
+// Arguments of the force kernel for one domain at step iter.  kind: bit0 mixed precision,
+// bit1 Mie tables, bit2 bonds or tethers present.
+int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* kind_out) {
     BG g = make_bg(c);
     FbArgs a;
     const int cur = c->cur;
@@ -847,15 +893,30 @@ int launch_forces(fm_ctx* c, int iter, bool kick) {
     a.mie = mie ? c->d_mie : nullptr;
     a.kick = kick ? 1 : 0;
     a.hdt = 0.5 * c->p.delta_t;
-    const size_t lds = brick_lds_bytes(c, c->maxreg, mop ? 2 : 1);
+    // LDS: the MOP layout whenever MOP is enabled, so that one launch can mix record steps
+    const bool mop_enabled = c->p.vflux_outflag >= 1 && c->p.vflux_outflag <= 3 && c->pxy;
+    const size_t lds = brick_lds_bytes(c, c->maxreg, mop_enabled ? 2 : 1);
     if (mop && (region_lds_bytes(g, c->maxreg, true) % sizeof(double)) != 0)
         return fm_set_error(FM_EINVAL, "MOP tallies would be misaligned in LDS");
+    memcpy(out, &a, sizeof a);
+    if (lds_out) *lds_out = lds;
+    if (kind_out) *kind_out = (c->p.precision == FM_PREC_MIXED && !mie && !poly ? 1 : 0) | (mie ? 2 : 0) |
+                              (poly || tm ? 4 : 0);
+    return 0;
+}
+
+int launch_forces(fm_ctx* c, int iter, bool kick) {
+    if (c->np == 0) return 0;
+    FbArgs a;
+    size_t lds = 0;
+    int kind = 0;
+    if (int rc = make_fargs(c, iter, kick, &a, &lds, &kind)) return rc;
+    const bool mixed = kind & 1, mie = kind & 2, extra = kind & 4;
     ktimer_begin(c, KT_FORCES);
     dim3 grid(brick_count(c)), blk(FM_BLOCK);
-    const bool extra = poly || tm;
 #define FB(M, R, E) k_forces_brick<M, R, E><<<grid, blk, lds, c->stream>>>(a)
 #define FM(R, E) k_forces_brick_mixed<R, E><<<grid, blk, lds, c->stream>>>(a)
-    if (c->p.precision == FM_PREC_MIXED && !mie && !poly) {
+    if (mixed) {
         if (extra) { if (a.rec) FM(true, true); else FM(false, true); }
         else { if (a.rec) FM(true, false); else FM(false, false); }
     } else if (mie) {
@@ -868,6 +929,24 @@ int launch_forces(fm_ctx* c, int iter, bool kick) {
 #undef FB
 #undef FM
     ktimer_end(c, KT_FORCES);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_forces_multi(hipStream_t st, int kind, const void* d_descs, const int2* d_table, int nblk, size_t lds) {
+    if (nblk == 0) return 0;
+    const FbArgs* d = (const FbArgs*)d_descs;
+    const bool mixed = kind & 1, mie = kind & 2, extra = kind & 4;
+    if (mixed) {
+        if (extra) k_forces_brick_mixed_multi<true><<<nblk, FM_BLOCK, lds, st>>>(d, d_table);
+        else k_forces_brick_mixed_multi<false><<<nblk, FM_BLOCK, lds, st>>>(d, d_table);
+    } else if (mie) {
+        if (extra) k_forces_brick_multi<true, true><<<nblk, FM_BLOCK, lds, st>>>(d, d_table);
+        else k_forces_brick_multi<true, false><<<nblk, FM_BLOCK, lds, st>>>(d, d_table);
+    } else {
+        if (extra) k_forces_brick_multi<false, true><<<nblk, FM_BLOCK, lds, st>>>(d, d_table);
+        else k_forces_brick_multi<false, false><<<nblk, FM_BLOCK, lds, st>>>(d, d_table);
+    }
     LAUNCH_CHECK();
     return 0;
 }

@@ -131,6 +131,12 @@ int fm_set_error(int code, const char* fmt, ...);
                                 hipGetErrorString(_e), __FILE__, __LINE__);            \
     } while (0)
 
+#define TRY(x)               \
+    do {                     \
+        int _rc = (x);       \
+        if (_rc) return _rc; \
+    } while (0)
+
 // ---- kernel timing (HIP events on the context stream) -------------------------------------------
 void ktimer_begin(fm_ctx* c, int cls);
 void ktimer_end(fm_ctx* c, int cls);
@@ -156,6 +162,22 @@ int launch_ext_to_slot_i32(fm_ctx* c, int* dst, const int* src, int nf);
 int launch_ext_to_slot_f64(fm_ctx* c, double* dst, const double* src, int nf);
 int launch_slot_to_ext_f64(fm_ctx* c, double* dst, const double* src, int nf);
 int launch_slot_to_ext_i32(fm_ctx* c, int* dst, const int* src, int nf);
+
+// packed launches over many domains (replica batches)
+struct P1Desc {               // VV pass 1 (+ displacement check) of one domain
+    double* st;               // state block (9*cap doubles)
+    double* rd;               // rdisp (3*cap)
+    int* flag_out;            // rebuild flag of this domain
+    int np, cap;
+    int check, fresh;         // apply the displacement update / start it from zero
+    double dt, hdt2, hdt, thresh;
+};
+int launch_pass1_multi(hipStream_t st, const P1Desc* d_descs, const int2* d_table, int nblk);
+size_t fargs_bytes();
+// force-kernel arguments of one domain for step iter; kind: bit0 mixed, bit1 Mie, bit2 bonds/tethers
+int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds, int* kind);
+int launch_forces_multi(hipStream_t st, int kind, const void* d_descs, const int2* d_table, int nblk, size_t lds);
+int rebuild_lists_internal(fm_ctx* c);
 
 // LDS-brick kernels (fm_brick.hip)
 int brick_count(const fm_ctx* c);

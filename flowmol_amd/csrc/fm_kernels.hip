@@ -262,55 +262,73 @@ __global__ __launch_bounds__(1024) void k_sendmols_passes(St s, double* teth, co
     }
 }
 
+// Lanes of a wavefront that add to the same counter are served by one atomic: returns each
+// active lane's position among the additions to its counter (consecutive slots mostly share a
+// cell, so a wavefront issues a handful of atomics instead of 64).
+__device__ inline int wave_aggregated_add(int* counters, int key, bool active) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long mask = __ballot(active);
+    int pos = -1;
+    while (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        const int lk = __shfl(key, leader, 64);
+        const unsigned long long same = __ballot(active && key == lk) & mask;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counters[lk], __popcll(same));
+        base = __shfl(base, leader, 64);
+        if ((same >> lane) & 1ull) pos = base + __popcll(same & ((1ull << lane) - 1ull));
+        mask &= ~same;
+    }
+    return pos;
+}
+
 // ---- assign_to_cell as a counting sort, src/linklist.f90:2645-2676 -----------------------------
 __global__ __launch_bounds__(FM_BLOCK) void k_bin_count(St s, const int* ext, int np, double hx, double hy,
                                                         double hz, double csx, double csy, double csz,
                                                         int nx, int ny, int nz, int* cell_of,
                                                         int* count, int* flags) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    double r[3] = {s.x[i], s.y[i], s.z[i]};
-    const double h[3] = {hx, hy, hz}, cs[3] = {csx, csy, csz};
-    const int n[3] = {nx, ny, nz};
-    int c[3];
-    for (int d = 0; d < 3; ++d) {
-        double t = r[d] + h[d];
-        if (!(t > -cs[d] && t < (double)(n[d] + 1) * cs[d])) {  // also rejects NaN
-            set_err(flags, FM_EESCAPE, ext[i]);
-            return;
-        }
-        int ic = (int)ceil(t / cs[d]) + 1;  // 1-based with one halo layer
-        if (ic < 1 || ic > n[d] + 2) {
-            set_err(flags, FM_EESCAPE, ext[i]);
-            return;
-        }
-        if (ic == 1) {
-            if (t == 0.0) {  // exactly on the lower face: the reference's ceiling() gives the halo
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int cell = -1;
+    if (i < np) {
+        double r[3] = {s.x[i], s.y[i], s.z[i]};
+        const double h[3] = {hx, hy, hz}, cs[3] = {csx, csy, csz};
+        const int n[3] = {nx, ny, nz};
+        int c[3];
+        bool ok = true;
+        for (int d = 0; d < 3 && ok; ++d) {
+            double t = r[d] + h[d];
+            if (!(t > -cs[d] && t < (double)(n[d] + 1) * cs[d])) {  // also rejects NaN
+                ok = false;
+                break;
+            }
+            int ic = (int)ceil(t / cs[d]) + 1;  // 1-based with one halo layer
+            if (ic == 1 && t == 0.0) {  // exactly on the lower face: the reference's ceiling() gives the halo
                 atomicAdd(&flags[FLAG_EDGE], 1);
                 ic = 2;
-            } else {
-                set_err(flags, FM_EESCAPE, ext[i]);
-                return;
             }
+            if (ic < 2 || ic > n[d] + 1) {
+                ok = false;
+                break;
+            }
+            c[d] = ic - 2;
         }
-        if (ic == n[d] + 2) {
+        if (ok) {
+            cell = c[0] + nx * (c[1] + ny * c[2]);
+            cell_of[i] = cell;
+        } else {
             set_err(flags, FM_EESCAPE, ext[i]);
-            return;
         }
-        c[d] = ic - 2;
     }
-    int cell = c[0] + nx * (c[1] + ny * c[2]);
-    cell_of[i] = cell;
-    atomicAdd(&count[cell], 1);
+    wave_aggregated_add(count, cell, cell >= 0);
 }
 
 __global__ __launch_bounds__(FM_BLOCK) void k_bin_scatter(const int* cell_of, const int* start, int* cursor,
                                                           int* perm, int np) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    int c = cell_of[i];
-    int pos = start[c] + atomicAdd(&cursor[c], 1);
-    perm[pos] = i;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < np;
+    const int c = active ? cell_of[i] : 0;
+    const int pos = wave_aggregated_add(cursor, c, active);
+    if (active) perm[start[c] + pos] = i;
 }
 
 // order each cell by increasing x (ties: molecule number), which lets the list build window
@@ -493,6 +511,46 @@ __global__ __launch_bounds__(FM_BLOCK) void k_slot_to_ext(T* dst, const T* src, 
     if (s >= np) return;
     size_t e = ext[s];
     for (int f = 0; f < nf; ++f) dst[e * nf + f] = src[s + (size_t)f * cap];
+}
+
+// packed VV pass 1 (+ displacement check) over many domains: block b handles molecules
+// table[b].y*FM_BLOCK .. of domain table[b].x (same operations as k_pass1 / k_pass1_disp)
+__global__ __launch_bounds__(FM_BLOCK) void k_pass1_multi(const P1Desc* descs, const int2* table) {
+    const int2 t = table[blockIdx.x];
+    const P1Desc& d = descs[t.x];
+    const int i = t.y * FM_BLOCK + threadIdx.x;
+    bool hit = false;
+    if (i < d.np) {
+        St s = state_of(d.st, d.cap);
+        const double ax = s.ax[i], ay = s.ay[i], az = s.az[i];
+        double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+        s.x[i] = s.x[i] + d.dt * vx + d.hdt2 * ax;
+        s.y[i] = s.y[i] + d.dt * vy + d.hdt2 * ay;
+        s.z[i] = s.z[i] + d.dt * vz + d.hdt2 * az;
+        vx = vx + d.hdt * ax;
+        vy = vy + d.hdt * ay;
+        vz = vz + d.hdt * az;
+        s.vx[i] = vx;
+        s.vy[i] = vy;
+        s.vz[i] = vz;
+        if (d.check) {
+            const size_t cap = d.cap;
+            double d0 = d.fresh ? 0.0 : d.rd[i];
+            double d1 = d.fresh ? 0.0 : d.rd[i + cap];
+            double d2 = d.fresh ? 0.0 : d.rd[i + 2 * cap];
+            d0 = d0 + vx * d.dt;
+            d1 = d1 + vy * d.dt;
+            d2 = d2 + vz * d.dt;
+            d.rd[i] = d0;
+            d.rd[i + cap] = d1;
+            d.rd[i + 2 * cap] = d2;
+            hit = fabs(d0) > d.thresh || fabs(d1) > d.thresh || fabs(d2) > d.thresh;
+        }
+    }
+    unsigned long long m = __ballot(hit);
+    if (m != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1 &&
+        __atomic_load_n(d.flag_out, __ATOMIC_RELAXED) == 0)
+        atomicOr(d.flag_out, 1);
 }
 
 }  // namespace
@@ -696,5 +754,12 @@ int launch_pass1_check_disp(fm_ctx* c) {
     ktimer_end(c, KT_PASS1);
     LAUNCH_CHECK();
     c->rdisp_valid = true;
+    return 0;
+}
+
+int launch_pass1_multi(hipStream_t st, const P1Desc* d_descs, const int2* d_table, int nblk) {
+    if (nblk == 0) return 0;
+    k_pass1_multi<<<nblk, FM_BLOCK, 0, st>>>(d_descs, d_table);
+    LAUNCH_CHECK();
     return 0;
 }

@@ -1,0 +1,198 @@
+"""Replica serving: many independent NEMD/NVE replicas of uneven size on one GPU.
+
+Continuous batching over libflowmol_hip's replica batches (fm_batch_*):
+  * a request is one replica (parameters, initial state, walls, number of steps) with an arrival
+    time;
+  * the loop admits every request that has arrived while the device footprint of the admitted
+    replicas (fm_memory_footprint) stays within the HBM budget;
+  * admitted replicas keep their state on the device and advance together, every step one packed
+    launch for all of them, in chunks of at most `chunk` steps;
+  * a finished replica leaves at the end of its chunk, its results are read back and its
+    memory is released, so waiting requests can enter at the next chunk boundary.
+Latency of a replica = time its results are available - its arrival time.
+"""
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from . import config
+from ._abi import FmParams
+from ._lib import check, load
+from .md import MDContext
+
+
+def memory_footprint(params, np_):
+    b = C.c_int64()
+    check(load().fm_memory_footprint(C.byref(params), int(np_), C.byref(b)))
+    return b.value
+
+
+class Batch:
+    """Contexts advanced together by packed launches (fm_batch)."""
+
+    def __init__(self, graphs=True):
+        h = C.c_void_p()
+        check(load().fm_batch_create(C.byref(h)))
+        self._h = h
+        self.members = []
+        if not graphs:
+            check(load().fm_batch_use_graphs(self._h, 0))
+
+    def add(self, ctx, iter0):
+        check(load().fm_batch_add(self._h, ctx._h, int(iter0)))
+        self.members.append(ctx)
+
+    def remove(self, ctx):
+        check(load().fm_batch_remove(self._h, ctx._h))
+        self.members.remove(ctx)
+
+    def run(self, nsteps):
+        nrb = C.c_int32()
+        check(load().fm_batch_run(self._h, int(nsteps), C.byref(nrb)))
+        return nrb.value
+
+    def iter_of(self, ctx):
+        it = C.c_int32()
+        check(load().fm_batch_get_iter(self._h, ctx._h, C.byref(it)))
+        return it.value
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            for c in list(self.members):
+                self.remove(c)
+            load().fm_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Request:
+    def __init__(self, rid, arrival, params, r, v, nsteps, tag=None, rtether=None):
+        self.rid = rid
+        self.arrival = float(arrival)
+        self.params = params
+        self.r = r
+        self.v = v
+        self.tag = tag
+        self.rtether = rtether
+        self.nsteps = int(nsteps)
+        self.np = r.shape[0]
+        self.footprint = memory_footprint(params, self.np)
+
+
+def couette_request(rid, arrival, n_target, wall_speed, nsteps, seed, density=0.8, tplot=10, precision=0):
+    """One Couette NEMD replica: FCC cube of 4*n^3 molecules (n from the target size), tethered
+    walls 3 units thick, top wall sliding at wall_speed, MOP on 16 planes every record step."""
+    n = max(7, int(round((n_target / 4.0) ** (1.0 / 3.0))))
+    p, r, v, tag, rt = config.config_couette(nunits=(n, n, n), density=density, wallslidev=wall_speed,
+                                             nplanes=16, seed=seed, tplot=tplot, precision=precision)
+    return Request(rid, arrival, p, r, v, nsteps, tag=tag, rtether=rt)
+
+
+def poisson_workload(n_requests, rate, seed=5, nmin=4096, nmax=262144, nsteps=2000, umin=0.5, umax=2.0,
+                     precision=0):
+    """Seeded Poisson arrivals; sizes log-uniform in [nmin, nmax]; wall speed uniform in
+    [umin, umax]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for k in range(n_requests):
+        t += rng.exponential(1.0 / rate)
+        size = math.exp(rng.uniform(math.log(nmin), math.log(nmax)))
+        u = rng.uniform(umin, umax)
+        reqs.append(couette_request(k, t, size, u, nsteps, seed=1000 + k, precision=precision))
+    return reqs
+
+
+class Active:
+    def __init__(self, req, ctx, t_admit):
+        self.req = req
+        self.ctx = ctx
+        self.remaining = req.nsteps
+        self.t_admit = t_admit
+
+
+def make_context(req):
+    ctx = MDContext(req.params, req.r, req.v)
+    if req.tag is not None:
+        ctx.set_tags(req.tag, req.rtether)
+    ctx.setup(0)
+    return ctx
+
+
+def result_of(ctx):
+    m = ctx.evaluate_macroscopic_properties()
+    out = {"macros": m}
+    if ctx.params.vflux_outflag in (1, 2, 3):
+        out["mop"] = ctx.mop()
+    return out
+
+
+class ReplicaServer:
+    """Continuous-batching request loop for replicas (see the module docstring)."""
+
+    def __init__(self, hbm_budget_bytes=200 * 2**30, chunk=50, graphs=True):
+        self.budget = int(hbm_budget_bytes)
+        self.chunk = int(chunk)
+        self.batch = Batch(graphs=graphs)
+
+    def serve(self, requests, realtime=True, on_result=None):
+        """Run the requests to completion.  realtime: arrivals follow the wall clock from the
+        start of the call; otherwise every request counts as arrived at t=0."""
+        pending = sorted(requests, key=lambda q: q.arrival)
+        active = []
+        used = 0
+        results = {}
+        t0 = time.perf_counter()
+        atom_steps = 0
+        while pending or active:
+            now = time.perf_counter() - t0
+            # admit what has arrived, in arrival order, while the budget allows
+            while pending and (not realtime or pending[0].arrival <= now) and used + pending[0].footprint <= self.budget:
+                q = pending.pop(0)
+                ctx = make_context(q)
+                self.batch.add(ctx, 1)
+                used += q.footprint
+                active.append(Active(q, ctx, time.perf_counter() - t0))
+            if not active:
+                if pending and realtime:
+                    time.sleep(max(0.0, min(pending[0].arrival - now, 0.01)))
+                    continue
+                if pending:
+                    raise RuntimeError("a request does not fit the HBM budget on its own")
+                break
+            k = min(self.chunk, min(a.remaining for a in active))
+            self.batch.run(k)
+            atom_steps += k * sum(a.req.np for a in active)
+            done = [a for a in active if a.remaining - k == 0]
+            for a in active:
+                a.remaining -= k
+            for a in done:
+                res = result_of(a.ctx)
+                t_done = time.perf_counter() - t0
+                arrival = a.req.arrival if realtime else 0.0
+                res.update(rid=a.req.rid, np=a.req.np, latency=t_done - arrival, t_admit=a.t_admit,
+                           t_done=t_done, queue_wait=a.t_admit - arrival)
+                results[a.req.rid] = res
+                if on_result:
+                    on_result(res)
+                self.batch.remove(a.ctx)
+                a.ctx.close()
+                used -= a.req.footprint
+                active.remove(a)
+        wall = time.perf_counter() - t0
+        lat = np.array([results[k]["latency"] for k in sorted(results)])
+        stats = {"replicas": len(results), "wall_s": wall, "atom_steps": atom_steps,
+                 "atom_steps_per_s": atom_steps / wall if wall > 0 else 0.0,
+                 "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
+                 "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None}
+        return results, stats
+
+    def close(self):
+        self.batch.close()

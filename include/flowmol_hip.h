@@ -200,6 +200,27 @@ int  fm_download_neighbourlist(fm_ctx* ctx, int32_t* mol_i, int32_t* mol_j, int3
 /* Bond lists: bond[b + FM_MAX_FUNCY*n] = partner molno or -1, with shift code as above. */
 int  fm_download_bonds(fm_ctx* ctx, int32_t* bond, int32_t* bond_shift);
 
+/* ---- replica batches ---------------------------------------------------------------------------
+   Many independent contexts (e.g. ensemble replicas of uneven size) advanced together: each step
+   is one packed launch of VV pass 1 + displacement check over all members and one packed launch
+   of the forces (+ VV pass 2) over all members' bricks, captured as HIP graphs, with one host
+   synchronisation per step.  Results are identical to advancing each context alone (fm_run_vv).
+   No counterpart in the reference (one process runs one domain); it replaces running many
+   reference processes side by side.  Members share precision, pair law and bond/tether kind and
+   use rebuild criterion 1 or 2; while a member is in a batch, drive it only through the batch. */
+typedef struct fm_batch fm_batch;   /* opaque */
+int  fm_batch_create(fm_batch** out);
+void fm_batch_destroy(fm_batch* b);
+int  fm_batch_use_graphs(fm_batch* b, int32_t enable);
+/* ctx must have its initial neighbour list and forces; iter0 = iteration number of its next step */
+int  fm_batch_add(fm_batch* b, fm_ctx* ctx, int32_t iter0);
+int  fm_batch_remove(fm_batch* b, fm_ctx* ctx);
+int  fm_batch_run(fm_batch* b, int32_t nsteps, int32_t* n_rebuilds);
+int  fm_batch_size(const fm_batch* b, int32_t* n);
+int  fm_batch_get_iter(const fm_batch* b, const fm_ctx* ctx, int32_t* iter);
+/* device bytes fm_init would allocate (admission control by HBM footprint) */
+int  fm_memory_footprint(const fm_params* p, int32_t np, int64_t* bytes);
+
 /* ---- timing / synchronisation -------------------------------------------------------------- */
 int  fm_synchronize(fm_ctx* ctx);
 /* Average duration in ms of each kernel class over the last fm_run_vv (HIP events on the
