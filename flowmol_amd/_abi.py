@@ -64,6 +64,9 @@ class FmParams(C.Structure):
         ("max_cell_np", C.c_int32),
         ("brick", C.c_int32 * 3),
         ("_pad2", C.c_int32),
+        ("binspercell", C.c_double * 3),
+        ("cv_conserve", C.c_int32),
+        ("_pad3", C.c_int32),
     ]
 
 

@@ -45,6 +45,9 @@ SIGNATURES = {
     "fm_evaluate_macroscopic_properties": (C.c_int, [_vp, C.POINTER(FmMacros)]),
     "fm_get_mop": (C.c_int, [_vp, _dp, _i32]),
     "fm_reset_mop": (C.c_int, [_vp]),
+    "fm_get_cv_bins": (C.c_int, [_vp, _ip, _ip, _dp]),
+    "fm_get_cv_stress": (C.c_int, [_vp, _dp, _i64]),
+    "fm_reset_cv_stress": (C.c_int, [_vp]),
     "fm_run_vv": (C.c_int, [_vp, _i32, _i32, C.POINTER(FmMacros), _i32, _ip]),
     "fm_download_cells": (C.c_int, [_vp, _ip]),
     "fm_download_neighbourlist": (C.c_int, [_vp, _ip, _ip, _ip, _i64, _lp]),

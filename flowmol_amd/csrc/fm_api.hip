@@ -125,7 +125,7 @@ static void free_ctx(fm_ctx* c) {
     void* ptrs[] = {c->st[0], c->st[1], c->rdisp, c->ext[0], c->ext[1], c->slot_of, c->moltype[0], c->moltype[1],
                     c->tag[0], c->tag[1], c->teth[0], c->teth[1], c->mono[0], c->mono[1], c->cell_of,
                     c->cell_count, c->cell_start, c->cell_cursor, c->perm, c->nbl, c->nbn, c->bond, c->bondn,
-                    c->pe_lj, c->pe_poly, c->vir, c->rf, c->pxy, c->planes, c->out_flag, c->out_vals,
+                    c->pe_lj, c->pe_poly, c->vir, c->rf, c->pxy, c->cvface, c->planes, c->out_flag, c->out_vals,
                     c->out_slot, c->out_count, c->red, c->d_macros, c->d_mie, c->d_vmax, c->d_rmax,
                     c->d_flags, c->stage, c->cub_tmp};
     for (void* p : ptrs)
@@ -183,7 +183,9 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
     if (p->precision != FM_PREC_FP64 && p->precision != FM_PREC_MIXED) return fm_set_error(FM_EINVAL, "bad precision");
     if (p->precision == FM_PREC_MIXED && (p->mie_potential || p->potential_flag))
         return fm_set_error(FM_EINVAL, "mixed precision supports the LJ pair law without bonds");
-    if (p->vflux_outflag == 4) return fm_set_error(FM_EINVAL, "vflux_outflag 4 (CV stress) is not on this path");
+    if (p->vflux_outflag < 0 || p->vflux_outflag > 4) return fm_set_error(FM_EINVAL, "vflux_outflag must be 0..4");
+    if (p->vflux_outflag == 4 && p->cv_conserve != 0)
+        return fm_set_error(FM_EINVAL, "CV_conserve 1 (CV stresses every step) is not on this path");
     if (p->vflux_outflag >= 1 && p->vflux_outflag <= 3 &&
         (p->nplanes < 1 || p->nplanes > FM_MAX_PLANES || !(p->planespacing > 0.0)))
         return fm_set_error(FM_EINVAL, "MOP needs 1 <= nplanes <= %d and planespacing > 0", FM_MAX_PLANES);
@@ -310,6 +312,25 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
             return fm_set_error(FM_EHIP, "planes upload failed");
         }
     }
+    if (p->vflux_outflag == 4) {
+        // averaging bins for one process, src/setup_set_parameters.f90:1990-2011
+        size_t nbo = 1;
+        for (int d = 0; d < 3; ++d) {
+            c->cv_nbins[d] = (int)(p->binspercell[d] * c->nc[d]);  // real -> integer assignment
+            if (c->cv_nbins[d] < 1) {
+                free_ctx(c);
+                return fm_set_error(FM_EINVAL, "Error -- bin size less than one, decrease bin to cell ratio or use bigger domain");
+            }
+            c->cv_nhb[d] = (int)ceil((double)c->cv_nbins[d] / (double)c->nc[d]);
+            c->cv_nbo[d] = c->cv_nbins[d] + 2 * c->cv_nhb[d];
+            c->cv_bs[d] = p->domain[d] / c->cv_nbins[d];
+            nbo *= (size_t)c->cv_nbo[d];
+        }
+        if ((rc = dalloc(&c->cvface, 9 * nbo)) || (rc = launch_fill_f64(c, c->cvface, 0.0, 9 * nbo))) {
+            free_ctx(c);
+            return rc;
+        }
+    }
     {
         std::vector<double> mie(7 * T2 + FM_MAX_TYPES, 0.0);
         for (int t = 0; t < T2; ++t) {
@@ -386,6 +407,15 @@ extern "C" int fm_memory_footprint(const fm_params* p, int32_t np, int64_t* byte
     if (p->mie_potential == 1) b += 2 * cap * 4;
     if (p->ensemble == FM_ENSEMBLE_TAG_MOVE) b += 2 * cap * 4 + 2 * 6 * cap * 8;
     if (p->pressure_outflag == 1) b += 9 * cap * 8;
+    if (p->vflux_outflag >= 1 && p->vflux_outflag <= 3) b += 4 * (size_t)std::max(p->nplanes, 0) * 8;
+    if (p->vflux_outflag == 4) {
+        size_t nbo = 1;
+        for (int d = 0; d < 3; ++d) {
+            const int nb = std::max((int)(p->binspercell[d] * nc[d]), 1);
+            nbo *= (size_t)(nb + 2 * (int)ceil((double)nb / nc[d]));
+        }
+        b += 9 * nbo * 8;
+    }
     *bytes = (int64_t)b;
     return 0;
 }
@@ -739,6 +769,52 @@ extern "C" int fm_reset_mop(fm_ctx* c) {
     CHECK_CTX(c);
     if (!c->pxy) return 0;
     TRY(launch_fill_f64(c, c->pxy, 0.0, 3 * (size_t)c->p.nplanes));
+    return 0;
+}
+
+extern "C" int fm_get_cv_bins(const fm_ctx* c, int32_t nbins[3], int32_t nhb[3], double binsize[3]) {
+    CHECK_CTX(c);
+    if (!c->cvface) return fm_set_error(FM_EINVAL, "CV stresses need vflux_outflag 4");
+    for (int d = 0; d < 3; ++d) {
+        nbins[d] = c->cv_nbins[d];
+        nhb[d] = c->cv_nhb[d];
+        binsize[d] = c->cv_bs[d];
+    }
+    return 0;
+}
+
+// Faces 4..6 are not tallied on the device: cumulative_flux_opt only ever assigns them the running
+// total of face n of the bin one up along n (src/simulation_record.f90:5053-5057), and each bin's
+// face n changes only together with that copy, so the copy is taken here.
+extern "C" int fm_get_cv_stress(fm_ctx* c, double* out, int64_t n) {
+    CHECK_CTX(c);
+    if (!c->cvface) return fm_set_error(FM_EINVAL, "CV stresses need vflux_outflag 4");
+    const int64_t nb0 = c->cv_nbo[0], nb1 = c->cv_nbo[1], nb2 = c->cv_nbo[2];
+    const int64_t nbin = nb0 * nb1 * nb2;
+    if (!out || n != 18 * nbin) return fm_set_error(FM_EINVAL, "Pxyface has %lld elements", (long long)(18 * nbin));
+    FM_HIP(hipMemcpyAsync(out, c->cvface, 9 * (size_t)nbin * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FM_HIP(hipStreamSynchronize(c->stream));
+    std::fill(out + 9 * nbin, out + 18 * nbin, 0.0);
+    for (int f = 0; f < 3; ++f)
+        for (int comp = 0; comp < 3; ++comp) {
+            const double* src = out + (comp + 3 * (int64_t)f) * nbin;
+            double* dst = out + (comp + 3 * (int64_t)(f + 3)) * nbin;
+            for (int64_t k = 0; k < nb2; ++k)
+                for (int64_t j = 0; j < nb1; ++j)
+                    for (int64_t i = 0; i < nb0; ++i) {
+                        int64_t u[3] = {i, j, k};
+                        u[f] += 1;
+                        if (u[f] >= c->cv_nbo[f]) continue;
+                        dst[i + nb0 * (j + nb1 * k)] = src[u[0] + nb0 * (u[1] + nb1 * u[2])];
+                    }
+        }
+    return 0;
+}
+
+extern "C" int fm_reset_cv_stress(fm_ctx* c) {
+    CHECK_CTX(c);
+    if (!c->cvface) return 0;
+    TRY(launch_fill_f64(c, c->cvface, 0.0, 9 * (size_t)c->cv_nbo[0] * c->cv_nbo[1] * c->cv_nbo[2]));
     return 0;
 }
 

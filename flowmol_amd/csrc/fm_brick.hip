@@ -390,6 +390,9 @@ struct FbArgs {
     int nplanes;
     double hp, spacing, shift;
     const double* mie;     // [7][FM_MAX_TYPES^2] + mass[FM_MAX_TYPES]
+    double* cv;            // faces 1..3 of the control-volume tally, or null
+    int cvnbo[3], cvnhb[3];
+    double cvbs[3], cvhd[3];
 };
 
 // forward cell offsets of the reference's half-list scan (src/linklist.f90:686-688)
@@ -421,6 +424,51 @@ __device__ void mop_entry(const FbArgs& a, const double ri[3], const double rj[3
         if (fx != 0.0 && hxx != 0.0 && hzz != 0.0)
             for (int d = 0; d < 3; ++d) atomicAdd(&spxy[3 * (n - 1) + d], acc * rij[d] * fx * hxx * hzz);
         if (n == pj) break;
+    }
+}
+
+// control_volume_stresses_opt -> cumulative_flux_opt -> get_crossings for one list entry
+// (src/simulation_record.f90:5281-5317, 4878-5064, 308-353) with fij = f*rij: the segment ri->rj
+// crosses the bin planes between the two molecules' bins along each axis n; every crossing adds
+// crossdir*2*fij to face n of the bin just above the plane.  (Face n+3 is a copy taken on
+// download.)  Bins, crossing points and the floor of tiny separations follow the reference's
+// FP64 operations exactly, so every contribution lands in the reference's bin.
+__device__ void cv_entry(const FbArgs& a, const double ri[3], const double rj[3], double f, int ext_i) {
+    const double eps = (double)1e-12f;  // a default-real literal in the reference
+    double q[3], r12[3];
+    int b1[3], b2[3];
+    for (int d = 0; d < 3; ++d) {
+        const double rd = ri[d] - rj[d];
+        q[d] = 2.0 * (f * rd);
+        r12[d] = fabs(rd) < 0.000001 ? 0.000001 : rd;
+        b1[d] = (int)ceil((ri[d] + a.cvhd[d]) / a.cvbs[d]) + a.cvnhb[d];
+        b2[d] = (int)ceil((rj[d] + a.cvhd[d]) / a.cvbs[d]) + a.cvnhb[d];
+    }
+    const size_t nbin = (size_t)a.cvnbo[0] * a.cvnbo[1] * a.cvnbo[2];
+    for (int n = 0; n < 3; ++n) {
+        if (b1[n] == b2[n]) continue;
+        const int t1 = n == 2 ? 0 : n + 1, t2 = n == 0 ? 2 : n - 1;
+        const double crossdir = copysign(1.0, r12[n]);
+        const int lo = min(b1[n], b2[n]), hi = max(b1[n], b2[n]);
+        for (int i = lo; i < hi; ++i) {
+            const double pt = (double)(i - a.cvnhb[n]) * a.cvbs[n] - a.cvhd[n];
+            double rc[3];
+            rc[t1] = ri[t1] + (r12[t1] / r12[n]) * (pt - ri[n]);
+            rc[t2] = ri[t2] + (r12[t2] / r12[n]) * (pt - ri[n]);
+            rc[n] = pt + eps;
+            int cb[3];
+            bool ok = true;
+            for (int d = 0; d < 3; ++d) {
+                cb[d] = (int)ceil((rc[d] + a.cvhd[d]) / a.cvbs[d]) + a.cvnhb[d];
+                ok = ok && cb[d] >= 1 && cb[d] <= a.cvnbo[d];
+            }
+            if (!ok) {
+                set_err(a.flags, FM_EOVERFLOW, ext_i);
+                continue;
+            }
+            double* F = a.cv + (size_t)(cb[0] - 1) + a.cvnbo[0] * ((size_t)(cb[1] - 1) + a.cvnbo[1] * (size_t)(cb[2] - 1));
+            for (int c = 0; c < 3; ++c) atomicAdd(F + (size_t)(c + 3 * n) * nbin, crossdir * q[c]);
+        }
     }
 }
 
@@ -467,7 +515,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
             ti = a.moltype[i];
             mi = a.mie[7 * T2 + ti];
         }
-        const int exti = mop ? a.ext[i] : 0;
+        const int exti = (mop || (RECORD && a.cv)) ? a.ext[i] : 0;
         const uint64_t* row = a.nbl + i;
         // four 16-bit entries per word: unrolled so the shifts are constants and the four
         // LDS reads of a word can be in flight together
@@ -515,11 +563,15 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                     for (int p = 0; p < 3; ++p)
                         for (int qq = 0; qq < 3; ++qq) rf[3 * p + qq] = rf[3 * p + qq] + acc * rr[p] * rr[qq];
                 }
-                if (mop) {
+                if (mop || a.cv) {
                     const int rcj = region_cell_of(r, a.g.nreg, j);
                     if (mop_owner(r, a.g, rci, rcj, exti, a.ext[r.lslot[j]])) {
                         const double ri[3] = {xi, yi, zi}, rj[3] = {r.lx[j], r.ly[j], r.lz[j]}, rij[3] = {rx, ry, rz};
-                        mop_entry(a, ri, rj, rij, acc, spxy);
+                        if (mop) mop_entry(a, ri, rj, rij, acc, spxy);
+                        // interior pairs carry 2 acc rij, entries against a periodic image acc rij
+                        // (each such pair is met once from either end)
+                        const bool img_entry = r.sh[rcj] != 13 || r.sh[rci] != 13;
+                        if (a.cv) cv_entry(a, ri, rj, img_entry ? acc : 2.0 * acc, exti);
                     }
                 }
             }
@@ -533,8 +585,9 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                 const int j = (int)(w >> 5);
                 int sx, sy, sz;
                 decode_shift((int)(w & 31u), sx, sy, sz);
-                const double rx = xi - img(a.s.x[j], sx, a.g.L[0]), ry = yi - img(a.s.y[j], sy, a.g.L[1]),
-                             rz = zi - img(a.s.z[j], sz, a.g.L[2]);
+                const double xj = img(a.s.x[j], sx, a.g.L[0]), yj = img(a.s.y[j], sy, a.g.L[1]),
+                             zj = img(a.s.z[j], sz, a.g.L[2]);
+                const double rx = xi - xj, ry = yi - yj, rz = zi - zj;
                 const double r2 = (rx * rx + ry * ry) + rz * rz;
                 if (r2 >= a.R02) {
                     set_err(a.flags, FM_EBOND, a.ext[i]);
@@ -546,6 +599,10 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                 fz = fz + acc * rz / mi;
                 if (RECORD || a.ev) vir = vir + acc * r2;
                 if (RECORD) {
+                    if (a.cv) {
+                        const double ri[3] = {xi, yi, zi}, rj[3] = {xj, yj, zj};
+                        cv_entry(a, ri, rj, acc, a.ext[i]);
+                    }
                     pp = pp + 0.5 * a.k_c * a.R0 * a.R0 * log(1.0 - (r2 / a.R02));
                     if (do_rf) {
                         const double rr[3] = {rx, ry, rz};
@@ -631,7 +688,7 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
         const int n = a.nbn[i];
         float fx = 0.f, fy = 0.f, fz = 0.f;
         double pe = 0.0, vir = 0.0;
-        const int exti = mop ? a.ext[i] : 0;
+        const int exti = (mop || (RECORD && a.cv)) ? a.ext[i] : 0;
         const uint64_t* row = a.nbl + i;
         // four 16-bit entries per word: unrolled so the shifts are constants and the four
         // LDS reads of a word can be in flight together
@@ -659,12 +716,14 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
             if (RECORD) {
                 const double id = (double)inv;
                 pe = pe + (4.0 * (p6(id) - p3(id)) - a.potshift);
-                if (mop) {
+                if (mop || a.cv) {
                     const int rcj = region_cell_of(r, a.g.nreg, j);
                     if (mop_owner(r, a.g, rci, rcj, exti, a.ext[r.lslot[j]])) {
                         const double ri[3] = {xi, yi, zi}, rj[3] = {r.lx[j], r.ly[j], r.lz[j]},
                                      rij[3] = {rxd, ryd, rzd};
-                        mop_entry(a, ri, rj, rij, (double)acc, spxy);
+                        if (mop) mop_entry(a, ri, rj, rij, (double)acc, spxy);
+                        const bool img_entry = r.sh[rcj] != 13 || r.sh[rci] != 13;
+                        if (a.cv) cv_entry(a, ri, rj, img_entry ? (double)acc : 2.0 * (double)acc, exti);
                     }
                 }
             }
@@ -891,6 +950,13 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     a.spacing = c->p.planespacing;
     a.shift = c->mop_shift;
     a.mie = mie ? c->d_mie : nullptr;
+    a.cv = (a.rec && c->cvface) ? c->cvface : nullptr;
+    for (int d = 0; d < 3; ++d) {
+        a.cvnbo[d] = c->cv_nbo[d];
+        a.cvnhb[d] = c->cv_nhb[d];
+        a.cvbs[d] = c->cv_bs[d];
+        a.cvhd[d] = c->half[d];
+    }
     a.kick = kick ? 1 : 0;
     a.hdt = 0.5 * c->p.delta_t;
     // LDS: the MOP layout whenever MOP is enabled, so that one launch can mix record steps

@@ -85,6 +85,9 @@ struct fm_ctx {
     double* vir = nullptr;
     double* rf = nullptr;                 // 9*cap
     double* pxy = nullptr;                // 3*nplanes
+    double* cvface = nullptr;             // faces 1..3 of Pxyface: nbinso^3 x 3 components x 3 faces
+    int cv_nbins[3] = {0, 0, 0}, cv_nhb[3] = {0, 0, 0}, cv_nbo[3] = {0, 0, 0};
+    double cv_bs[3] = {0.0, 0.0, 0.0};
     double* planes = nullptr;
     double mop_shift = 0.0;
 

@@ -38,6 +38,8 @@ module flowmol_hip_c
         real(c_double)     :: planespacing
         integer(c_int32_t) :: max_neighbours, max_cell_np
         integer(c_int32_t) :: brick(3), pad2
+        real(c_double)     :: binspercell(3)
+        integer(c_int32_t) :: cv_conserve, pad3
     end type fm_params
 
     type, bind(C) :: fm_macros

@@ -162,6 +162,25 @@ class MDContext:
     def reset_mop(self):
         check(load().fm_reset_mop(self._h))
 
+    def cv_bins(self):
+        """(nbins, nhb, binsize) of the control-volume tally (vflux_outflag 4)."""
+        nb = np.zeros(3, np.int32)
+        nh = np.zeros(3, np.int32)
+        bs = np.zeros(3)
+        check(load().fm_get_cv_bins(self._h, iptr(nb), iptr(nh), dptr(bs)))
+        return nb, nh, bs
+
+    def cv_stress(self):
+        """Pxyface indexed [bx, by, bz, component, face] (0-based, faces 0..5, halo bins included)."""
+        nb, nh, _ = self.cv_bins()
+        nbo = nb + 2 * nh
+        out = np.zeros(int(nbo.prod()) * 18)
+        check(load().fm_get_cv_stress(self._h, dptr(out), out.size))
+        return out.reshape((6, 3, nbo[2], nbo[1], nbo[0])).transpose(4, 3, 2, 1, 0)
+
+    def reset_cv_stress(self):
+        check(load().fm_reset_cv_stress(self._h))
+
     # ---- call sequences of src/md.f90 -----------------------------------------------------------
     def setup(self, iter0=0):
         """setup_MD list build and the initial force call (src/md.f90:60-64,

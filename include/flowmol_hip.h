@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FM_ABI_VERSION 1
+#define FM_ABI_VERSION 2
 
 #define FM_MAX_TYPES 4      /* moltype values 0..FM_MAX_TYPES-1 (Mie lookup tables) */
 #define FM_MAX_FUNCY 4      /* max bonds per monomer, src/modules.f90:468 */
@@ -95,7 +95,8 @@ typedef struct fm_params {
 
     /* stress tallies */
     int32_t vflux_outflag;       /* 0 none; 1,2,3: MOP planes normal to x,y,z (the tally always uses y, as
-                                    src/simulation_record.f90:7090-7109 does); 4: not on this path */
+                                    src/simulation_record.f90:7090-7109 does); 4: control-volume surface
+                                    stresses Pxyface (src/simulation_record.f90:5281-5317) */
     int32_t pressure_outflag;    /* 1: per-molecule virial tensor rfmol (src/simulation_record.f90:6239-6254) */
     int32_t nplanes;             /* number of MOP planes (<= FM_MAX_PLANES) */
     int32_t _pad1;
@@ -106,6 +107,12 @@ typedef struct fm_params {
     int32_t max_cell_np;         /* per-cell capacity check */
     int32_t brick[3];            /* cells per LDS brick along x,y,z (0,0,0: chosen from the occupancy) */
     int32_t _pad2;
+
+    /* averaging bins of the control-volume tally (vflux_outflag 4): BIN2CELLRATIO bins per cell,
+       src/setup_set_parameters.f90:1990-2011 */
+    double binspercell[3];
+    int32_t cv_conserve;         /* CV_conserve: 0 tally on record steps only; 1 (every step) is rejected */
+    int32_t _pad3;
 } fm_params;
 
 typedef struct fm_ctx fm_ctx;   /* opaque */
@@ -180,6 +187,14 @@ int  fm_evaluate_macroscopic_properties(fm_ctx* ctx, fm_macros* out);
    (src/simulation_record.f90:7068-7112) */
 int  fm_get_mop(fm_ctx* ctx, double* Pxy_plane, int32_t nplanes);
 int  fm_reset_mop(fm_ctx* ctx);
+/* Control-volume bins: nbins, nhb (halo bins per side) and binsize as set_parameters_allocate
+   computes them for one process (src/setup_set_parameters.f90:1990-2011); nbinso = nbins+2*nhb. */
+int  fm_get_cv_bins(const fm_ctx* ctx, int32_t nbins[3], int32_t nhb[3], double binsize[3]);
+/* Pxyface(nbinso(1),nbinso(2),nbinso(3),3,6), column-major, as control_volume_stresses_opt leaves
+   it (src/simulation_record.f90:5281-5317, 4878-5064) since the last reset, before the halo fold
+   and scaling of surface_stress_io (src/parallel_io.MPI2.f90:4075-4134).  n = element count. */
+int  fm_get_cv_stress(fm_ctx* ctx, double* pxyface, int64_t n);
+int  fm_reset_cv_stress(fm_ctx* ctx);
 
 /* ---- fused time loop: md_advance_vv (src/md.f90:143-163) nsteps times from iter0, without a
    host round trip per step.  Same kernels and arithmetic as the call-per-routine sequence.

@@ -81,6 +81,10 @@ struct orc_state {
     int nplanes;
     double* planes;
     double* Pxy_plane;           /* (3, nplanes) */
+    /* control-volume surface stresses */
+    int nbins[3], nhb[3], nbinso[3];
+    double binsize[3];
+    double* Pxyface;             /* (nbinso(1),nbinso(2),nbinso(3),3,6) column-major */
     double simtime;
 };
 
@@ -98,7 +102,7 @@ void orc_destroy(orc_state* s) {
     free(s->cell_head); free(s->cell_np); free(s->mol_next);
     free(s->halo_src); free(s->halo_shift);
     free(s->nb_head); free(s->nb_n); free(s->nodes);
-    free(s->rdisp); free(s->planes); free(s->Pxy_plane);
+    free(s->rdisp); free(s->planes); free(s->Pxy_plane); free(s->Pxyface);
     free(s);
 }
 
@@ -187,6 +191,27 @@ orc_state* orc_create(const fm_params* p, int32_t np, int32_t capacity) {
         double shift = 0.5 * (p->domain[pn] - p->planespacing * (s->nplanes - 1));
         for (int n = 1; n <= s->nplanes; ++n)
             s->planes[n - 1] = p->planespacing * (n - 1) + shift - s->halfdomain[pn];
+    }
+    if (p->vflux_outflag == 4) {
+        /* src/setup_set_parameters.f90:1990-2011 with one process per direction: gnbins is the
+           real product assigned to an integer (truncation), nbins = nint(gnbins/1) */
+        for (int d = 0; d < 3; ++d) {
+            s->nbins[d] = (int)(p->binspercell[d] * s->ncells[d]);
+            if (s->nbins[d] < 1) {
+                orc_destroy(s);
+                fail("Error -- bin size less than one, decrease bin to cell ratio or use bigger domain");
+                return NULL;
+            }
+            s->nhb[d] = f_ceiling((double)s->nbins[d] / (double)s->ncells[d]);
+            s->nbinso[d] = s->nbins[d] + 2 * s->nhb[d];
+            s->binsize[d] = p->domain[d] / s->nbins[d];
+        }
+        s->Pxyface = calloc((size_t)s->nbinso[0] * s->nbinso[1] * s->nbinso[2] * 18, sizeof(double));
+        if (!s->Pxyface) {
+            orc_destroy(s);
+            fail("out of memory");
+            return NULL;
+        }
     }
     return s;
 }
@@ -645,6 +670,64 @@ static void mop(orc_state* s, int pnxyz, const double* ri, const double* rj, con
     }
 }
 
+/* get_bin => bin_from_integer_division (src/simulation_record.f90:108-122): 1-based with nhb halo
+   bins below */
+static inline void cv_bin(const orc_state* s, const double* r, int bin[3]) {
+    for (int d = 0; d < 3; ++d) bin[d] = f_ceiling((r[d] + s->halfdomain[d]) / s->binsize[d]) + s->nhb[d];
+}
+
+static inline size_t cv_index(const orc_state* s, const int b[3], int comp, int face) { /* 1-based b, face */
+    return (size_t)(b[0] - 1) +
+           (size_t)s->nbinso[0] * ((size_t)(b[1] - 1) +
+                                   (size_t)s->nbinso[1] * ((size_t)(b[2] - 1) +
+                                                           (size_t)s->nbinso[2] * (comp + 3 * (size_t)(face - 1))));
+}
+
+/* control_volume_stresses_opt (src/simulation_record.f90:5281-5317): quantity = 2 fij passed to
+   cumulative_flux_opt (4878-5064) without an intrinsic surface, whose crossings come from
+   get_crossings (308-353).  The segment ri->rj crosses the bin planes between bin1(n) and
+   bin2(n); each crossing adds crossdir*quantity to the face n of the bin just above the plane and
+   copies that face's running total to face n+3 of the bin below. */
+static int cv_stress(orc_state* s, const double* ri, const double* rj, const double* fij) {
+    const double eps = (double)1e-12f; /* `real(kind(0.d0)),parameter :: eps = 1e-12` is a default-real literal */
+    double q[3], r12[3];
+    int b1[3], b2[3];
+    for (int d = 0; d < 3; ++d) q[d] = 2.0 * fij[d];
+    cv_bin(s, ri, b1);
+    cv_bin(s, rj, b2);
+    for (int d = 0; d < 3; ++d) {
+        r12[d] = ri[d] - rj[d];
+        if (fabs(r12[d]) < 0.000001) r12[d] = 0.000001;
+    }
+    for (int n = 0; n < 3; ++n) {
+        if (b1[n] == b2[n]) continue;
+        const int t1 = (n + 1) % 3, t2 = (n + 2) % 3;
+        const double crossdir = f_sign(1.0, r12[n]);
+        const int minbin = b1[n] < b2[n] ? b1[n] : b2[n], maxbin = b1[n] < b2[n] ? b2[n] : b1[n];
+        for (int i = minbin; i <= maxbin - 1; ++i) {
+            const double pt = (double)(i - s->nhb[n]) * s->binsize[n] - s->halfdomain[n];
+            double rc[3];
+            rc[n] = pt;
+            rc[t1] = ri[t1] + (r12[t1] / r12[n]) * (pt - ri[n]);
+            rc[t2] = ri[t2] + (r12[t2] / r12[n]) * (pt - ri[n]);
+            rc[n] = rc[n] + eps;
+            int cb[3];
+            cv_bin(s, rc, cb);
+            for (int d = 0; d < 3; ++d)
+                if (cb[d] < 1 || cb[d] > s->nbinso[d]) return fail("CV crossing outside the bins (%d,%d,%d)", cb[0], cb[1], cb[2]);
+            for (int c = 0; c < 3; ++c) {
+                size_t k = cv_index(s, cb, c, n + 1);
+                s->Pxyface[k] = s->Pxyface[k] + crossdir * q[c];
+            }
+            int cm[3] = {cb[0], cb[1], cb[2]};
+            cm[n] -= 1;
+            if (cm[0] >= 1 && cm[1] >= 1 && cm[2] >= 1)
+                for (int c = 0; c < 3; ++c) s->Pxyface[cv_index(s, cm, c, n + 4)] = s->Pxyface[cv_index(s, cb, c, n + 1)];
+        }
+    }
+    return 0;
+}
+
 /* pressure_tensor_forces (src/simulation_record.f90:6239-6254) */
 static inline void rf_tally(orc_state* s, int molno, const double* rij, double acc) {
     for (int i = 0; i < 3; ++i)
@@ -655,7 +738,7 @@ static inline void rf_tally(orc_state* s, int molno, const double* rij, double a
 static inline int fmod_i(int a, int b) { return a % b; } /* Fortran mod: sign of a */
 
 /* simulation_compute_forces_LJ_neigbr_halfint (src/simulation_compute_forces.f90:628-726) */
-static void forces_halfint(orc_state* s, int iter) {
+static int forces_halfint(orc_state* s, int iter) {
     int rec = fmod_i(iter, s->p.tplot) == 0;
     int ev = s->p.teval > 0 && fmod_i(iter, s->p.teval) == 0;
     for (int mi = 0; mi < s->np; ++mi) {
@@ -674,6 +757,12 @@ static void forces_halfint(orc_state* s, int iter) {
                 double fmag = fabs(acc) * sqrt(rij2);
                 s->fscale[mi] += fmag;
                 s->fscale[mj] += fmag;
+                if (s->p.vflux_outflag == 4 && (s->p.cv_conserve == 1 || rec)) {
+                    /* interior pairs carry 2 fij, pairs with a halo image fij (each is met twice) */
+                    double f2 = (mj >= s->np || mi >= s->np) ? acc : 2.0 * acc;
+                    double fij[3] = {f2 * rij[0], f2 * rij[1], f2 * rij[2]};
+                    if (cv_stress(s, ri, rj, fij)) return 1;
+                }
                 if (rec) {
                     double e = energy(s, invrij2, mi, mj);
                     s->pe_lj[mi] = s->pe_lj[mi] + e;
@@ -694,6 +783,7 @@ static void forces_halfint(orc_state* s, int iter) {
         }
     }
     for (int n = 0; n < s->cap; ++n) s->pe[n] = s->pe[n] + s->pe_lj[n];
+    return 0;
 }
 
 /* simulation_compute_forces_poly (src/simulation_compute_forces.f90:731-789) with FENE
@@ -715,6 +805,10 @@ static int forces_poly(orc_state* s, int iter) {
             double m = mass_of(s, mi);
             for (int d = 0; d < 3; ++d) s->a[3 * mi + d] = s->a[3 * mi + d] + acc * rij[d] / m;
             s->fscale[mi] += fabs(acc) * sqrt(rij2);
+            if (s->p.vflux_outflag == 4 && (s->p.cv_conserve == 1 || rec)) {
+                double fij[3] = {acc * rij[0], acc * rij[1], acc * rij[2]};
+                if (cv_stress(s, ri, rj, fij)) return 1;
+            }
             if (rec) {
                 if (s->p.pressure_outflag == 1) rf_tally(s, mi, rij, acc);
                 s->pe_poly[mi] = s->pe_poly[mi] + 0.5 * s->p.k_c * s->p.R_0 * s->p.R_0 * log(1.0 - (rij2 / R02));
@@ -755,7 +849,7 @@ int orc_compute_forces(orc_state* s, int32_t iter) {
     memset(s->fscale, 0, s->cap * sizeof(double));
     if (s->p.pressure_outflag == 1 && fmod_i(iter, s->p.tplot) == 0)
         memset(s->rfmol, 0, 9 * (size_t)s->cap * sizeof(double));
-    forces_halfint(s, iter);
+    if (forces_halfint(s, iter)) return 1;
     if (s->p.potential_flag == 1 && forces_poly(s, iter)) return 1;
     if (s->p.ensemble == FM_ENSEMBLE_TAG_MOVE) forces_tether(s);
     return 0;
@@ -929,6 +1023,23 @@ int orc_get_mop(const orc_state* s, double* Pxy) {
 
 int orc_reset_mop(orc_state* s) {
     if (s->Pxy_plane) memset(s->Pxy_plane, 0, 3 * (size_t)s->nplanes * sizeof(double));
+    return 0;
+}
+
+int orc_get_cv_bins(const orc_state* s, int32_t nbins[3], int32_t nhb[3], double binsize[3]) {
+    if (!s->Pxyface) return fail("CV stresses not enabled (vflux_outflag 4)");
+    for (int d = 0; d < 3; ++d) { nbins[d] = s->nbins[d]; nhb[d] = s->nhb[d]; binsize[d] = s->binsize[d]; }
+    return 0;
+}
+
+int orc_get_cv_stress(const orc_state* s, double* pxyface) {
+    if (!s->Pxyface) return fail("CV stresses not enabled (vflux_outflag 4)");
+    memcpy(pxyface, s->Pxyface, (size_t)s->nbinso[0] * s->nbinso[1] * s->nbinso[2] * 18 * sizeof(double));
+    return 0;
+}
+
+int orc_reset_cv_stress(orc_state* s) {
+    if (s->Pxyface) memset(s->Pxyface, 0, (size_t)s->nbinso[0] * s->nbinso[1] * s->nbinso[2] * 18 * sizeof(double));
     return 0;
 }
 

@@ -64,6 +64,9 @@ int orc_checkrebuild(orc_state* s, int32_t iter, int32_t* rebuild);
 int orc_macros(orc_state* s, fm_macros* out);
 int orc_get_mop(const orc_state* s, double* Pxy_plane);
 int orc_reset_mop(orc_state* s);
+int orc_get_cv_bins(const orc_state* s, int32_t nbins[3], int32_t nhb[3], double binsize[3]);
+int orc_get_cv_stress(const orc_state* s, double* pxyface);
+int orc_reset_cv_stress(orc_state* s);
 int orc_get_planes(const orc_state* s, double* planes);
 
 /* md_advance_vv (src/md.f90:143-163) nsteps times; macros of record steps into macros[] */

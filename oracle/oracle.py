@@ -50,6 +50,9 @@ def lib():
             "orc_get_mop": (C.c_int, [vp, dp]),
             "orc_reset_mop": (C.c_int, [vp]),
             "orc_get_planes": (C.c_int, [vp, dp]),
+            "orc_get_cv_bins": (C.c_int, [vp, ip, ip, dp]),
+            "orc_get_cv_stress": (C.c_int, [vp, dp]),
+            "orc_reset_cv_stress": (C.c_int, [vp]),
             "orc_run_vv": (C.c_int, [vp, i32, i32, C.POINTER(FmMacros), i32, ip]),
             "orc_get_cells": (C.c_int, [vp, ip]),
             "orc_neighbour_count": (i64, [vp]),
@@ -190,6 +193,26 @@ class Oracle:
 
     def reset_mop(self):
         _check(lib().orc_reset_mop(self._h))
+
+    def cv_bins(self):
+        """(nbins, nhb, binsize) of the control-volume tally."""
+        nb = np.zeros(3, np.int32)
+        nh = np.zeros(3, np.int32)
+        bs = np.zeros(3)
+        _check(lib().orc_get_cv_bins(self._h, nb.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     nh.ctypes.data_as(C.POINTER(C.c_int32)), dptr(bs)))
+        return nb, nh, bs
+
+    def cv_stress(self):
+        """Pxyface as an array indexed [bx, by, bz, component, face] (0-based; faces 0..5)."""
+        nb, nh, _ = self.cv_bins()
+        nbo = nb + 2 * nh
+        out = np.zeros(int(nbo.prod()) * 18)
+        _check(lib().orc_get_cv_stress(self._h, dptr(out)))
+        return out.reshape((6, 3, nbo[2], nbo[1], nbo[0])).transpose(4, 3, 2, 1, 0)
+
+    def reset_cv_stress(self):
+        _check(lib().orc_reset_cv_stress(self._h))
 
     def planes(self):
         out = np.zeros(self.params.nplanes)

@@ -38,7 +38,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_loads_and_reports_version():
     L = _lib.load()
-    assert L.fm_abi_version() == 1
+    txt = open(HEADER).read()
+    assert L.fm_abi_version() == int(re.search(r"#define FM_ABI_VERSION (\d+)", txt).group(1))
 
 
 def _c_layout():
@@ -47,9 +48,10 @@ def _c_layout():
 #include <stdio.h>
 #include "flowmol_hip.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(fm_params), sizeof(fm_macros),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(fm_params), sizeof(fm_macros),
          offsetof(fm_params, k_c), offsetof(fm_params, planespacing),
-         offsetof(fm_params, max_cell_np), offsetof(fm_macros, max_potenergymol));
+         offsetof(fm_params, max_cell_np), offsetof(fm_macros, max_potenergymol),
+         offsetof(fm_params, binspercell), offsetof(fm_params, cv_conserve));
   return 0;
 }
 """
@@ -62,10 +64,12 @@ int main(void) {
 
 
 def test_struct_layout_matches_header():
-    sp, sm, okc, ops, omc, omax = _c_layout()
+    sp, sm, okc, ops, omc, omax, obin, ocv = _c_layout()
     assert C.sizeof(_abi.FmParams) == sp
     assert C.sizeof(_abi.FmMacros) == sm
     assert _abi.FmParams.k_c.offset == okc
     assert _abi.FmParams.planespacing.offset == ops
     assert _abi.FmParams.max_cell_np.offset == omc
     assert _abi.FmMacros.max_potenergymol.offset == omax
+    assert _abi.FmParams.binspercell.offset == obin
+    assert _abi.FmParams.cv_conserve.offset == ocv

@@ -137,6 +137,58 @@ def test_mop_planes_and_virial_tensor():
     g.close()
 
 
+@pytest.mark.parametrize("ratio", [1.0, 2.0])
+def test_cv_surface_stress_lj(ratio):
+    """Control-volume surface stresses (vflux_outflag 4): every Pxyface entry, halo bins and the
+    face n+3 copies included, at a record step and accumulated over a 12-step trajectory."""
+    p, r, v, a = _lj((8, 8, 8), seed=23, tplot=1)
+    p.vflux_outflag = 4
+    p.tplot = 4
+    p.teval = 4
+    for d in range(3):
+        p.binspercell[d] = ratio
+    say(f"case: CV stress 2048 LJ, {ratio} bins per cell")
+    o = Oracle(p, r, v, a)
+    g = MDContext(p, r, v, a)
+    assert [list(x) for x in o.cv_bins()[:2]] == [list(x) for x in g.cv_bins()[:2]]
+    o.setup()
+    o.reset_cv_stress()
+    o.compute_forces(0)
+    g.setup(0)
+    Fo, Fg = o.cv_stress(), g.cv_stress()
+    assert Fo.shape == Fg.shape and np.abs(Fo).max() > 1.0
+    assert np.abs(Fg - Fo).max() <= 1e-11 * np.abs(Fo).max()
+    ro, rg = drop_in_steps(o, g, 1, 12)
+    assert ro == rg
+    Fo, Fg = o.cv_stress(), g.cv_stress()
+    assert np.abs(Fg - Fo).max() <= 1e-10 * np.abs(Fo).max()
+    g.close()
+
+
+def test_cv_surface_stress_polymer(polymer_state):
+    """CV stresses with FENE bonds (each bond met from both ends) and Mie/WCA pairs."""
+    p, r, v, a, mt, mono = polymer_state
+    p = type(p).from_buffer_copy(p)   # the fixture's parameters stay as they are
+    p.vflux_outflag = 4
+    for d in range(3):
+        p.binspercell[d] = 1.0
+    say("case: CV stress, FENE + Mie melt")
+    o = Oracle(p, r, v, a)
+    o.set_moltype(mt)
+    o.set_monomers(*mono)
+    g = MDContext(p, r, v, a)
+    g.set_moltype(mt)
+    g.set_monomers(*mono)
+    o.setup()
+    o.reset_cv_stress()
+    o.compute_forces(0)
+    g.setup(0)
+    Fo, Fg = o.cv_stress(), g.cv_stress()
+    assert np.abs(Fo).max() > 1.0
+    assert np.abs(Fg - Fo).max() <= 1e-11 * np.abs(Fo).max()
+    g.close()
+
+
 def test_couette_walls_short_trajectory():
     p, r, v, tag, rt = config.config_couette(nunits=(8, 8, 8), tplot=5, nplanes=16)
     say("case: tethered walls, sliding top, 30 steps")

@@ -382,6 +382,143 @@ def test_mop_counts_each_crossing_pair_once():
 
 
 # ---- macros --------------------------------------------------------------------------------------
+def _cv_setup(L, pos, binspercell=1.0, rc=2.5):
+    p = config.make_params((L, L, L), 0.5, rc, vflux_outflag=4, tplot=1, binspercell=binspercell)
+    o = Oracle(p, pos, np.zeros_like(pos))
+    o.setup()
+    o.compute_forces(0)
+    return p, o
+
+
+def _lj_acc(d):
+    inv = 1.0 / float(np.dot(d, d))
+    return 48.0 * (inv ** 7 - 0.5 * inv ** 4)
+
+
+def test_cv_bins_follow_the_cell_ratio():
+    """nbins = int(binspercell*ncells), nhb = ceiling(nbins/ncells), binsize = domain/nbins
+    (src/setup_set_parameters.f90:1990-2011)."""
+    pos = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    for ratio, nb, nh in [(1.0, 3, 1), (2.0, 6, 2), (1.5, 4, 2), (0.5, 1, 1)]:
+        _, o = _cv_setup(9.0, pos, binspercell=ratio)
+        nbins, nhb, bs = o.cv_bins()
+        assert list(nbins) == [nb] * 3 and list(nhb) == [nh] * 3
+        assert np.allclose(bs, 9.0 / nb)
+
+
+def test_cv_single_interior_crossing():
+    """Two molecules straddling the bin plane x = -1.5 (3 bins of 3.0, one halo bin per side):
+    face x of the bin above the plane gets crossdir * 2 * (2 acc rij) with crossdir = sign(rij_x),
+    face x+3 of the bin below holds the same running total, every other entry stays 0."""
+    pos = np.array([[-1.9, 0.2, 0.1], [-0.8, 0.2, 0.1]])
+    _, o = _cv_setup(9.0, pos)
+    F = o.cv_stress()
+    assert F.shape == (5, 5, 5, 3, 6)
+    d = pos[0] - pos[1]
+    expect = 4.0 * _lj_acc(d) * d * np.sign(d[0])
+    # bins (1-based with halo) x: ceil((-1.5+eps+4.5)/3)+1 = 3; y, z: ceil(4.7/3)+1 = 3
+    assert np.allclose(F[2, 2, 2, :, 0], expect, rtol=1e-13, atol=0)
+    assert np.allclose(F[1, 2, 2, :, 3], expect, rtol=1e-13, atol=0)
+    G = F.copy()
+    G[2, 2, 2, :, 0] = 0
+    G[1, 2, 2, :, 3] = 0
+    assert not G.any()
+
+
+def test_cv_periodic_pair_is_met_once_per_end():
+    """A pair across the x boundary is in the half list twice, against either image; each entry
+    carries fij = acc rij (halo pair) and crosses the boundary plane into a halo bin: x = +4.5
+    (bin 5, copy in bin 4) and x = -4.5 (bin 2, copy in bin 1)."""
+    pos = np.array([[4.0, 0.3, -0.2], [-3.9, 0.3, -0.2]])
+    _, o = _cv_setup(9.0, pos)
+    F = o.cv_stress()
+    d = pos[0] - (pos[1] + [9.0, 0, 0])
+    expect = 2.0 * _lj_acc(d) * d * np.sign(d[0])
+    assert np.allclose(F[4, 2, 2, :, 0], expect, rtol=1e-12, atol=0)
+    assert np.allclose(F[3, 2, 2, :, 3], expect, rtol=1e-12, atol=0)
+    assert np.allclose(F[1, 2, 2, :, 0], expect, rtol=1e-12, atol=0)
+    assert np.allclose(F[0, 2, 2, :, 3], expect, rtol=1e-12, atol=0)
+    G = F.copy()
+    for b, f in [(4, 0), (3, 3), (1, 0), (0, 3)]:
+        G[b, 2, 2, :, f] = 0
+    assert not G.any()
+
+
+def cv_bruteforce_folded(r, L, rc, nbins, acc_of):
+    """Independent statement of the CV surface stress: every pair within rc (minimum image) adds
+    4 acc rij sign(rij_n) to face n of the bin just above each plane of bins normal to n that its
+    segment crosses, with the bins folded into the periodic box."""
+    L = np.asarray(L, float)
+    nbins = np.asarray(nbins)
+    hd, bs = 0.5 * L, L / nbins
+    F = np.zeros(tuple(nbins) + (3, 3))
+    N = r.shape[0]
+    iu, ju = np.triu_indices(N, 1)
+    d = r[iu] - r[ju]
+    d -= L * np.round(d / L)
+    r2 = (d * d).sum(1)
+    m = r2 < rc * rc
+    i, d, r2 = iu[m], d[m], r2[m]
+    acc = acc_of(r2)
+    ri = r[i]
+    rj = ri - d
+    q = 4.0 * acc[:, None] * d
+    b1 = np.ceil((ri + hd) / bs).astype(int)
+    b2 = np.ceil((rj + hd) / bs).astype(int)
+    for n in range(3):
+        lo = np.minimum(b1[:, n], b2[:, n])
+        hi = np.maximum(b1[:, n], b2[:, n])
+        cross = np.sign(d[:, n])
+        for k in range(int((hi - lo).max(initial=0))):
+            sel = lo + k < hi
+            plane = (lo[sel] + k) * bs[n] - hd[n]
+            t = (plane - ri[sel, n]) / (rj[sel, n] - ri[sel, n])
+            pc = ri[sel] + t[:, None] * (rj[sel] - ri[sel])
+            cb = np.ceil((pc + hd) / bs).astype(int)
+            cb[:, n] = lo[sel] + k + 1
+            cb = (cb - 1) % nbins
+            for c in range(3):
+                np.add.at(F[..., c, n], (cb[:, 0], cb[:, 1], cb[:, 2]), cross[sel] * q[sel, c])
+    return F
+
+
+def fold_cv(F, nbins, nhb):
+    """Faces 1-3 of Pxyface with the halo bins added onto their periodic images."""
+    out = np.zeros(tuple(nbins) + (3, 3))
+    nbo = np.asarray(nbins) + 2 * np.asarray(nhb)
+    idx = [(np.arange(nbo[d]) - nhb[d]) % nbins[d] for d in range(3)]
+    for a in range(nbo[0]):
+        for b in range(nbo[1]):
+            for c in range(nbo[2]):
+                out[idx[0][a], idx[1][b], idx[2][c]] += F[a, b, c, :, :3]
+    return out
+
+
+@pytest.mark.parametrize("ratio", [1.0, 2.0])
+def test_cv_equals_independent_bruteforce(ratio):
+    """Equilibrated 2048-molecule LJ liquid (rc 2.5): the folded CV tally equals the all-pairs
+    statement above; faces 4-6 are faces 1-3 moved down one bin along their normal."""
+    from tests.helpers import equilibrated
+    p, r, v, _ = equilibrated(config.config_fcc, 200, nunits=(8, 8, 8), density=0.8, rcutoff=2.5, seed=21)
+    L = np.array(p.domain[:])
+    p2 = config.make_params(L, 0.8, 2.5, vflux_outflag=4, tplot=1, binspercell=ratio)
+    o = Oracle(p2, r, v)
+    o.setup()
+    o.compute_forces(0)
+    F = o.cv_stress()
+    nbins, nhb, _ = o.cv_bins()
+    got = fold_cv(F, nbins, nhb)
+    ref = cv_bruteforce_folded(r, L, 2.5, nbins, lambda r2: 48.0 * ((1 / r2) ** 7 - 0.5 * (1 / r2) ** 4))
+    scale = np.abs(ref).max()
+    assert scale > 0
+    assert np.abs(got - ref).max() <= 1e-9 * scale
+    for n in range(3):
+        src = np.take(F[..., n], np.arange(1, F.shape[n]), axis=n)
+        dst = np.take(F[..., n + 3], np.arange(0, F.shape[n] - 1), axis=n)
+        assert np.array_equal(src, dst)
+        assert not np.take(F[..., n + 3], [F.shape[n] - 1], axis=n).any()
+
+
 def test_macros_kinetic_and_temperature():
     p, r, v = config.config_fcc((3, 3, 3), 0.8, 2 ** (1 / 6), seed=2)
     o = Oracle(p, r, v)
