@@ -18,8 +18,12 @@ FM_REBUILD_DISPLACEMENT = 2
 FM_PREC_FP64 = 0
 FM_PREC_MIXED = 1
 FM_TAG_FREE = 0
+FM_TAG_FIXED = 1
+FM_TAG_FIXED_SLIDE = 2
 FM_TAG_TETH = 3
 FM_TAG_TETH_SLIDE = 6
+FM_INTEGRATION_LFV = 0
+FM_INTEGRATION_VV = 1
 
 _T2 = FM_MAX_TYPES * FM_MAX_TYPES
 
@@ -50,7 +54,7 @@ class FmParams(C.Structure):
         ("k_c", C.c_double),
         ("R_0", C.c_double),
         ("ensemble", C.c_int32),
-        ("_pad0", C.c_int32),
+        ("integration_algorithm", C.c_int32),
         ("teth_k2", C.c_double),
         ("teth_k4", C.c_double),
         ("teth_k6", C.c_double),

@@ -34,6 +34,7 @@ SIGNATURES = {
     "fm_get_np": (C.c_int, [_vp, _ip]),
     "fm_simtime": (C.c_double, [_vp]),
     "fm_move_particles_vv": (C.c_int, [_vp, _i32]),
+    "fm_move_particles_lfv": (C.c_int, [_vp]),
     "fm_checkrebuild": (C.c_int, [_vp, _i32, _ip]),
     "fm_linklist_deallocateall": (C.c_int, [_vp]),
     "fm_sendmols": (C.c_int, [_vp]),
@@ -49,6 +50,7 @@ SIGNATURES = {
     "fm_get_cv_stress": (C.c_int, [_vp, _dp, _i64]),
     "fm_reset_cv_stress": (C.c_int, [_vp]),
     "fm_run_vv": (C.c_int, [_vp, _i32, _i32, C.POINTER(FmMacros), _i32, _ip]),
+    "fm_run_lfv": (C.c_int, [_vp, _i32, _i32, C.POINTER(FmMacros), _i32, _ip]),
     "fm_download_cells": (C.c_int, [_vp, _ip]),
     "fm_download_neighbourlist": (C.c_int, [_vp, _ip, _ip, _ip, _i64, _lp]),
     "fm_download_bonds": (C.c_int, [_vp, _ip, _ip]),

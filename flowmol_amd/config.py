@@ -13,7 +13,7 @@ import math
 import numpy as np
 
 from ._abi import (FM_MAX_TYPES, FM_PREC_FP64, FM_REBUILD_DISPLACEMENT, FM_ENSEMBLE_NVE,
-                   FmParams)
+                   FM_INTEGRATION_VV, FmParams)
 
 WCA_CUT = 2.0 ** (1.0 / 6.0)
 
@@ -84,7 +84,7 @@ def make_params(domain, density, rcutoff, delta=0.3, dt=0.005, tplot=10, teval=N
                 rebuild_criteria=FM_REBUILD_DISPLACEMENT, fixed_rebuild=20, precision=FM_PREC_FP64,
                 potential_flag=0, k_c=30.0, R_0=1.5, ensemble=FM_ENSEMBLE_NVE,
                 vflux_outflag=0, nplanes=None, planespacing=None, pressure_outflag=0,
-                max_neighbours=0, binspercell=1.0):
+                max_neighbours=0, binspercell=1.0, integration=None):
     p = FmParams()
     for d in range(3):
         p.domain[d] = float(domain[d])
@@ -105,6 +105,7 @@ def make_params(domain, density, rcutoff, delta=0.3, dt=0.005, tplot=10, teval=N
     p.k_c = float(k_c)
     p.R_0 = float(R_0)
     p.ensemble = int(ensemble)
+    p.integration_algorithm = FM_INTEGRATION_VV if integration is None else int(integration)
     p.vflux_outflag = int(vflux_outflag)
     p.pressure_outflag = int(pressure_outflag)
     if 1 <= vflux_outflag <= 3:

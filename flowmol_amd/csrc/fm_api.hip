@@ -180,6 +180,8 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
     if (p->rebuild_criteria == FM_REBUILD_FIXED && p->fixed_rebuild < 1) return fm_set_error(FM_EINVAL, "fixed_rebuild must be >= 1");
     if (p->ensemble != FM_ENSEMBLE_NVE && p->ensemble != FM_ENSEMBLE_TAG_MOVE)
         return fm_set_error(FM_EINVAL, "ensemble must be NVE (0) or tag_move (6)");
+    if (p->integration_algorithm != FM_INTEGRATION_VV && p->integration_algorithm != FM_INTEGRATION_LFV)
+        return fm_set_error(FM_EINVAL, "integration_algorithm must be leap-frog (0) or velocity Verlet (1)");
     if (p->precision != FM_PREC_FP64 && p->precision != FM_PREC_MIXED) return fm_set_error(FM_EINVAL, "bad precision");
     if (p->precision == FM_PREC_MIXED && (p->mie_potential || p->potential_flag))
         return fm_set_error(FM_EINVAL, "mixed precision supports the LJ pair law without bonds");
@@ -490,8 +492,10 @@ extern "C" int fm_upload_tags(fm_ctx* c, const int32_t* tag, const double* rteth
     CHECK_CTX(c);
     if (!c->tag[0]) return fm_set_error(FM_EINVAL, "tags need ensemble = tag_move");
     for (int n = 0; n < c->np; ++n)
-        if (tag[n] != FM_TAG_FREE && tag[n] != FM_TAG_TETH && tag[n] != FM_TAG_TETH_SLIDE)
-            return fm_set_error(FM_EINVAL, "unsupported tag %d at molecule %d", tag[n], n + 1);
+        if (tag[n] != FM_TAG_FREE && tag[n] != FM_TAG_FIXED && tag[n] != FM_TAG_FIXED_SLIDE && tag[n] != FM_TAG_TETH &&
+            tag[n] != FM_TAG_TETH_SLIDE)
+            return fm_set_error(FM_EINVAL, "Invalid molecular Tag %d for molecule %d (thermostatted tags are not on this path)",
+                                tag[n], n + 1);
     TRY(upload_i32(c, c->tag[c->cur], tag, 1));
     // rtether (AoS) and slidev from read_tag (src/molecular_properties.f90:764-830)
     std::vector<double> h(6 * (size_t)c->np, 0.0), r(3 * (size_t)c->np);
@@ -500,7 +504,10 @@ extern "C" int fm_upload_tags(fm_ctx* c, const int32_t* tag, const double* rteth
     for (int n = 0; n < c->np; ++n) {
         for (int d = 0; d < 3; ++d) h[6 * (size_t)n + d] = rtether ? rtether[3 * (size_t)n + d] : r[3 * (size_t)n + d];
         double sg = tag[n] == FM_TAG_TETH_SLIDE ? copysign(1.0, r[3 * (size_t)n + 1]) : 0.0;
-        for (int d = 0; d < 3; ++d) h[6 * (size_t)n + 3 + d] = tag[n] == FM_TAG_TETH_SLIDE ? c->p.wallslidev[d] * sg : 0.0;
+        for (int d = 0; d < 3; ++d)
+            h[6 * (size_t)n + 3 + d] = tag[n] == FM_TAG_TETH_SLIDE    ? c->p.wallslidev[d] * sg
+                                       : tag[n] == FM_TAG_FIXED_SLIDE ? c->p.wallslidev[d]
+                                                                      : 0.0;
     }
     FM_HIP(hipMemcpyAsync(c->stage, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     TRY(launch_ext_to_slot_f64(c, c->teth[c->cur], c->stage, 6));
@@ -590,6 +597,8 @@ extern "C" int fm_download_rfmol(fm_ctx* c, double* rfmol) {
 // ---- the routines -----------------------------------------------------------------------------
 extern "C" int fm_move_particles_vv(fm_ctx* c, int32_t pass_num) {
     CHECK_CTX(c);
+    if (c->p.integration_algorithm != FM_INTEGRATION_VV)
+        return fm_set_error(FM_ESTATE, "integration_algorithm is leap-frog: use fm_move_particles_lfv");
     if (pass_num == 1) {
         c->moved_since_bin = true;
         TRY(launch_pass1(c));
@@ -599,6 +608,18 @@ extern "C" int fm_move_particles_vv(fm_ctx* c, int32_t pass_num) {
     } else {
         return fm_set_error(FM_EINVAL, "pass_num must be 1 or 2");
     }
+    return 0;
+}
+
+extern "C" int fm_move_particles_lfv(fm_ctx* c) {
+    CHECK_CTX(c);
+    if (c->p.integration_algorithm != FM_INTEGRATION_LFV)
+        return fm_set_error(FM_ESTATE, "integration_algorithm is velocity Verlet: use fm_move_particles_vv");
+    if (c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && !c->has_tags) return fm_set_error(FM_ESTATE, "tag_move needs tags");
+    c->moved_since_bin = true;
+    TRY(launch_move_lfv(c, false));
+    c->simtime = c->simtime + c->p.delta_t;
+    if (c->p.ensemble == FM_ENSEMBLE_TAG_MOVE) TRY(sync_flags(c));
     return 0;
 }
 
@@ -833,6 +854,8 @@ int rebuild_lists_internal(fm_ctx* c) { return rebuild_lists(c); }
 extern "C" int fm_run_vv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* macros, int32_t n_macros,
                          int32_t* n_rebuilds) {
     CHECK_CTX(c);
+    if (c->p.integration_algorithm != FM_INTEGRATION_VV)
+        return fm_set_error(FM_ESTATE, "integration_algorithm is leap-frog: use fm_run_lfv");
     if (!c->nlist_valid) return fm_set_error(FM_ESTATE, "run before the initial neighbour list");
     if (c->p.mie_potential == 1 && !c->has_moltype) return fm_set_error(FM_ESTATE, "Mie potential needs moltype");
     if (c->p.potential_flag == 1 && !c->has_mono) return fm_set_error(FM_ESTATE, "potential_flag 1 needs monomer data");
@@ -891,6 +914,71 @@ extern "C" int fm_run_vv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* ma
         if (rec && macros && nm < n_macros) {
             TRY(fm_evaluate_macroscopic_properties(c, &macros[nm]));
             ++nm;
+        }
+    }
+    TRY(sync_flags(c));
+    if (n_rebuilds) *n_rebuilds = nrb;
+    return 0;
+}
+
+// ---- fused md_advance_lfv loop (src/md.f90:109-139): forces, record, move, rebuild check ------
+extern "C" int fm_run_lfv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* macros, int32_t n_macros,
+                          int32_t* n_rebuilds) {
+    CHECK_CTX(c);
+    if (c->p.integration_algorithm != FM_INTEGRATION_LFV)
+        return fm_set_error(FM_ESTATE, "integration_algorithm is velocity Verlet: use fm_run_vv");
+    if (!c->nlist_valid) return fm_set_error(FM_ESTATE, "run before the initial neighbour list");
+    if (c->p.mie_potential == 1 && !c->has_moltype) return fm_set_error(FM_ESTATE, "Mie potential needs moltype");
+    if (c->p.potential_flag == 1 && !c->has_mono) return fm_set_error(FM_ESTATE, "potential_flag 1 needs monomer data");
+    if (c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && !c->has_tags) return fm_set_error(FM_ESTATE, "tag_move needs tags");
+    const bool sync_each_force = c->p.potential_flag == 1;
+    int nrb = 0, nm = 0;
+    for (int iter = iter0; iter < iter0 + nsteps; ++iter) {
+        TRY(launch_forces(c, iter, false));
+        const bool rec = (iter % c->p.tplot) == 0;
+        const bool ev = c->p.teval > 0 && (iter % c->p.teval) == 0;
+        c->pe_valid = rec;
+        c->vir_valid = rec || ev;
+        if (sync_each_force) TRY(sync_flags(c));
+        if (rec && macros && nm < n_macros) {
+            TRY(fm_evaluate_macroscopic_properties(c, &macros[nm]));
+            ++nm;
+        }
+        const int forced = ((iter + 1) % c->p.tplot) == 0;
+        int rb = forced;
+        bool need_flag = false;
+        c->moved_since_bin = true;
+        switch (c->p.rebuild_criteria) {
+        case FM_REBUILD_DISPLACEMENT:
+            TRY(launch_move_lfv(c, !forced));
+            need_flag = !forced;
+            break;
+        case FM_REBUILD_VMAX:
+            TRY(launch_move_lfv(c, false));
+            TRY(launch_check_vmax(c));
+            need_flag = !forced;
+            break;
+        default:
+            TRY(launch_move_lfv(c, false));
+            if (iter % c->p.fixed_rebuild == 0) rb = 1;
+            break;
+        }
+        c->simtime = c->simtime + c->p.delta_t;
+        if (need_flag || c->p.ensemble == FM_ENSEMBLE_TAG_MOVE) {
+            TRY(sync_flags(c));
+            if (c->h_flags[FLAG_REBUILD]) {
+                if (need_flag) rb = 1;
+                TRY(launch_fill_i32(c, c->d_flags + FLAG_REBUILD, 0, 1));
+            }
+        }
+        if (rb) {
+            c->rdisp_valid = false;
+            if (c->p.rebuild_criteria == FM_REBUILD_VMAX) {
+                TRY(launch_fill_f64(c, c->d_rmax, 0.0, 1));
+                TRY(launch_fill_i32(c, c->d_flags + FLAG_REBUILD, 0, 1));
+            }
+            TRY(rebuild_lists(c));
+            ++nrb;
         }
     }
     TRY(sync_flags(c));

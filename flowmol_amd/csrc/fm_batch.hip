@@ -195,6 +195,8 @@ extern "C" int fm_batch_add(fm_batch* b, fm_ctx* c, int32_t iter0) {
     if (!c->nlist_valid) return fm_set_error(FM_ESTATE, "a batch member needs its neighbour list and initial forces");
     if (c->p.rebuild_criteria == FM_REBUILD_VMAX)
         return fm_set_error(FM_EINVAL, "batch members use the displacement or fixed rebuild criterion");
+    if (c->p.integration_algorithm != FM_INTEGRATION_VV)
+        return fm_set_error(FM_EINVAL, "batch members use velocity Verlet");
     char tmp[1024];
     size_t lds;
     int kind;

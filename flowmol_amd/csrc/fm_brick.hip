@@ -917,7 +917,9 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     a.ext = c->ext[cur];
     const bool mie = c->p.mie_potential == 1;
     a.moltype = mie ? c->moltype[cur] : nullptr;
-    const bool tm = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && c->has_tags;
+    // velocity Verlet: tethers act with the forces; leap-frog applies them in the move
+    const bool tm = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && c->has_tags &&
+                    c->p.integration_algorithm == FM_INTEGRATION_VV;
     a.tag = tm ? c->tag[cur] : nullptr;
     a.teth = tm ? c->teth[cur] : nullptr;
     a.pe_lj = c->pe_lj;

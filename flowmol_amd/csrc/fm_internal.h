@@ -158,6 +158,7 @@ int launch_bin(fm_ctx* c);
 int launch_nlist(fm_ctx* c);
 int launch_forces(fm_ctx* c, int iter, bool kick = false);
 int launch_pass1_check_disp(fm_ctx* c);
+int launch_move_lfv(fm_ctx* c, bool check_disp);
 int launch_macros(fm_ctx* c);
 int launch_scatter_state_to_host_order(fm_ctx* c, double* d_out9, double* d_tally);
 int launch_from_host(fm_ctx* c, const double* d_in9);

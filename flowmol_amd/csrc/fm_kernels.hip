@@ -84,6 +84,77 @@ __global__ __launch_bounds__(FM_BLOCK) void k_pass2(St s, int np, double hdt) {
     s.vz[i] = s.vz[i] + hdt * s.az[i];
 }
 
+// ---- simulation_move_particles_lfv, src/simulation_move_particles_lfv.f90:38-177 ---------------
+// NVE, or tag_move through simulation_move_particles_lfv_tag (308-490) for the tags without a
+// thermostat; tethered molecules get tether_force (src/molecular_properties.f90:833-915) here,
+// inside the move.  With disp != 0 the displacement update of simulation_checkrebuild (rebuild
+// criterion 2) follows in the same pass.
+__global__ __launch_bounds__(FM_BLOCK) void k_move_lfv(St s, const int* tag, double* teth, const int* ext, int np,
+                                                        int cap, double dt, double k2, double k4, double k6,
+                                                        double* rd, double thresh, int disp, int fresh,
+                                                        int* flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    if (i < np) {
+        const int tg = tag ? tag[i] : FM_TAG_FREE;
+        double r[3] = {s.x[i], s.y[i], s.z[i]};
+        double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
+        double a[3] = {s.ax[i], s.ay[i], s.az[i]};
+        const size_t c = (size_t)cap;
+        if (tg == FM_TAG_TETH || tg == FM_TAG_TETH_SLIDE) {
+            double rio[3];
+            for (int d = 0; d < 3; ++d) rio[d] = r[d] - teth[i + d * c];
+            const double mag = sqrt((rio[0] * rio[0] + rio[1] * rio[1]) + rio[2] * rio[2]);
+            const double am = -2.0 * k2 * mag - 4.0 * k4 * (mag * mag) - 6.0 * k6 * pow(mag, 4.0);
+            for (int d = 0; d < 3; ++d) a[d] = a[d] + am * rio[d];
+            for (int d = 0; d < 3; ++d) teth[i + d * c] = teth[i + d * c] + teth[i + (3 + d) * c] * dt;
+            s.ax[i] = a[0];
+            s.ay[i] = a[1];
+            s.az[i] = a[2];
+        }
+        switch (tg) {
+        case FM_TAG_FREE:
+        case FM_TAG_TETH:
+            for (int d = 0; d < 3; ++d) v[d] = v[d] + dt * a[d];
+            for (int d = 0; d < 3; ++d) r[d] = r[d] + dt * v[d];
+            break;
+        case FM_TAG_TETH_SLIDE:
+            for (int d = 0; d < 3; ++d) v[d] = v[d] + dt * a[d];
+            for (int d = 0; d < 3; ++d) r[d] = r[d] + dt * v[d] + dt * teth[i + (3 + d) * c];
+            break;
+        case FM_TAG_FIXED_SLIDE:
+            for (int d = 0; d < 3; ++d) r[d] = r[d] + dt * teth[i + (3 + d) * c];
+            break;
+        case FM_TAG_FIXED:
+            break;
+        default:
+            set_err(flags, FM_EINVAL, ext[i]);
+        }
+        s.x[i] = r[0];
+        s.y[i] = r[1];
+        s.z[i] = r[2];
+        s.vx[i] = v[0];
+        s.vy[i] = v[1];
+        s.vz[i] = v[2];
+        if (disp) {
+            double d0 = fresh ? 0.0 : rd[i];
+            double d1 = fresh ? 0.0 : rd[i + c];
+            double d2 = fresh ? 0.0 : rd[i + 2 * c];
+            d0 = d0 + v[0] * dt;
+            d1 = d1 + v[1] * dt;
+            d2 = d2 + v[2] * dt;
+            rd[i] = d0;
+            rd[i + c] = d1;
+            rd[i + 2 * c] = d2;
+            hit = fabs(d0) > thresh || fabs(d1) > thresh || fabs(d2) > thresh;
+        }
+    }
+    unsigned long long m = __ballot(hit);
+    if (m != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1 &&
+        __atomic_load_n(&flags[FLAG_REBUILD], __ATOMIC_RELAXED) == 0)
+        atomicOr(&flags[FLAG_REBUILD], 1);
+}
+
 // ---- rebuild criterion 2, src/simulation_checkrebuild.f90:105-121 ----------------------------
 __global__ __launch_bounds__(FM_BLOCK) void k_check_disp(St s, double* rd, int np, int cap, double dt,
                                                           double thresh, int fresh, int* flags) {
@@ -248,10 +319,13 @@ __global__ __launch_bounds__(1024) void k_sendmols_passes(St s, double* teth, co
                     int slot = tmp[m];
                     sl[lst[m]] = slot;
                     if (tag_move) {  // read_tag on the renumbered molecule
-                        double sg = tag[slot] == FM_TAG_TETH_SLIDE ? copysign(1.0, s.y[slot]) : 0.0;
-                        teth[slot + 3 * (size_t)cap] = tag[slot] == FM_TAG_TETH_SLIDE ? wsx * sg : 0.0;
-                        teth[slot + 4 * (size_t)cap] = tag[slot] == FM_TAG_TETH_SLIDE ? wsy * sg : 0.0;
-                        teth[slot + 5 * (size_t)cap] = tag[slot] == FM_TAG_TETH_SLIDE ? wsz * sg : 0.0;
+                        const int tg = tag[slot];
+                        const double sg = tg == FM_TAG_TETH_SLIDE   ? copysign(1.0, s.y[slot])
+                                          : tg == FM_TAG_FIXED_SLIDE ? 1.0
+                                                                     : 0.0;
+                        teth[slot + 3 * (size_t)cap] = sg != 0.0 ? wsx * sg : 0.0;
+                        teth[slot + 4 * (size_t)cap] = sg != 0.0 ? wsy * sg : 0.0;
+                        teth[slot + 5 * (size_t)cap] = sg != 0.0 ? wsz * sg : 0.0;
                     }
                 }
                 __syncthreads();
@@ -418,7 +492,8 @@ __global__ __launch_bounds__(FM_BLOCK) void k_permute(PermArgs a) {
 #define NRED 8
 __global__ __launch_bounds__(FM_BLOCK) void k_macros_partial(St s, const double* pe_lj, const double* pe_poly,
                                                              const double* vir, const int* moltype,
-                                                             const double* mass, int np, double* part) {
+                                                             const double* mass, int np, int lfv, double dt,
+                                                             double* part) {
     __shared__ double red[NRED][FM_BLOCK / 64];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[NRED] = {0, 0, 0, 0, 0, 0, 0, -1.0e300};
@@ -430,6 +505,11 @@ __global__ __launch_bounds__(FM_BLOCK) void k_macros_partial(St s, const double*
         v[2] = pl + pp;
         v[3] = vir[i];
         double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+        if (lfv) {  // leap-frog: velocity shifted half a step, src/simulation_record.f90:1238-1247
+            vx = vx + 0.5 * s.ax[i] * dt;
+            vy = vy + 0.5 * s.ay[i] * dt;
+            vz = vz + 0.5 * s.az[i] * dt;
+        }
         v[4] = (m * (vx * vx) + m * (vy * vy)) + m * (vz * vz);
         v[5] = (m * vx + m * vy) + m * vz;
         v[6] = m;
@@ -610,6 +690,21 @@ int launch_check_disp(fm_ctx* c) {
     return 0;
 }
 
+// simulation_move_particles_lfv; check_disp: fused displacement update of rebuild criterion 2
+int launch_move_lfv(fm_ctx* c, bool check_disp) {
+    if (c->np == 0) return 0;
+    const bool tm = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE;
+    ktimer_begin(c, KT_PASS1);
+    k_move_lfv<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(
+        state_of(c->x(), c->cap), tm ? c->tag[c->cur] : nullptr, tm ? c->teth[c->cur] : nullptr, c->ext[c->cur],
+        c->np, c->cap, c->p.delta_t, c->p.teth_k2, c->p.teth_k4, c->p.teth_k6, c->rdisp, 0.5 * c->dmin,
+        check_disp ? 1 : 0, c->rdisp_valid ? 0 : 1, c->d_flags);
+    ktimer_end(c, KT_PASS1);
+    LAUNCH_CHECK();
+    if (check_disp) c->rdisp_valid = true;
+    return 0;
+}
+
 int launch_check_vmax(fm_ctx* c) {
     ktimer_begin(c, KT_CHECK);
     if (c->np > 0) k_vmax<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(state_of(c->x(), c->cap), c->np, c->d_vmax);
@@ -694,7 +789,8 @@ int launch_macros(fm_ctx* c) {
     k_macros_partial<<<nb, FM_BLOCK, 0, c->stream>>>(state_of(c->x(), c->cap), c->pe_lj, poly ? c->pe_poly : nullptr,
                                                       c->vir, mie ? c->moltype[c->cur] : nullptr,
                                                       c->d_mie + 7 * FM_MAX_TYPES * FM_MAX_TYPES,
-                                                      c->np, c->red);
+                                                      c->np, c->p.integration_algorithm == FM_INTEGRATION_LFV ? 1 : 0,
+                                                      c->p.delta_t, c->red);
     k_macros_final<<<1, 64, 0, c->stream>>>(c->red, nb, c->d_macros);
     ktimer_end(c, KT_MACROS);
     LAUNCH_CHECK();

@@ -12,6 +12,7 @@ module flowmol_hip_c
     integer, parameter :: FM_MAX_TYPES = 4, FM_MAX_FUNCY = 4
     integer(c_int32_t), parameter :: FM_REBUILD_VMAX = 0, FM_REBUILD_FIXED = 1, FM_REBUILD_DISPLACEMENT = 2
     integer(c_int32_t), parameter :: FM_ENSEMBLE_NVE = 0, FM_ENSEMBLE_TAG_MOVE = 6
+    integer(c_int32_t), parameter :: FM_INTEGRATION_LFV = 0, FM_INTEGRATION_VV = 1
     integer(c_int32_t), parameter :: FM_PREC_FP64 = 0, FM_PREC_MIXED = 1
 
     type, bind(C) :: fm_params
@@ -31,7 +32,7 @@ module flowmol_hip_c
         real(c_double)     :: potshift_lookup(FM_MAX_TYPES*FM_MAX_TYPES)
         real(c_double)     :: rcutoff2_lookup(FM_MAX_TYPES*FM_MAX_TYPES)
         real(c_double)     :: k_c, R_0
-        integer(c_int32_t) :: ensemble, pad0
+        integer(c_int32_t) :: ensemble, integration_algorithm
         real(c_double)     :: teth_k2, teth_k4, teth_k6
         real(c_double)     :: wallslidev(3)
         integer(c_int32_t) :: vflux_outflag, pressure_outflag, nplanes, pad1
@@ -81,6 +82,16 @@ module flowmol_hip_c
             import :: c_int, c_int32_t, c_ptr
             type(c_ptr), value        :: ctx
             integer(c_int32_t), value :: pass_num
+        end function
+        integer(c_int) function fm_move_particles_lfv(ctx) bind(C, name="fm_move_particles_lfv")
+            import :: c_int, c_ptr
+            type(c_ptr), value        :: ctx
+        end function
+        integer(c_int) function fm_upload_tags(ctx, tag, rtether) bind(C, name="fm_upload_tags")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            type(c_ptr), value        :: ctx
+            integer(c_int32_t), intent(in) :: tag(*)
+            real(c_double), intent(in) :: rtether(*)
         end function
         integer(c_int) function fm_checkrebuild(ctx, iter, rebuild) bind(C, name="fm_checkrebuild")
             import :: c_int, c_int32_t, c_ptr

@@ -4,7 +4,8 @@
 ! Link these in place of the reference's own definitions (src/linklist.f90 assign_to_cell,
 ! assign_to_neighbourlist, linklist_deallocateall, sort_mols; src/simulation_compute_forces.f90
 ! simulation_compute_forces; src/simulation_checkrebuild.f90 simulation_checkrebuild;
-! src/simulation_move_particles_vv.f90 simulation_move_particles_vv; src/messenger.MPI.f90
+! src/simulation_move_particles_vv.f90 simulation_move_particles_vv;
+! src/simulation_move_particles_lfv.f90 simulation_move_particles_lfv; src/messenger.MPI.f90
 ! sendmols, messenger_updateborders) when building the reference with FLOWMOL_HIP.  The names,
 ! argument lists and abort behaviour are those of the reference; the state stays on the GPU
 ! and the host arrays r, v, a are refreshed where the reference's other consumers read them.
@@ -26,11 +27,13 @@ contains
 
     ! called once after setup_set_parameters (src/md.f90:58): parameters and initial state
     subroutine fm_setup()
-        use physical_constants_MD, only : np, density
+        use physical_constants_MD, only : np, density, wallslidev
         use computational_constants_MD, only : globaldomain, rcutoff, delta_rneighbr, delta_t, tplot, &
                                                rebuild_criteria, fixed_rebuild, potential_flag, &
-                                               Mie_potential, vflux_outflag, pressure_outflag
-        use arrays_MD, only : r, v, a
+                                               Mie_potential, vflux_outflag, pressure_outflag, &
+                                               integration_algorithm, ensemble, tag_move, &
+                                               teth_k2, teth_k4, teth_k6, binspercell, cv_conserve
+        use arrays_MD, only : r, v, a, tag, rtether
         type(fm_params) :: p
         p%domain = globaldomain
         p%density = density
@@ -47,8 +50,17 @@ contains
         p%mass_lookup = 1.d0
         p%vflux_outflag = vflux_outflag
         p%pressure_outflag = pressure_outflag
+        p%integration_algorithm = integration_algorithm
+        p%ensemble = ensemble
+        p%teth_k2 = teth_k2
+        p%teth_k4 = teth_k4
+        p%teth_k6 = teth_k6
+        p%wallslidev = wallslidev
+        p%binspercell = binspercell
+        p%cv_conserve = cv_conserve
         call fm_check(fm_init(p, int(np, c_int32_t), int(np, c_int32_t), fm_ctx))
         call fm_check(fm_upload_state(fm_ctx, int(np, c_int32_t), r, v, a))
+        if (ensemble == tag_move) call fm_check(fm_upload_tags(fm_ctx, tag, rtether))
     end subroutine fm_setup
 end module flowmol_hip_driver
 
@@ -97,6 +109,11 @@ subroutine simulation_move_particles_vv(pass_num)
     integer, intent(in) :: pass_num
     call fm_check(fm_move_particles_vv(fm_ctx, int(pass_num, c_int32_t)))
 end subroutine simulation_move_particles_vv
+
+subroutine simulation_move_particles_lfv
+    use flowmol_hip_driver
+    call fm_check(fm_move_particles_lfv(fm_ctx))
+end subroutine simulation_move_particles_lfv
 
 subroutine simulation_compute_forces
     use flowmol_hip_driver

@@ -65,6 +65,7 @@ program md_vv_driver
     p%precision = FM_PREC_FP64
     p%mass_lookup = 1.d0
     p%ensemble = FM_ENSEMBLE_NVE
+    p%integration_algorithm = FM_INTEGRATION_VV
     p%vflux_outflag = 0
     p%pressure_outflag = 0
     p%nplanes = 0

@@ -122,6 +122,9 @@ class MDContext:
     def simulation_move_particles_vv(self, pass_num):
         check(load().fm_move_particles_vv(self._h, int(pass_num)))
 
+    def simulation_move_particles_lfv(self):
+        check(load().fm_move_particles_lfv(self._h))
+
     def simulation_checkrebuild(self, iter_):
         rb = C.c_int32()
         check(load().fm_checkrebuild(self._h, int(iter_), C.byref(rb)))
@@ -215,6 +218,27 @@ class MDContext:
         ms = (FmMacros * max(n_macros, 1))()
         nrb = C.c_int32()
         check(load().fm_run_vv(self._h, int(iter0), int(nsteps), ms, int(n_macros), C.byref(nrb)))
+        return [ms[k].as_dict() for k in range(n_macros)], nrb.value
+
+    def md_advance_lfv(self, iter_, macros=False):
+        """One leap-frog step, routine by routine (src/md.f90:109-139): forces, record, move,
+        rebuild check.  Returns (rebuilt, macros of the record step or None)."""
+        self.simulation_compute_forces(iter_)
+        m = None
+        if macros and iter_ % self.params.tplot == 0:
+            m = self.evaluate_macroscopic_properties()
+        self.simulation_move_particles_lfv()
+        self.messenger_updateborders(0)
+        rb = self.simulation_checkrebuild(iter_)
+        if rb:
+            self.rebuild()
+        return rb, m
+
+    def run_lfv(self, iter0, nsteps, n_macros=0):
+        """Fused leap-frog loop (same kernels and arithmetic as md_advance_lfv per step)."""
+        ms = (FmMacros * max(n_macros, 1))()
+        nrb = C.c_int32()
+        check(load().fm_run_lfv(self._h, int(iter0), int(nsteps), ms, int(n_macros), C.byref(nrb)))
         return [ms[k].as_dict() for k in range(n_macros)], nrb.value
 
     # ---- inspection ----------------------------------------------------------------------------

@@ -54,8 +54,14 @@ extern "C" {
 #define FM_PREC_FP64 0      /* everything in FP64 (parity path) */
 #define FM_PREC_MIXED 1     /* pair law in FP32; positions, image shifts, MOP tests, accumulators FP64 */
 
-/* per-molecule tags (src/modules.f90:79-93); only these are used on this path */
+/* integration_algorithm, src/modules.f90:112-113 */
+#define FM_INTEGRATION_LFV 0
+#define FM_INTEGRATION_VV 1
+
+/* per-molecule tags (src/modules.f90:78-88); the tags without a thermostat are on this path */
 #define FM_TAG_FREE 0
+#define FM_TAG_FIXED 1
+#define FM_TAG_FIXED_SLIDE 2
 #define FM_TAG_TETH 3
 #define FM_TAG_TETH_SLIDE 6
 
@@ -89,9 +95,11 @@ typedef struct fm_params {
 
     /* ensemble and walls: src/molecular_properties.f90:833-915 */
     int32_t ensemble;            /* FM_ENSEMBLE_NVE or FM_ENSEMBLE_TAG_MOVE */
-    int32_t _pad0;
+    int32_t integration_algorithm; /* FM_INTEGRATION_VV or FM_INTEGRATION_LFV.  Leap-frog applies the
+                                    tethers inside the move, as the reference does; velocity Verlet
+                                    (no tag_move in the reference) applies them with the forces. */
     double teth_k2, teth_k4, teth_k6;
-    double wallslidev[3];        /* slide velocity of FM_TAG_TETH_SLIDE molecules */
+    double wallslidev[3];        /* slide velocity of FM_TAG_TETH_SLIDE (times sign(y)) and FM_TAG_FIXED_SLIDE molecules */
 
     /* stress tallies */
     int32_t vflux_outflag;       /* 0 none; 1,2,3: MOP planes normal to x,y,z (the tally always uses y, as
@@ -162,6 +170,9 @@ double fm_simtime(const fm_ctx* ctx);
 /* ---- the hot path: one call per reference routine ------------------------------------------ */
 /* simulation_move_particles_vv(pass_num), src/simulation_move_particles_vv.f90:44-193 */
 int  fm_move_particles_vv(fm_ctx* ctx, int32_t pass_num);
+/* simulation_move_particles_lfv, src/simulation_move_particles_lfv.f90:38-177 (NVE) and 308-490
+   (tag_move: free, fixed, fixed_slide, teth, teth_slide; tether_force is applied here) */
+int  fm_move_particles_lfv(fm_ctx* ctx);
 /* simulation_checkrebuild(rebuild), src/simulation_checkrebuild.f90:17-174 (criteria 0,1,2) */
 int  fm_checkrebuild(fm_ctx* ctx, int32_t iter, int32_t* rebuild);
 /* linklist_deallocateall, src/linklist.f90:2872-2963 */
@@ -202,6 +213,10 @@ int  fm_reset_cv_stress(fm_ctx* ctx);
    (capacity n_macros). */
 int  fm_run_vv(fm_ctx* ctx, int32_t iter0, int32_t nsteps, fm_macros* macros, int32_t n_macros,
                int32_t* n_rebuilds);
+/* md_advance_lfv (src/md.f90:109-139) nsteps times from iter0: compute_forces, record (macros on
+   record steps, leap-frog kinetic terms), move_particles_lfv, checkrebuild and rebuild. */
+int  fm_run_lfv(fm_ctx* ctx, int32_t iter0, int32_t nsteps, fm_macros* macros, int32_t n_macros,
+                int32_t* n_rebuilds);
 
 /* ---- inspection (parity tests) --------------------------------------------------------------
    Cell membership: cell_of_mol[n] = flattened 1-based-with-halo cell index of molecule n,

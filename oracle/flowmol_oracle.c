@@ -234,10 +234,17 @@ static void read_tag(orc_state* s, int n) {
     if (s->tag[n] == FM_TAG_TETH_SLIDE) {
         double sg = f_sign(1.0, s->r[3 * n + 1]);
         for (int d = 0; d < 3; ++d) s->slidev[3 * n + d] = s->p.wallslidev[d] * sg;
+    } else if (s->tag[n] == FM_TAG_FIXED_SLIDE) {
+        for (int d = 0; d < 3; ++d) s->slidev[3 * n + d] = s->p.wallslidev[d];
     }
 }
 
 static int is_tethered(int tag) { return tag == FM_TAG_TETH || tag == FM_TAG_TETH_SLIDE; }
+/* the tags of src/modules.f90:78-88 without a thermostat */
+static int tag_supported(int tag) {
+    return tag == FM_TAG_FREE || tag == FM_TAG_FIXED || tag == FM_TAG_FIXED_SLIDE || tag == FM_TAG_TETH ||
+           tag == FM_TAG_TETH_SLIDE;
+}
 
 int orc_set_state(orc_state* s, int32_t np, const double* r, const double* v, const double* a) {
     if (np > s->cap) return fail("np %d exceeds capacity %d", np, s->cap);
@@ -260,6 +267,8 @@ int orc_set_moltype(orc_state* s, const int32_t* moltype) {
 }
 
 int orc_set_tags(orc_state* s, const int32_t* tag, const double* rtether) {
+    for (int n = 0; n < s->np; ++n)
+        if (!tag_supported(tag[n])) return fail("Invalid molecular Tag %d for molecule %d", tag[n], n + 1);
     memcpy(s->tag, tag, s->np * sizeof(int32_t));
     if (rtether) memcpy(s->rtether, rtether, 3 * (size_t)s->np * sizeof(double));
     for (int n = 0; n < s->np; ++n) read_tag(s, n);
@@ -824,18 +833,22 @@ static int forces_poly(orc_state* s, int iter) {
 
 /* tether_force (src/molecular_properties.f90:833-915), k2/k4/k6 branch; applied once per force
    evaluation to tethered molecules, after the pair and bond forces. */
+static void tether_force(orc_state* s, int n) {
+    double rio[3];
+    for (int d = 0; d < 3; ++d) rio[d] = s->r[3 * n + d] - s->rtether[3 * n + d];
+    double mag = sqrt((rio[0] * rio[0] + rio[1] * rio[1]) + rio[2] * rio[2]);
+    /* magnitude**2.d0 is an exact square; magnitude**4.d0 is a call of the real power */
+    double acctmag = -2.0 * s->p.teth_k2 * mag - 4.0 * s->p.teth_k4 * (mag * mag) -
+                     6.0 * s->p.teth_k6 * pow(mag, 4.0);
+    for (int d = 0; d < 3; ++d) s->a[3 * n + d] = s->a[3 * n + d] + acctmag * rio[d];
+    for (int d = 0; d < 3; ++d)
+        s->rtether[3 * n + d] = s->rtether[3 * n + d] + s->slidev[3 * n + d] * s->p.delta_t;
+}
+
+/* velocity Verlet has no tag_move in the reference; here the tethers act with the pair forces */
 static void forces_tether(orc_state* s) {
-    for (int n = 0; n < s->np; ++n) {
-        if (!is_tethered(s->tag[n])) continue;
-        double rio[3];
-        for (int d = 0; d < 3; ++d) rio[d] = s->r[3 * n + d] - s->rtether[3 * n + d];
-        double mag = sqrt((rio[0] * rio[0] + rio[1] * rio[1]) + rio[2] * rio[2]);
-        double acctmag = -2.0 * s->p.teth_k2 * mag - 4.0 * s->p.teth_k4 * (mag * mag) -
-                         6.0 * s->p.teth_k6 * p4(mag);
-        for (int d = 0; d < 3; ++d) s->a[3 * n + d] = s->a[3 * n + d] + acctmag * rio[d];
-        for (int d = 0; d < 3; ++d)
-            s->rtether[3 * n + d] = s->rtether[3 * n + d] + s->slidev[3 * n + d] * s->p.delta_t;
-    }
+    for (int n = 0; n < s->np; ++n)
+        if (is_tethered(s->tag[n])) tether_force(s, n);
 }
 
 /* simulation_compute_forces (src/simulation_compute_forces.f90:182-275), force_list 3 */
@@ -851,7 +864,7 @@ int orc_compute_forces(orc_state* s, int32_t iter) {
         memset(s->rfmol, 0, 9 * (size_t)s->cap * sizeof(double));
     if (forces_halfint(s, iter)) return 1;
     if (s->p.potential_flag == 1 && forces_poly(s, iter)) return 1;
-    if (s->p.ensemble == FM_ENSEMBLE_TAG_MOVE) forces_tether(s);
+    if (s->p.ensemble == FM_ENSEMBLE_TAG_MOVE && s->p.integration_algorithm == FM_INTEGRATION_VV) forces_tether(s);
     return 0;
 }
 
@@ -919,6 +932,46 @@ int orc_move_particles_vv(orc_state* s, int32_t pass_num) {
     } else {
         return fail("pass_num must be 1 or 2");
     }
+    return 0;
+}
+
+/* simulation_move_particles_lfv (src/simulation_move_particles_lfv.f90:38-177): NVE, or tag_move
+   through simulation_move_particles_lfv_tag (308-490) for the tags without a thermostat; the
+   tethers act here, inside the move. */
+int orc_move_particles_lfv(orc_state* s) {
+    const double dt = s->p.delta_t;
+    if (s->p.integration_algorithm != FM_INTEGRATION_LFV) return fail("integration_algorithm is not leap-frog");
+    for (int n = 0; n < s->np; ++n) {
+        double* r = &s->r[3 * n];
+        double* v = &s->v[3 * n];
+        double* a = &s->a[3 * n];
+        const double* sv = &s->slidev[3 * n];
+        int tag = s->p.ensemble == FM_ENSEMBLE_TAG_MOVE ? s->tag[n] : FM_TAG_FREE;
+        switch (tag) {
+        case FM_TAG_FREE:
+            for (int d = 0; d < 3; ++d) v[d] = v[d] + dt * a[d];
+            for (int d = 0; d < 3; ++d) r[d] = r[d] + dt * v[d];
+            break;
+        case FM_TAG_FIXED:
+            break;
+        case FM_TAG_FIXED_SLIDE:
+            for (int d = 0; d < 3; ++d) r[d] = r[d] + dt * sv[d];
+            break;
+        case FM_TAG_TETH:
+            tether_force(s, n);
+            for (int d = 0; d < 3; ++d) v[d] = v[d] + dt * a[d];
+            for (int d = 0; d < 3; ++d) r[d] = r[d] + dt * v[d];
+            break;
+        case FM_TAG_TETH_SLIDE:
+            tether_force(s, n);
+            for (int d = 0; d < 3; ++d) v[d] = v[d] + dt * a[d];
+            for (int d = 0; d < 3; ++d) r[d] = r[d] + dt * v[d] + dt * sv[d];
+            break;
+        default:
+            return fail("Invalid molecular Tag %d for molecule %d", tag, n + 1);
+        }
+    }
+    s->simtime = s->simtime + dt;
     return 0;
 }
 
@@ -991,7 +1044,10 @@ int orc_macros(orc_state* s, fm_macros* o) {
         double m = mass_of(s, n);
         msum = msum + m;
         for (int d = 0; d < 3; ++d) {
-            double vel = s->v[3 * n + d];
+            /* leap-frog: velocity shifted half a step, src/simulation_record.f90:1238-1247 */
+            double vel = s->p.integration_algorithm == FM_INTEGRATION_LFV
+                             ? s->v[3 * n + d] + 0.5 * s->a[3 * n + d] * s->p.delta_t
+                             : s->v[3 * n + d];
             vsum = vsum + m * vel;
             mv2 = mv2 + m * (vel * vel);
         }
@@ -1077,6 +1133,30 @@ int orc_run_vv(orc_state* s, int32_t iter0, int32_t nsteps, fm_macros* macros, i
         if (fmod_i(iter, s->p.tplot) == 0 && macros && nm < n_macros) {
             orc_macros(s, &macros[nm]);
             nm++;
+        }
+    }
+    if (n_rebuilds) *n_rebuilds = nrb;
+    return 0;
+}
+
+/* md_advance_lfv (src/md.f90:109-139) nsteps times: forces, record, move, rebuild check */
+int orc_run_lfv(orc_state* s, int32_t iter0, int32_t nsteps, fm_macros* macros, int32_t n_macros,
+                int32_t* n_rebuilds) {
+    int nrb = 0, nm = 0;
+    if (!s->nlist_valid) return fail("run_lfv before the initial neighbour list");
+    for (int iter = iter0; iter < iter0 + nsteps; ++iter) {
+        int32_t rb;
+        if (orc_compute_forces(s, iter)) return 1;
+        if (fmod_i(iter, s->p.tplot) == 0 && macros && nm < n_macros) {
+            orc_macros(s, &macros[nm]);
+            nm++;
+        }
+        if (orc_move_particles_lfv(s)) return 1;
+        if (orc_updateborders(s, 0)) return 1;
+        if (orc_checkrebuild(s, iter, &rb)) return 1;
+        if (rb) {
+            if (rebuild_all(s)) return 1;
+            nrb++;
         }
     }
     if (n_rebuilds) *n_rebuilds = nrb;

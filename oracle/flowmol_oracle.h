@@ -60,6 +60,9 @@ int orc_assign_to_neighbourlist(orc_state* s);
 int orc_compute_forces(orc_state* s, int32_t iter);
 int orc_compute_forces_allpairs(orc_state* s, int32_t iter);
 int orc_move_particles_vv(orc_state* s, int32_t pass_num);
+int orc_move_particles_lfv(orc_state* s);
+int orc_run_lfv(orc_state* s, int32_t iter0, int32_t nsteps, fm_macros* macros, int32_t n_macros,
+                int32_t* n_rebuilds);
 int orc_checkrebuild(orc_state* s, int32_t iter, int32_t* rebuild);
 int orc_macros(orc_state* s, fm_macros* out);
 int orc_get_mop(const orc_state* s, double* Pxy_plane);

@@ -54,6 +54,8 @@ def lib():
             "orc_get_cv_stress": (C.c_int, [vp, dp]),
             "orc_reset_cv_stress": (C.c_int, [vp]),
             "orc_run_vv": (C.c_int, [vp, i32, i32, C.POINTER(FmMacros), i32, ip]),
+            "orc_move_particles_lfv": (C.c_int, [vp]),
+            "orc_run_lfv": (C.c_int, [vp, i32, i32, C.POINTER(FmMacros), i32, ip]),
             "orc_get_cells": (C.c_int, [vp, ip]),
             "orc_neighbour_count": (i64, [vp]),
             "orc_get_neighbourlist": (C.c_int, [vp, ip, ip, ip]),
@@ -176,6 +178,9 @@ class Oracle:
     def move_particles_vv(self, pass_num):
         _check(lib().orc_move_particles_vv(self._h, int(pass_num)))
 
+    def move_particles_lfv(self):
+        _check(lib().orc_move_particles_lfv(self._h))
+
     def checkrebuild(self, iter_):
         rb = C.c_int32()
         _check(lib().orc_checkrebuild(self._h, int(iter_), C.byref(rb)))
@@ -236,6 +241,13 @@ class Oracle:
         ms = (FmMacros * max(n_macros, 1))()
         nrb = C.c_int32()
         _check(lib().orc_run_vv(self._h, int(iter0), int(nsteps), ms, int(n_macros), C.byref(nrb)))
+        return [ms[k].as_dict() for k in range(n_macros)], nrb.value
+
+    def run_lfv(self, iter0, nsteps, n_macros=0):
+        """md_advance_lfv (src/md.f90:109-139) nsteps times."""
+        ms = (FmMacros * max(n_macros, 1))()
+        nrb = C.c_int32()
+        _check(lib().orc_run_lfv(self._h, int(iter0), int(nsteps), ms, int(n_macros), C.byref(nrb)))
         return [ms[k].as_dict() for k in range(n_macros)], nrb.value
 
     # inspection -----------------------------------------------------------------------------

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from flowmol_amd import config
-from flowmol_amd._abi import FM_PREC_MIXED, FM_REBUILD_FIXED, FM_REBUILD_VMAX
+from flowmol_amd._abi import FM_INTEGRATION_LFV, FM_PREC_MIXED, FM_REBUILD_FIXED, FM_REBUILD_VMAX
 from flowmol_amd.md import MDContext
 from flowmol_amd.restart import read_final_state
 from oracle.oracle import Oracle
@@ -210,6 +210,79 @@ def test_couette_walls_short_trajectory():
     assert np.allclose(g.rtether()[top, 0] - rt[top, 0], 30 * p.delta_t * 1.0 + p.delta_t, atol=1e-9)
     Po, Pg = o.mop(), g.mop()
     assert np.abs(Pg - Po).max() <= 1e-10 * max(1.0, np.abs(Po).max())
+    g.close()
+
+
+def _lfv(p):
+    q = type(p).from_buffer_copy(p)
+    q.integration_algorithm = FM_INTEGRATION_LFV
+    return q
+
+
+def lfv_steps(o, g, it0, n):
+    """md_advance_lfv routine by routine on both sides; returns the rebuild steps of each."""
+    ro, rg = [], []
+    for it in range(it0, it0 + n):
+        o.compute_forces(it)
+        o.move_particles_lfv()
+        o.updateborders(0)
+        if o.checkrebuild(it):
+            o.rebuild()
+            ro.append(it)
+        if g.md_advance_lfv(it)[0]:
+            rg.append(it)
+    return ro, rg
+
+
+def test_lfv_couette_walls_trajectory():
+    """Leap-frog with tag_move (the reference's Couette path): tethers inside the move, sliding
+    top wall, MOP every record step; the fused loop repeats the routine-by-routine one bit for bit."""
+    p, r, v, tag, rt = config.config_couette(nunits=(8, 8, 8), tplot=5, nplanes=16)
+    p = _lfv(p)
+    say("case: leap-frog tethered walls, 40 steps")
+    o = Oracle(p, r, v)
+    o.set_tags(tag, rt)
+    g = MDContext(p, r, v)
+    g.set_tags(tag, rt)
+    o.setup()
+    o.compute_forces(0)   # the initial force call of setup (src/setup_initial_record.f90:88)
+    g.setup(0)
+    ro, rg = lfv_steps(o, g, 0, 40)
+    assert ro == rg and len(ro) > 0
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-9
+    assert np.abs(g.rtether() - o.rtether()).max() < 1e-12
+    Po, Pg = o.mop(), g.mop()
+    assert np.abs(Pg - Po).max() <= 1e-10 * max(1.0, np.abs(Po).max())
+    # fused loop from the same start
+    f = MDContext(p, r, v)
+    f.set_tags(tag, rt)
+    f.setup(0)
+    _, nrb = f.run_lfv(0, 40)
+    assert nrb == len(rg)
+    for x, y in zip(f.state(), g.state()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(f.rtether(), g.rtether())
+    f.close()
+    g.close()
+
+
+def test_lfv_nve_macros():
+    """Leap-frog NVE: record-step energies use v + dt/2 a (src/simulation_record.f90:1238-1247)."""
+    p, r, v, a = _lj((6, 6, 6), seed=33, tplot=5)
+    p = _lfv(p)
+    say("case: leap-frog NVE 864 LJ, 30 steps")
+    o = Oracle(p, r, v, a)
+    g = MDContext(p, r, v, a)
+    o.setup()
+    o.compute_forces(0)
+    g.setup(0)
+    mo, nro = o.run_lfv(0, 30, n_macros=6)
+    mg, nrg = g.run_lfv(0, 30, n_macros=6)
+    assert nro == nrg
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-9
+    for x, y in zip(mo, mg):
+        for key in ("kinenergy", "potenergy", "pressure", "totenergy"):
+            assert y[key] == pytest.approx(x[key], rel=1e-11, abs=1e-13), key
     g.close()
 
 

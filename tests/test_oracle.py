@@ -324,6 +324,108 @@ def test_tether_force_is_minus_gradient_and_slides():
     assert o.rtether()[0, 0] == pytest.approx(1.0 * 0.005, abs=1e-15)
 
 
+# ---- leap-frog (simulation_move_particles_lfv) ---------------------------------------------------
+def _match(ra, rb, L):
+    """Largest distance between two position sets after wrapping into the box and pairing each
+    point with its nearest partner (the rebuilds renumber molecules at different steps)."""
+    from scipy.spatial import cKDTree
+    L = np.asarray(L)
+    wa = (ra + 0.5 * L) % L
+    wb = (rb + 0.5 * L) % L
+    d, _ = cKDTree(wb, boxsize=L).query(wa)
+    return float(d.max())
+
+
+def _lfv_params(p):
+    from flowmol_amd._abi import FM_INTEGRATION_LFV
+    q = type(p).from_buffer_copy(p)
+    q.integration_algorithm = FM_INTEGRATION_LFV
+    return q
+
+
+def test_lfv_positions_equal_vv_nve():
+    """Leap-frog and velocity Verlet give the same Stormer-Verlet positions when the leap-frog
+    velocity starts half a step back, v(-dt/2) = v(0) - dt/2 a(0); leap-frog step k records state
+    k with v(k-dt/2) + dt/2 a(k) = v(k), so its energies equal the VV record of step k."""
+    from tests.helpers import equilibrated
+    p, r, v, a = equilibrated(config.config_fcc, 100, nunits=(5, 5, 5), density=0.8, rcutoff=2.5, seed=31,
+                              tplot=5)
+    dt = p.delta_t
+    ov = Oracle(p, r, v, a)
+    ov.setup()
+    ov.compute_forces(0)
+    a0 = ov.state()[2]
+    mv, nrv = ov.run_vv(1, 60, n_macros=12)
+    ol = Oracle(_lfv_params(p), r, v - 0.5 * dt * a0, a0)
+    ol.setup()
+    ml, nrl = ol.run_lfv(0, 60, n_macros=12)
+    assert nrv > 0 and nrl > 0
+    assert _match(ov.state()[0], ol.state()[0], p.domain[:]) < 1e-10
+    # VV records steps 5..60, leap-frog steps 0..55
+    for k in range(11):
+        for key in ("kinenergy", "potenergy", "virial"):
+            assert ml[k + 1][key] == pytest.approx(mv[k][key], rel=1e-10, abs=1e-12), (k, key)
+
+
+def test_lfv_sliding_walls_equal_vv_in_the_lab_frame():
+    """Tethered walls (teth bottom, teth_slide top): leap-frog moves a sliding wall molecule by
+    dt*(v + slidev) with v its peculiar velocity and applies tether_force inside the move; the same
+    positions come from velocity Verlet with the lab velocity v + slidev and the tethers applied
+    with the forces."""
+    from flowmol_amd._abi import FM_TAG_TETH_SLIDE
+    p, r, v, tag, rt = config.config_couette(nunits=(6, 6, 6), tplot=5, nplanes=8)
+    dt = p.delta_t
+    ov = Oracle(p, r, v)
+    ov.set_tags(tag, rt)
+    ov.setup()
+    ov.compute_forces(0)
+    a0 = ov.state()[2]
+    slide = tag == FM_TAG_TETH_SLIDE
+    assert slide.any()
+    # restart the VV run from the lab velocity
+    vlab = v.copy()
+    vlab[slide, 0] += p.wallslidev[0]
+    ov = Oracle(p, r, vlab)
+    ov.set_tags(tag, rt)
+    ov.setup()
+    ov.compute_forces(0)
+    ov.run_vv(1, 40)
+    ol = Oracle(_lfv_params(p), r, v - 0.5 * dt * a0, a0)
+    ol.set_tags(tag, rt)
+    ol.setup()
+    ol.run_lfv(0, 40)
+    assert _match(ov.state()[0], ol.state()[0], p.domain[:]) < 1e-9
+    # the tether sites of the sliding wall moved by 40 dt wallslidev, the others stayed
+    expect = rt.copy()
+    expect[slide, 0] += 40 * dt * p.wallslidev[0]
+    assert _match(ol.rtether(), expect, p.domain[:]) < 1e-12
+
+
+def test_lfv_fixed_and_fixed_slide_tags():
+    """fixed molecules stay put whatever the force on them; fixed_slide molecules move by
+    dt*wallslidev per step (read_tag: slidev = wallslidev, no sign); free molecules feel them."""
+    from flowmol_amd._abi import (FM_ENSEMBLE_TAG_MOVE, FM_TAG_FIXED, FM_TAG_FIXED_SLIDE, FM_TAG_FREE,
+                                  FM_INTEGRATION_LFV)
+    p = box_params(ensemble=FM_ENSEMBLE_TAG_MOVE, integration=FM_INTEGRATION_LFV, tplot=1000)
+    p.wallslidev[0] = 0.7
+    pos = np.array([[0.0, -5.0, 0.0], [1.0, -5.0, 0.0], [5.0, 5.0, 5.0], [-5.0, 4.0, -3.0]])
+    vel = np.array([[0.3, 0.0, 0.0], [0.0, 0.0, 0.0], [0.1, 0.2, 0.3], [0.0, 0.0, 0.0]])
+    tags = np.array([FM_TAG_FIXED, FM_TAG_FREE, FM_TAG_FREE, FM_TAG_FIXED_SLIDE], np.int32)
+    o = Oracle(p, pos, vel)
+    o.set_tags(tags)
+    o.setup()
+    n = 30
+    o.run_lfv(0, n)
+    r, v, a = o.state()
+    assert np.array_equal(r[0], pos[0]) and np.array_equal(v[0], vel[0])
+    assert r[1, 0] > pos[1, 0] + 0.01                    # pushed away by the fixed neighbour
+    assert np.allclose(r[2], pos[2] + n * p.delta_t * vel[2], rtol=0, atol=1e-13)  # free flight
+    assert np.allclose(r[3], pos[3] + n * p.delta_t * np.array([0.7, 0.0, 0.0]), rtol=0, atol=1e-13)
+    # thermostatted tags are not on this path
+    with pytest.raises(OracleError):
+        o.set_tags(np.array([4, 0, 0, 0], np.int32))
+
+
 # ---- MOP -----------------------------------------------------------------------------------------
 def test_mop_single_crossing():
     L = 12.0
