@@ -396,11 +396,10 @@ struct FbArgs {
 };
 
 // forward cell offsets of the reference's half-list scan (src/linklist.f90:686-688)
+// (dz = 1: all nine; dz = 0: (1,0), (1,1), (0,1), (-1,1))
 __device__ inline bool is_forward(int dx, int dy, int dz) {
-    if (dz == 1) return true;
-    if (dz == -1) return false;
-    if (dy == 1) return true;
-    if (dy == -1) return dx == 1;
+    if (dz != 0) return dz == 1;
+    if (dy != 0) return dy == 1;
     return dx == 1;
 }
 
@@ -489,7 +488,9 @@ template <bool MIE, bool RECORD, bool EXTRA>
 __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* smem) {
     Region r;
     const bool mop = RECORD && a.pxy != nullptr;
-    carve(r, smem, a.g, a.maxreg, mop || MIE);
+    // the stress tallies need each staged molecule's slot (molecule numbers for the owner rule)
+    const bool cvrec = RECORD && a.cv != nullptr;
+    carve(r, smem, a.g, a.maxreg, mop || MIE || cvrec);
     double* spxy = nullptr;
     if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
     stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr);
@@ -671,7 +672,8 @@ template <bool RECORD, bool EXTRA>
 __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, char* smem) {
     Region r;
     const bool mop = RECORD && a.pxy != nullptr;
-    carve(r, smem, a.g, a.maxreg, mop);
+    const bool cvrec = RECORD && a.cv != nullptr;
+    carve(r, smem, a.g, a.maxreg, mop || cvrec);
     double* spxy = nullptr;
     if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
     stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr);
@@ -855,7 +857,7 @@ size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind) {
         bool poly = c->p.potential_flag == 1;
         return region_lds_bytes(g, maxreg, poly) + (poly ? 2 * (size_t)maxreg * sizeof(int) : 0);
     }
-    if (kind == 1) return region_lds_bytes(g, maxreg, c->p.mie_potential == 1);
+    if (kind == 1) return region_lds_bytes(g, maxreg, c->p.mie_potential == 1 || c->p.vflux_outflag == 4);
     return region_lds_bytes(g, maxreg, true) + 3 * (size_t)c->p.nplanes * sizeof(double);
 }
 
