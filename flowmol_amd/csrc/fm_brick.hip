@@ -260,6 +260,7 @@ struct NlbArgs {
     double rn2;
     int xsorted;     // cells are sorted by x (positions unchanged since binning)
     double xwin;     // window half-width: above sqrt(rn2), so no pair inside rn2 is skipped
+    double cs[3], hd[3];  // cell side and half domain (cell faces in image coordinates)
 };
 
 __device__ inline int get_bondflag(const int* mono, int cap, int i, int scID) {
@@ -303,6 +304,15 @@ __global__ __launch_bounds__(FM_BLOCK) void k_nlist_brick(NlbArgs a) {
                 funcy = a.mono[slot + 2 * cap];
             }
             uint64_t word = 0;
+            // Distances from the molecule to its cell's faces in y and z (cell faces in image
+            // coordinates; at a rebuild every molecule lies in its cell).  A run of cells one row
+            // up or down is at least that far away, which narrows its x window, or removes the
+            // run when the faces alone are beyond rn.  The margin keeps the window conservative.
+            const int by = (blockIdx.x / a.g.nb[0]) % a.g.nb[1], bz = blockIdx.x / (a.g.nb[0] * a.g.nb[1]);
+            const int vy = by * a.g.B[1] + ry - 1, vz = bz * a.g.B[2] + rz - 1;
+            const double ylo = -a.hd[1] + vy * a.cs[1], zlo = -a.hd[2] + vz * a.cs[2];
+            const double gyl = fmax(yi - ylo - 1e-9, 0.0), gyh = fmax(ylo + a.cs[1] - yi - 1e-9, 0.0);
+            const double gzl = fmax(zi - zlo - 1e-9, 0.0), gzh = fmax(zlo + a.cs[2] - zi - 1e-9, 0.0);
             // the 27 neighbour cells as 9 runs of three x-consecutive region cells, which are
             // contiguous in the LDS layout
             for (int dz = -1; dz <= 1; ++dz)
@@ -310,8 +320,13 @@ __global__ __launch_bounds__(FM_BLOCK) void k_nlist_brick(NlbArgs a) {
                     const int c0 = (rx - 1) + R0 * ((ry + dy) + R1 * (rz + dz));
                     int jb = r.off[c0], je = r.off[c0 + 3];
                     if (!POLY && a.xsorted) {
-                        // the run is sorted by x: keep only xi - xwin <= x <= xi + xwin
-                        const double xlo = xi - a.xwin, xhi = xi + a.xwin;
+                        const double gy = dy < 0 ? gyl : (dy > 0 ? gyh : 0.0);
+                        const double gz = dz < 0 ? gzl : (dz > 0 ? gzh : 0.0);
+                        const double w2 = a.rn2 - (gy * gy + gz * gz);
+                        if (!(w2 > 0.0)) continue;
+                        const double xw = (dy == 0 && dz == 0) ? a.xwin : sqrt(w2) * (1.0 + 1e-9) + 1e-12;
+                        // the run is sorted by x: keep only xi - xw <= x <= xi + xw
+                        const double xlo = xi - xw, xhi = xi + xw;
                         int lo = jb, hi = je;
                         while (lo < hi) {
                             const int mid = (lo + hi) >> 1;
@@ -891,6 +906,10 @@ int launch_nlist(fm_ctx* c) {
     a.rn2 = c->rn2;
     a.xsorted = c->moved_since_bin ? 0 : 1;
     a.xwin = sqrt(c->rn2) * (1.0 + 1e-9) + 1e-12;
+    for (int d = 0; d < 3; ++d) {
+        a.cs[d] = c->cs[d];
+        a.hd[d] = c->half[d];
+    }
     const size_t lds = brick_lds_bytes(c, c->maxreg, 0);
     ktimer_begin(c, KT_NLIST);
     if (a.mono) k_nlist_brick<true><<<brick_count(c), FM_BLOCK, lds, c->stream>>>(a);
