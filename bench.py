@@ -65,14 +65,16 @@ def copy_calibration(mb):
 
 
 def traffic_from_profile():
-    """HBM bytes per force launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), scaled
-    by the same counters on a copy of known size; see profiles/forces_c2_traffic.json."""
-    path = os.path.join(ROOT, "profiles", "forces_c2_traffic.json")
+    """HBM bytes per force launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), each
+    scaled by the same counter on a device copy of known size; the summary with its commands is
+    profiles/forces_c2_traffic.md."""
+    path = os.path.join(ROOT, "profiles", "forces_c2_traffic.md")
     if not os.path.exists(path):
         return None
-    with open(path) as f:
-        d = json.load(f)
-    return d.get("bytes_per_launch")
+    for line in open(path):
+        if line.startswith("bytes_per_launch"):
+            return int(line.split(":")[1].split()[0])
+    return None
 
 
 def serve_main(a):

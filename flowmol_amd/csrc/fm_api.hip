@@ -147,9 +147,9 @@ static int alloc_nlist(fm_ctx* c, int maxnb) {
 }
 
 // Brick shape: the largest number of cells whose expected molecule count fits one pass of the
-// workgroup (240 of 256 threads), preferring the shape with the least staged halo per cell.
+// workgroup (15/16 of its threads), preferring the shape with the least staged halo per cell.
 static void choose_brick(const int nc[3], double occ, int out[3]) {
-    const double target = 240.0;
+    const double target = FB_BLOCK * 15.0 / 16.0;
     double best = -1.0;
     out[0] = out[1] = out[2] = 1;
     for (int bx = 1; bx <= std::min(nc[0], 12); ++bx)

@@ -274,7 +274,7 @@ __device__ inline int get_bondflag(const int* mono, int cap, int i, int scID) {
 }
 
 template <bool POLY>
-__global__ __launch_bounds__(FM_BLOCK) void k_nlist_brick(NlbArgs a) {
+__global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
     extern __shared__ __align__(16) char smem[];
     Region r;
     carve(r, smem, a.g, a.maxreg, POLY);
@@ -666,14 +666,14 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
 }
 
 template <bool MIE, bool RECORD, bool EXTRA>
-__global__ __launch_bounds__(FM_BLOCK) void k_forces_brick(FbArgs a) {
+__global__ __launch_bounds__(FB_BLOCK) void k_forces_brick(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     forces_body<MIE, RECORD, EXTRA>(a, blockIdx.x, smem);
 }
 
 // packed launch over many domains: block b works on brick table[b].y of domain table[b].x
 template <bool MIE, bool EXTRA>
-__global__ __launch_bounds__(FM_BLOCK) void k_forces_brick_multi(const FbArgs* descs, const int2* table) {
+__global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_multi(const FbArgs* descs, const int2* table) {
     extern __shared__ __align__(16) char smem[];
     const int2 t = table[blockIdx.x];
     const FbArgs& a = descs[t.x];
@@ -780,13 +780,13 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
 }
 
 template <bool RECORD, bool EXTRA>
-__global__ __launch_bounds__(FM_BLOCK) void k_forces_brick_mixed(FbArgs a) {
+__global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_mixed(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     forces_body_mixed<RECORD, EXTRA>(a, blockIdx.x, smem);
 }
 
 template <bool EXTRA>
-__global__ __launch_bounds__(FM_BLOCK) void k_forces_brick_mixed_multi(const FbArgs* descs, const int2* table) {
+__global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_mixed_multi(const FbArgs* descs, const int2* table) {
     extern __shared__ __align__(16) char smem[];
     const int2 t = table[blockIdx.x];
     const FbArgs& a = descs[t.x];
@@ -912,8 +912,8 @@ int launch_nlist(fm_ctx* c) {
     }
     const size_t lds = brick_lds_bytes(c, c->maxreg, 0);
     ktimer_begin(c, KT_NLIST);
-    if (a.mono) k_nlist_brick<true><<<brick_count(c), FM_BLOCK, lds, c->stream>>>(a);
-    else k_nlist_brick<false><<<brick_count(c), FM_BLOCK, lds, c->stream>>>(a);
+    if (a.mono) k_nlist_brick<true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
+    else k_nlist_brick<false><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
     ktimer_end(c, KT_NLIST);
     LAUNCH_CHECK();
     return 0;
@@ -1002,7 +1002,7 @@ int launch_forces(fm_ctx* c, int iter, bool kick) {
     if (int rc = make_fargs(c, iter, kick, &a, &lds, &kind)) return rc;
     const bool mixed = kind & 1, mie = kind & 2, extra = kind & 4;
     ktimer_begin(c, KT_FORCES);
-    dim3 grid(brick_count(c)), blk(FM_BLOCK);
+    dim3 grid(brick_count(c)), blk(FB_BLOCK);
 #define FB(M, R, E) k_forces_brick<M, R, E><<<grid, blk, lds, c->stream>>>(a)
 #define FM(R, E) k_forces_brick_mixed<R, E><<<grid, blk, lds, c->stream>>>(a)
     if (mixed) {
@@ -1027,14 +1027,14 @@ int launch_forces_multi(hipStream_t st, int kind, const void* d_descs, const int
     const FbArgs* d = (const FbArgs*)d_descs;
     const bool mixed = kind & 1, mie = kind & 2, extra = kind & 4;
     if (mixed) {
-        if (extra) k_forces_brick_mixed_multi<true><<<nblk, FM_BLOCK, lds, st>>>(d, d_table);
-        else k_forces_brick_mixed_multi<false><<<nblk, FM_BLOCK, lds, st>>>(d, d_table);
+        if (extra) k_forces_brick_mixed_multi<true><<<nblk, FB_BLOCK, lds, st>>>(d, d_table);
+        else k_forces_brick_mixed_multi<false><<<nblk, FB_BLOCK, lds, st>>>(d, d_table);
     } else if (mie) {
-        if (extra) k_forces_brick_multi<true, true><<<nblk, FM_BLOCK, lds, st>>>(d, d_table);
-        else k_forces_brick_multi<true, false><<<nblk, FM_BLOCK, lds, st>>>(d, d_table);
+        if (extra) k_forces_brick_multi<true, true><<<nblk, FB_BLOCK, lds, st>>>(d, d_table);
+        else k_forces_brick_multi<true, false><<<nblk, FB_BLOCK, lds, st>>>(d, d_table);
     } else {
-        if (extra) k_forces_brick_multi<false, true><<<nblk, FM_BLOCK, lds, st>>>(d, d_table);
-        else k_forces_brick_multi<false, false><<<nblk, FM_BLOCK, lds, st>>>(d, d_table);
+        if (extra) k_forces_brick_multi<false, true><<<nblk, FB_BLOCK, lds, st>>>(d, d_table);
+        else k_forces_brick_multi<false, false><<<nblk, FB_BLOCK, lds, st>>>(d, d_table);
     }
     LAUNCH_CHECK();
     return 0;

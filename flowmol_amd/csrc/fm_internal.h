@@ -20,6 +20,10 @@
 #include "../../include/flowmol_hip.h"
 
 #define FM_BLOCK 256
+// workgroup size of the brick kernels (neighbour list, forces): one workgroup per brick, so a
+// larger workgroup shares one staged region among more waves (less LDS per wave, less halo per
+// molecule).  The brick shape is chosen so that a brick holds about this many molecules.
+#define FB_BLOCK 512
 
 // device flag words (d_flags)
 enum {
