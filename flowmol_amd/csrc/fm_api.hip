@@ -127,7 +127,7 @@ static void free_ctx(fm_ctx* c) {
                     c->cell_count, c->cell_start, c->cell_cursor, c->perm, c->nbl, c->nbn, c->bond, c->bondn,
                     c->pe_lj, c->pe_poly, c->vir, c->rf, c->pxy, c->cvface, c->planes, c->out_flag, c->out_vals,
                     c->out_slot, c->out_count, c->red, c->d_macros, c->d_mie, c->d_vmax, c->d_rmax,
-                    c->d_flags, c->stage, c->cub_tmp};
+                    c->d_flags, c->stage, c->cub_tmp, c->brick_order, c->part};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->h_flags) hipHostFree(c->h_flags);
@@ -331,6 +331,34 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
         if ((rc = dalloc(&c->cvface, 9 * nbo)) || (rc = launch_fill_f64(c, c->cvface, 0.0, 9 * nbo))) {
             free_ctx(c);
             return rc;
+        }
+    }
+    {
+        // Brick order for the brick kernels: tiles of 4x4x4 bricks, x fastest, bricks x fastest
+        // inside a tile.  Workgroups are dealt to the 8 XCDs in turn and each XCD takes a
+        // contiguous run of this order, so the bricks running together on one XCD form compact
+        // blocks that share their staged halo cells in that XCD's L2.
+        int nb[3];
+        for (int d = 0; d < 3; ++d) nb[d] = (c->nc[d] + c->brick[d] - 1) / c->brick[d];
+        const int T = 4;
+        std::vector<int> order;
+        order.reserve((size_t)nb[0] * nb[1] * nb[2]);
+        for (int tz = 0; tz < nb[2]; tz += T)
+            for (int ty = 0; ty < nb[1]; ty += T)
+                for (int tx = 0; tx < nb[0]; tx += T)
+                    for (int z = tz; z < std::min(tz + T, nb[2]); ++z)
+                        for (int y = ty; y < std::min(ty + T, nb[1]); ++y)
+                            for (int x = tx; x < std::min(tx + T, nb[0]); ++x)
+                                order.push_back(x + nb[0] * (y + nb[1] * z));
+        c->npart = (int)std::max<size_t>(order.size(), ((size_t)c->ncell + 3) / 4) + 1;
+        if ((rc = dalloc(&c->part, (size_t)c->npart))) {
+            free_ctx(c);
+            return rc;
+        }
+        if ((rc = dalloc(&c->brick_order, order.size())) ||
+            hipMemcpy(c->brick_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            free_ctx(c);
+            return rc ? rc : fm_set_error(FM_EHIP, "brick order upload failed");
         }
     }
     {

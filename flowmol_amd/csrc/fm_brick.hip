@@ -42,13 +42,6 @@ __device__ inline void set_err(int* flags, int code, int mol) {
     if (atomicCAS(&flags[FLAG_ERROR], 0, code) == 0) flags[FLAG_ERROR_MOL] = mol;
 }
 
-// one atomicMax per wavefront instead of one per lane
-__device__ inline void wave_atomic_max(int* addr, int v, bool active) {
-    int m = active ? v : INT_MIN;
-    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0 && m != INT_MIN && m > __atomic_load_n(addr, __ATOMIC_RELAXED)) atomicMax(addr, m);
-}
-
 // 1/x from v_rcp_f64 refined by two Newton steps (within an ulp of the IEEE quotient; the
 // pair law needs 1e-12 relative, not the reference's last bit)
 __device__ inline double recip(double x) {
@@ -244,6 +237,14 @@ __device__ inline int region_cell_of(const Region& r, int nreg, int q) {
     return lo;
 }
 
+// Brick of workgroup b of a one-domain launch (gridDim.x == number of bricks).  Workgroups are
+// dealt to the 8 XCDs in turn; XCD x takes the contiguous run x of the tiled brick order.
+__device__ inline int brick_of_block(const int* order, int b, int nb) {
+    if (!order) return b;
+    const int x = b & 7, k = b >> 3, base = nb >> 3, rem = nb & 7;
+    return order[x * base + min(x, rem) + k];
+}
+
 // ---- neighbour list: src/linklist.f90:660-804 (set semantics), bonds 2542-2584 -----------------
 struct NlbArgs {
     St s;
@@ -261,6 +262,8 @@ struct NlbArgs {
     int xsorted;     // cells are sorted by x (positions unchanged since binning)
     double xwin;     // window half-width: above sqrt(rn2), so no pair inside rn2 is skipped
     double cs[3], hd[3];  // cell side and half domain (cell faces in image coordinates)
+    const int* order;     // brick order (brick_of_block)
+    int* part;            // per-workgroup longest list
 };
 
 __device__ inline int get_bondflag(const int* mono, int cap, int i, int scID) {
@@ -284,8 +287,9 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
         lchain = r.ic_rc + a.g.B[0] * a.g.B[1] * a.g.B[2];
         lsub = lchain + a.maxreg;
     }
-    stage_region(r, a.g, blockIdx.x, a.start, a.s, true, a.mono, a.cap, lchain, lsub);
-    if (r.total > a.maxreg) return;  // cannot happen: maxreg is the measured maximum
+    const int brick = brick_of_block(a.order, blockIdx.x, gridDim.x);
+    stage_region(r, a.g, brick, a.start, a.s, true, a.mono, a.cap, lchain, lsub);
+    int wmax = 0;  // longest list of this wavefront
     const size_t cap = a.cap;
     const int R0 = a.g.R[0], R1 = a.g.R[1];
     const int nrounds = (r.ni + blockDim.x - 1) / blockDim.x;
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
             // coordinates; at a rebuild every molecule lies in its cell).  A run of cells one row
             // up or down is at least that far away, which narrows its x window, or removes the
             // run when the faces alone are beyond rn.  The margin keeps the window conservative.
-            const int by = (blockIdx.x / a.g.nb[0]) % a.g.nb[1], bz = blockIdx.x / (a.g.nb[0] * a.g.nb[1]);
+            const int by = (brick / a.g.nb[0]) % a.g.nb[1], bz = brick / (a.g.nb[0] * a.g.nb[1]);
             const int vy = by * a.g.B[1] + ry - 1, vz = bz * a.g.B[2] + rz - 1;
             const double ylo = -a.hd[1] + vy * a.cs[1], zlo = -a.hd[2] + vz * a.cs[2];
             const double gyl = fmax(yi - ylo - 1e-9, 0.0), gyh = fmax(ylo + a.cs[1] - yi - 1e-9, 0.0);
@@ -370,7 +374,19 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                 if (nb > funcy) set_err(a.flags, FM_EBOND, a.ext[slot]);
             }
         }
-        wave_atomic_max(&a.flags[FLAG_MAX_NB], cnt, active);
+        int m = active ? cnt : 0;
+        for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+        wmax = max(wmax, m);
+    }
+    // longest list of the workgroup into part[blockIdx.x] (reduced by k_max_reduce); the region
+    // tables are no longer read, so their first words hold the per-wave values
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) r.off[threadIdx.x >> 6] = wmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = max(m, r.off[w]);
+        a.part[blockIdx.x] = m;
     }
 }
 
@@ -405,6 +421,7 @@ struct FbArgs {
     int nplanes;
     double hp, spacing, shift;
     const double* mie;     // [7][FM_MAX_TYPES^2] + mass[FM_MAX_TYPES]
+    const int* order;      // brick order of one-domain launches (brick_of_block)
     double* cv;            // faces 1..3 of the control-volume tally, or null
     int cvnbo[3], cvnhb[3];
     double cvbs[3], cvhd[3];
@@ -499,7 +516,8 @@ __device__ inline bool mop_owner(const Region& r, const BG& g, int rci, int rcj,
     return is_forward(dx, dy, dz);
 }
 
-template <bool MIE, bool RECORD, bool EXTRA>
+// VIR: virial on a non-record step (teval); record steps always tally it
+template <bool MIE, bool RECORD, bool EXTRA, bool VIR = RECORD>
 __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* smem) {
     Region r;
     const bool mop = RECORD && a.pxy != nullptr;
@@ -533,15 +551,27 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         }
         const int exti = (mop || (RECORD && a.cv)) ? a.ext[i] : 0;
         const uint64_t* row = a.nbl + i;
-        // four 16-bit entries per word: unrolled so the shifts are constants and the four
-        // LDS reads of a word can be in flight together
+        // Four 16-bit entries per word.  The next word is loaded while this one is worked on,
+        // and the four entries' positions are read from LDS together (the unused entries of a
+        // last, partial word are 0, a valid region index).
+        uint64_t wnext = n > 0 ? row[0] : 0;
         for (int kw = 0; kw < n; kw += 4) {
-            const uint64_t word = row[(size_t)(kw >> 2) * cap];
+            const uint64_t word = wnext;
+            if (kw + 4 < n) wnext = row[(size_t)((kw >> 2) + 1) * cap];
+            int jq[4];
+            double rxq[4], ryq[4], rzq[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                jq[q] = (int)((word >> (16 * q)) & 0xffffu);
+                rxq[q] = xi - r.lx[jq[q]];
+                ryq[q] = yi - r.ly[jq[q]];
+                rzq[q] = zi - r.lz[jq[q]];
+            }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
             if (kw + q >= n) break;
-            const int j = (int)((word >> (16 * q)) & 0xffffu);
-            const double rx = xi - r.lx[j], ry = yi - r.ly[j], rz = zi - r.lz[j];
+            const int j = jq[q];
+            const double rx = rxq[q], ry = ryq[q], rz = rzq[q];
             const double r2 = (rx * rx + ry * ry) + rz * rz;
             double rc2 = a.rc2;
             int t = 0;
@@ -571,7 +601,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                 fy = fmaf64(acc, ry, fy);
                 fz = fmaf64(acc, rz, fz);
             }
-            if (RECORD || a.ev) vir = vir + acc * r2;
+            if (VIR) vir = vir + acc * r2;
             if (RECORD) {
                 pe = pe + e;
                 if (do_rf) {
@@ -613,7 +643,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                 fx = fx + acc * rx / mi;
                 fy = fy + acc * ry / mi;
                 fz = fz + acc * rz / mi;
-                if (RECORD || a.ev) vir = vir + acc * r2;
+                if (VIR) vir = vir + acc * r2;
                 if (RECORD) {
                     if (a.cv) {
                         const double ri[3] = {xi, yi, zi}, rj[3] = {xj, yj, zj};
@@ -651,7 +681,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
             a.s.vy[i] = a.s.vy[i] + a.hdt * fy;
             a.s.vz[i] = a.s.vz[i] + a.hdt * fz;
         }
-        if (RECORD || a.ev) a.vir[i] = vir;
+        if (VIR) a.vir[i] = vir;
         if (RECORD) {
             a.pe_lj[i] = pe;
             if (do_rf)
@@ -665,10 +695,10 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
     }
 }
 
-template <bool MIE, bool RECORD, bool EXTRA>
+template <bool MIE, bool RECORD, bool EXTRA, bool VIR>
 __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
-    forces_body<MIE, RECORD, EXTRA>(a, blockIdx.x, smem);
+    forces_body<MIE, RECORD, EXTRA, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // packed launch over many domains: block b works on brick table[b].y of domain table[b].x
@@ -678,12 +708,13 @@ __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_multi(const FbArgs* d
     const int2 t = table[blockIdx.x];
     const FbArgs& a = descs[t.x];
     if (a.rec) forces_body<MIE, true, EXTRA>(a, t.y, smem);
+    else if (a.ev) forces_body<MIE, false, EXTRA, true>(a, t.y, smem);
     else forces_body<MIE, false, EXTRA>(a, t.y, smem);
 }
 
 // Mixed precision: image coordinates and differences in FP64, the pair law and the per-molecule
 // force sum in FP32; energy, virial and the tallies accumulated in FP64.
-template <bool RECORD, bool EXTRA>
+template <bool RECORD, bool EXTRA, bool VIR = RECORD>
 __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, char* smem) {
     Region r;
     const bool mop = RECORD && a.pxy != nullptr;
@@ -707,15 +738,25 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
         double pe = 0.0, vir = 0.0;
         const int exti = (mop || (RECORD && a.cv)) ? a.ext[i] : 0;
         const uint64_t* row = a.nbl + i;
-        // four 16-bit entries per word: unrolled so the shifts are constants and the four
-        // LDS reads of a word can be in flight together
+        // next word prefetched; the four entries' positions read together (see forces_body)
+        uint64_t wnext = n > 0 ? row[0] : 0;
         for (int kw = 0; kw < n; kw += 4) {
-            const uint64_t word = row[(size_t)(kw >> 2) * cap];
+            const uint64_t word = wnext;
+            if (kw + 4 < n) wnext = row[(size_t)((kw >> 2) + 1) * cap];
+            int jq[4];
+            double rxq[4], ryq[4], rzq[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                jq[q] = (int)((word >> (16 * q)) & 0xffffu);
+                rxq[q] = xi - r.lx[jq[q]];
+                ryq[q] = yi - r.ly[jq[q]];
+                rzq[q] = zi - r.lz[jq[q]];
+            }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
             if (kw + q >= n) break;
-            const int j = (int)((word >> (16 * q)) & 0xffffu);
-            const double rxd = xi - r.lx[j], ryd = yi - r.ly[j], rzd = zi - r.lz[j];
+            const int j = jq[q];
+            const double rxd = rxq[q], ryd = ryq[q], rzd = rzq[q];
             const float rx = (float)rxd, ry = (float)ryd, rz = (float)rzd;
             float acc, inv;
             {
@@ -729,7 +770,7 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
                 fy += acc * ry;
                 fz += acc * rz;
             }
-            if (RECORD || a.ev) vir = vir + (double)acc * ((rxd * rxd + ryd * ryd) + rzd * rzd);
+            if (VIR) vir = vir + (double)acc * ((rxd * rxd + ryd * ryd) + rzd * rzd);
             if (RECORD) {
                 const double id = (double)inv;
                 pe = pe + (4.0 * (p6(id) - p3(id)) - a.potshift);
@@ -769,7 +810,7 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
             a.s.vy[i] = a.s.vy[i] + a.hdt * dfy;
             a.s.vz[i] = a.s.vz[i] + a.hdt * dfz;
         }
-        if (RECORD || a.ev) a.vir[i] = vir;
+        if (VIR) a.vir[i] = vir;
         if (RECORD) a.pe_lj[i] = pe;
     }
     if (mop) {
@@ -779,10 +820,10 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
     }
 }
 
-template <bool RECORD, bool EXTRA>
+template <bool RECORD, bool EXTRA, bool VIR>
 __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_mixed(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
-    forces_body_mixed<RECORD, EXTRA>(a, blockIdx.x, smem);
+    forces_body_mixed<RECORD, EXTRA, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 template <bool EXTRA>
@@ -791,6 +832,7 @@ __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_mixed_multi(const FbA
     const int2 t = table[blockIdx.x];
     const FbArgs& a = descs[t.x];
     if (a.rec) forces_body_mixed<true, EXTRA>(a, t.y, smem);
+    else if (a.ev) forces_body_mixed<false, EXTRA, true>(a, t.y, smem);
     else forces_body_mixed<false, EXTRA>(a, t.y, smem);
 }
 
@@ -813,8 +855,8 @@ __global__ __launch_bounds__(FM_BLOCK) void k_decode_brick(St s, BG g, const int
     }
 }
 
-// largest staged region (LDS sizing): one wavefront per brick sums its region cells
-__global__ __launch_bounds__(FM_BLOCK) void k_region_max(BG g, const int* start, int nbrick, int* out) {
+// staged region sizes (LDS sizing): one wavefront per brick sums its region cells into part[brick]
+__global__ __launch_bounds__(FM_BLOCK) void k_region_max(BG g, const int* start, int nbrick, int* part) {
     const int b = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     if (b >= nbrick) return;  // whole wavefront
@@ -833,7 +875,7 @@ __global__ __launch_bounds__(FM_BLOCK) void k_region_max(BG g, const int* start,
         tot += start[c + 1] - start[c];
     }
     for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
-    if (lane == 0 && tot > __atomic_load_n(out, __ATOMIC_RELAXED)) atomicMax(out, tot);
+    if (lane == 0) part[b] = tot;
 }
 
 inline int nblk(size_t n) { return (int)((n + FM_BLOCK - 1) / FM_BLOCK); }
@@ -880,9 +922,9 @@ size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind) {
 int launch_region_max(fm_ctx* c) {
     BG g = make_bg(c);
     int nbrick = g.nb[0] * g.nb[1] * g.nb[2];
-    k_region_max<<<nblk((size_t)nbrick * 64), FM_BLOCK, 0, c->stream>>>(g, c->cell_start, nbrick,
-                                                                          c->d_flags + FLAG_MAX_REGION);
+    k_region_max<<<nblk((size_t)nbrick * 64), FM_BLOCK, 0, c->stream>>>(g, c->cell_start, nbrick, c->part);
     LAUNCH_CHECK();
+    if (int rc = launch_max_reduce(c, nbrick, FLAG_MAX_REGION)) return rc;
     return 0;
 }
 
@@ -906,6 +948,8 @@ int launch_nlist(fm_ctx* c) {
     a.rn2 = c->rn2;
     a.xsorted = c->moved_since_bin ? 0 : 1;
     a.xwin = sqrt(c->rn2) * (1.0 + 1e-9) + 1e-12;
+    a.order = c->brick_order;
+    a.part = c->part;
     for (int d = 0; d < 3; ++d) {
         a.cs[d] = c->cs[d];
         a.hd[d] = c->half[d];
@@ -914,8 +958,9 @@ int launch_nlist(fm_ctx* c) {
     ktimer_begin(c, KT_NLIST);
     if (a.mono) k_nlist_brick<true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
     else k_nlist_brick<false><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
-    ktimer_end(c, KT_NLIST);
     LAUNCH_CHECK();
+    if (int rc = launch_max_reduce(c, brick_count(c), FLAG_MAX_NB)) return rc;
+    ktimer_end(c, KT_NLIST);
     return 0;
 }
 
@@ -973,6 +1018,7 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     a.spacing = c->p.planespacing;
     a.shift = c->mop_shift;
     a.mie = mie ? c->d_mie : nullptr;
+    a.order = c->brick_order;
     a.cv = (a.rec && c->cvface) ? c->cvface : nullptr;
     for (int d = 0; d < 3; ++d) {
         a.cvnbo[d] = c->cv_nbo[d];
@@ -1003,8 +1049,14 @@ int launch_forces(fm_ctx* c, int iter, bool kick) {
     const bool mixed = kind & 1, mie = kind & 2, extra = kind & 4;
     ktimer_begin(c, KT_FORCES);
     dim3 grid(brick_count(c)), blk(FB_BLOCK);
-#define FB(M, R, E) k_forces_brick<M, R, E><<<grid, blk, lds, c->stream>>>(a)
-#define FM(R, E) k_forces_brick_mixed<R, E><<<grid, blk, lds, c->stream>>>(a)
+#define FB(M, R, E) \
+    (R ? k_forces_brick<M, true, E, true><<<grid, blk, lds, c->stream>>>(a) \
+       : (a.ev ? k_forces_brick<M, false, E, true><<<grid, blk, lds, c->stream>>>(a) \
+               : k_forces_brick<M, false, E, false><<<grid, blk, lds, c->stream>>>(a)))
+#define FM(R, E) \
+    (R ? k_forces_brick_mixed<true, E, true><<<grid, blk, lds, c->stream>>>(a) \
+       : (a.ev ? k_forces_brick_mixed<false, E, true><<<grid, blk, lds, c->stream>>>(a) \
+               : k_forces_brick_mixed<false, E, false><<<grid, blk, lds, c->stream>>>(a)))
     if (mixed) {
         if (extra) { if (a.rec) FM(true, true); else FM(false, true); }
         else { if (a.rec) FM(true, false); else FM(false, false); }

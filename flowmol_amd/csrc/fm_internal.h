@@ -72,6 +72,9 @@ struct fm_ctx {
 
     // LDS bricks
     int brick[3] = {1, 1, 1};
+    int* brick_order = nullptr;           // bricks in 4x4x4-tile order (XCD-aware workgroup mapping)
+    int* part = nullptr;                  // per-workgroup maxima (cell, region and list sizes)
+    int npart = 0;
     int maxreg = 0;                       // largest staged region (molecules) since the last binning
     size_t lds_limit = 65536;             // dynamic LDS per workgroup
 
@@ -163,6 +166,7 @@ int launch_nlist(fm_ctx* c);
 int launch_forces(fm_ctx* c, int iter, bool kick = false);
 int launch_pass1_check_disp(fm_ctx* c);
 int launch_move_lfv(fm_ctx* c, bool check_disp);
+int launch_max_reduce(fm_ctx* c, int n, int flag);
 int launch_macros(fm_ctx* c);
 int launch_scatter_state_to_host_order(fm_ctx* c, double* d_out9, double* d_tally);
 int launch_from_host(fm_ctx* c, const double* d_in9);

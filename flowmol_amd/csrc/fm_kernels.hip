@@ -410,36 +410,41 @@ __global__ __launch_bounds__(FM_BLOCK) void k_bin_scatter(const int* cell_of, co
 // molecules fall back to an insertion sort by lane 0
 __device__ inline bool xkey_less(double xa, int ea, double xb, int eb) { return xa < xb || (xa == xb && ea < eb); }
 
+// one wavefront per cell; part[blockIdx.x] = the largest cell of the workgroup
 __global__ __launch_bounds__(FM_BLOCK) void k_bin_sort(const int* start, int* perm, const int* ext, const double* x,
-                                                       int ncell, int maxcell, int* flags) {
+                                                       int ncell, int maxcell, int* flags, int* part) {
+    __shared__ int bmax[FM_BLOCK / 64];
     const int lane = threadIdx.x & 63;
     const int c = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
-    if (c >= ncell) return;  // whole wavefront
-    const int b = start[c], e = start[c + 1], n = e - b;
+    const bool valid = c < ncell;
+    const int b = valid ? start[c] : 0, e = valid ? start[c + 1] : 0, n = e - b;
+    if (lane == 0) bmax[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = bmax[0];
+        for (int w = 1; w < FM_BLOCK / 64; ++w) m = max(m, bmax[w]);
+        part[blockIdx.x] = m;
+    }
+    if (!valid) return;  // whole wavefront
     if (n > maxcell && lane == 0) set_err(flags, FM_EOVERFLOW, c);
-    // read before the atomic: the maximum settles after a few cells, so few atomics are issued
-    if (lane == 0 && n > __atomic_load_n(&flags[FLAG_MAX_CELL], __ATOMIC_RELAXED)) atomicMax(&flags[FLAG_MAX_CELL], n);
     if (n <= 1) return;
     if (n <= 64) {
-        int v = lane < n ? perm[b + lane] : -1;
-        double kx = lane < n ? x[v] : INFINITY;  // padding sorts last
-        int ke = lane < n ? ext[v] : INT_MAX;
-        for (int k = 2; k <= 64; k <<= 1)
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                const double okx = __shfl_xor(kx, j, 64);
-                const int oke = __shfl_xor(ke, j, 64);
-                const int ov = __shfl_xor(v, j, 64);
-                const bool asc = (lane & k) == 0;
-                const bool lower = (lane & j) == 0;
-                const bool o_less = xkey_less(okx, oke, kx, ke), o_more = xkey_less(kx, ke, okx, oke);
-                const bool take = lower ? (asc ? o_less : o_more) : (asc ? o_more : o_less);
-                if (take) {
-                    kx = okx;
-                    ke = oke;
-                    v = ov;
-                }
-            }
-        if (lane < n) perm[b + lane] = v;
+        // rank sort: each lane counts the keys below its own (keys are distinct: ties in x are
+        // broken by the molecule number); the other lanes' keys come through readlane, so the
+        // loop has no LDS round trips
+        const int v = lane < n ? perm[b + lane] : -1;
+        const double kx = lane < n ? x[v] : 0.0;
+        const int ke = lane < n ? ext[v] : 0;
+        const long long kb = __double_as_longlong(kx);
+        const int klo = (int)kb, khi = (int)(kb >> 32);
+        int rank = 0;
+        for (int k = 0; k < n; ++k) {
+            const int olo = __builtin_amdgcn_readlane(klo, k), ohi = __builtin_amdgcn_readlane(khi, k);
+            const int oe = __builtin_amdgcn_readlane(ke, k);
+            const double ox = __longlong_as_double(((long long)ohi << 32) | (long long)(unsigned)olo);
+            rank += xkey_less(ox, oe, kx, ke) ? 1 : 0;
+        }
+        if (lane < n) perm[b + rank] = v;
     } else if (lane == 0) {
         for (int k = b + 1; k < e; ++k) {
             const int v = perm[k];
@@ -690,6 +695,28 @@ int launch_check_disp(fm_ctx* c) {
     return 0;
 }
 
+// flags[flag] = max(flags[flag], part[0..n)): one workgroup, the only writer of that flag
+__global__ __launch_bounds__(FM_BLOCK) void k_max_reduce(const int* part, int n, int* flag) {
+    __shared__ int sm[FM_BLOCK / 64];
+    int m = INT_MIN;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) m = max(m, part[k]);
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < FM_BLOCK / 64; ++w) m = max(m, sm[w]);
+        m = max(m, sm[0]);
+        if (m > *flag) *flag = m;
+    }
+}
+
+int launch_max_reduce(fm_ctx* c, int n, int flag) {
+    if (n > c->npart) return fm_set_error(FM_EINVAL, "partial maxima exceed their buffer");
+    k_max_reduce<<<1, FM_BLOCK, 0, c->stream>>>(c->part, n, c->d_flags + flag);
+    LAUNCH_CHECK();
+    return 0;
+}
+
 // simulation_move_particles_lfv; check_disp: fused displacement update of rebuild criterion 2
 int launch_move_lfv(fm_ctx* c, bool check_disp) {
     if (c->np == 0) return 0;
@@ -751,9 +778,11 @@ int launch_bin(fm_ctx* c) {
                                                                c->np);
     LAUNCH_CHECK();
     int maxcell = c->p.max_cell_np > 0 ? c->p.max_cell_np : (1 << 26);
-    k_bin_sort<<<nblocks((size_t)c->ncell * 64), FM_BLOCK, 0, c->stream>>>(c->cell_start, c->perm, c->ext[c->cur],
-                                                                            s.x, c->ncell, maxcell, c->d_flags);
+    const int nsort = nblocks((size_t)c->ncell * 64);
+    k_bin_sort<<<nsort, FM_BLOCK, 0, c->stream>>>(c->cell_start, c->perm, c->ext[c->cur], s.x, c->ncell, maxcell,
+                                                  c->d_flags, c->part);
     LAUNCH_CHECK();
+    if (int rc = launch_max_reduce(c, nsort, FLAG_MAX_CELL)) return rc;
     PermArgs a;
     a.src = c->st[c->cur];
     a.dst = c->st[nxt];
