@@ -31,7 +31,12 @@ struct KTimer {
     bool on = false;
     std::vector<hipEvent_t> pool;
     size_t used = 0;
-    std::vector<std::pair<int, size_t>> rec;  // (class, index of start event)
+    struct Rec {
+        int cls;
+        size_t idx;  // index of the start event
+        int step;    // step the launch belongs to (fm_run_vv), for dropping launches that did nothing
+    };
+    std::vector<Rec> rec;
     int open_cls = -1;
     size_t open_idx = 0;
     double total_ms[KT_NCLASS] = {0};
@@ -53,9 +58,9 @@ static void kt_collect(fm_ctx* c) {
     hipStreamSynchronize(c->stream);
     for (auto& r : t->rec) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, t->pool[r.second], t->pool[r.second + 1]) == hipSuccess) {
-            t->total_ms[r.first] += ms;
-            t->count[r.first] += 1;
+        if (hipEventElapsedTime(&ms, t->pool[r.idx], t->pool[r.idx + 1]) == hipSuccess) {
+            t->total_ms[r.cls] += ms;
+            t->count[r.cls] += 1;
         }
     }
     t->rec.clear();
@@ -78,8 +83,25 @@ void ktimer_end(fm_ctx* c, int cls) {
     KTimer* t = c->timer;
     if (!t || !t->on || t->open_cls != cls) return;
     hipEventRecord(t->pool[t->open_idx + 1], c->stream);
-    t->rec.push_back({cls, t->open_idx});
+    t->rec.push_back({cls, t->open_idx, c->kt_step});
     t->open_cls = -1;
+}
+
+// forget the pending records of class cls from step `from` on (launches that found a rebuild
+// pending and did nothing)
+static void ktimer_drop(fm_ctx* c, int cls, int from) {
+    KTimer* t = c->timer;
+    if (!t || !t->on) return;
+    size_t k = 0;
+    for (auto& r : t->rec)
+        if (!(r.cls == cls && r.step >= from)) t->rec[k++] = r;
+    t->rec.resize(k);
+}
+
+// collect before a queue of launches whose records may be dropped afterwards
+static void ktimer_make_room(fm_ctx* c, size_t events) {
+    KTimer* t = c->timer;
+    if (t && t->on && t->used + events > 4096) kt_collect(c);
 }
 
 // ---- helpers ----------------------------------------------------------------------------------
@@ -879,6 +901,84 @@ static int rebuild_lists(fm_ctx* c) {
 
 int rebuild_lists_internal(fm_ctx* c) { return rebuild_lists(c); }
 
+// bookkeeping of a completed step of the fused loops
+static int finish_step(fm_ctx* c, int iter, fm_macros* macros, int n_macros, int* nm) {
+    const bool rec = (iter % c->p.tplot) == 0;
+    const bool ev = c->p.teval > 0 && (iter % c->p.teval) == 0;
+    c->pe_valid = rec;
+    c->vir_valid = rec || ev;
+    c->simtime = c->simtime + c->p.delta_t;
+    if (rec && macros && *nm < n_macros) {
+        TRY(fm_evaluate_macroscopic_properties(c, &macros[*nm]));
+        ++*nm;
+    }
+    return 0;
+}
+
+// md_advance_vv without a host synchronisation per step (rebuild criteria 1 and 2, no bonds or
+// tethers).  Steps are queued up to the next step whose rebuild the host knows in advance (forced
+// at mod(iter+1,tplot)==0, or the fixed interval), a record step whose sums are wanted, or 16
+// steps.  With the displacement criterion the queued launches are conditional: the first step
+// whose check fires records its number and the launches after it do nothing; the host then
+// rebuilds, repeats that step's force call and continues from the next step.  The arithmetic of
+// every completed step is that of the step-by-step loop.
+static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, int n_macros, int* nrb, int* nm) {
+    const int end = iter0 + nsteps, W = 16;
+    const bool disp = c->p.rebuild_criteria == FM_REBUILD_DISPLACEMENT;
+    auto host_rebuild = [&](int it) {
+        return ((it + 1) % c->p.tplot) == 0 || (!disp && it % c->p.fixed_rebuild == 0);
+    };
+    TRY(launch_fill_i32(c, c->d_flags + FLAG_REBUILD, 0, 1));
+    TRY(launch_fill_i32(c, c->d_flags + FLAG_HALT, 0, 1));
+    int iter = iter0;
+    while (iter < end) {
+        if (host_rebuild(iter)) {
+            c->kt_step = iter;
+            c->moved_since_bin = true;
+            TRY(launch_pass1(c));
+            c->rdisp_valid = false;
+            TRY(rebuild_lists(c));
+            ++*nrb;
+            TRY(launch_forces(c, iter, true));
+            TRY(finish_step(c, iter, macros, n_macros, nm));
+            ++iter;
+            continue;
+        }
+        ktimer_make_room(c, 4 * W + 16);
+        int wend = iter;
+        while (wend < end && wend - iter < W && !host_rebuild(wend)) {
+            c->kt_step = wend;
+            c->moved_since_bin = true;
+            if (disp) TRY(launch_pass1_check_disp(c, true, wend));
+            else TRY(launch_pass1(c));
+            TRY(launch_forces(c, wend, true, disp));
+            ++wend;
+            if (macros && *nm < n_macros && ((wend - 1) % c->p.tplot) == 0) break;
+        }
+        TRY(sync_flags(c));
+        if (disp && c->h_flags[FLAG_REBUILD]) {
+            const int s = c->h_flags[FLAG_RB_STEP];
+            if (s < iter || s >= wend) return fm_set_error(FM_ESTATE, "rebuild requested at step %d outside %d..%d", s, iter, wend - 1);
+            ktimer_drop(c, KT_FORCES, s);
+            ktimer_drop(c, KT_PASS1, s + 1);
+            TRY(launch_fill_i32(c, c->d_flags + FLAG_REBUILD, 0, 1));
+            TRY(launch_fill_i32(c, c->d_flags + FLAG_HALT, 0, 1));
+            for (int k = iter; k < s; ++k) TRY(finish_step(c, k, macros, n_macros, nm));
+            c->kt_step = s;
+            c->rdisp_valid = false;
+            TRY(rebuild_lists(c));
+            ++*nrb;
+            TRY(launch_forces(c, s, true));
+            TRY(finish_step(c, s, macros, n_macros, nm));
+            iter = s + 1;
+        } else {
+            for (int k = iter; k < wend; ++k) TRY(finish_step(c, k, macros, n_macros, nm));
+            iter = wend;
+        }
+    }
+    return 0;
+}
+
 extern "C" int fm_run_vv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* macros, int32_t n_macros,
                          int32_t* n_rebuilds) {
     CHECK_CTX(c);
@@ -890,6 +990,13 @@ extern "C" int fm_run_vv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* ma
     if (c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && !c->has_tags) return fm_set_error(FM_ESTATE, "tag_move needs tags");
     const bool sync_each_force = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE || c->p.potential_flag == 1;
     int nrb = 0, nm = 0;
+    if (!sync_each_force &&
+        (c->p.rebuild_criteria == FM_REBUILD_DISPLACEMENT || c->p.rebuild_criteria == FM_REBUILD_FIXED)) {
+        TRY(run_vv_queued(c, iter0, nsteps, macros, n_macros, &nrb, &nm));
+        TRY(sync_flags(c));
+        if (n_rebuilds) *n_rebuilds = nrb;
+        return 0;
+    }
     for (int iter = iter0; iter < iter0 + nsteps; ++iter) {
         const int forced = ((iter + 1) % c->p.tplot) == 0;
         int rb = forced;

@@ -74,7 +74,7 @@ struct Region {
     int* sh;       // nreg: shift code of the region cell
     int* ic_off;   // nic+1: prefix over interior (i-) cells
     int* ic_rc;    // nic: region cell of each i-cell
-    double *lx, *ly, *lz;
+    double* pos;   // staged positions, x y z interleaved (one address, three reads)
     int* lslot;    // optional: slot of each staged molecule
     int nic;
     int total;     // staged molecules
@@ -89,9 +89,7 @@ struct BG {  // brick geometry
 // Carve the LDS buffer (8-byte aligned pieces first).
 __device__ inline void carve(Region& r, char* smem, const BG& g, int maxreg, bool with_slot) {
     double* d = (double*)smem;
-    r.lx = d;
-    r.ly = d + maxreg;
-    r.lz = d + 2 * maxreg;
+    r.pos = d;
     int* ip = (int*)(d + 3 * maxreg);
     r.lslot = nullptr;
     if (with_slot) {
@@ -199,9 +197,9 @@ __device__ void stage_region(Region& r, const BG& g, int brick, const int* start
             if (stage_pos) {
                 int sx, sy, sz;
                 decode_shift(r.sh[rc], sx, sy, sz);
-                r.lx[q] = img(s.x[slot], sx, g.L[0]);
-                r.ly[q] = img(s.y[slot], sy, g.L[1]);
-                r.lz[q] = img(s.z[slot], sz, g.L[2]);
+                r.pos[3 * q] = img(s.x[slot], sx, g.L[0]);
+                r.pos[3 * q + 1] = img(s.y[slot], sy, g.L[1]);
+                r.pos[3 * q + 2] = img(s.z[slot], sz, g.L[2]);
             }
             if (r.lslot) r.lslot[q] = slot;
             if (lchain) {
@@ -301,7 +299,7 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
             int rc, li, slot;
             imol(r, a.start, q, rc, li, slot);
             const int rx = rc % R0, ry = (rc / R0) % R1, rz = rc / (R0 * R1);
-            const double xi = r.lx[li], yi = r.ly[li], zi = r.lz[li];
+            const double xi = r.pos[3 * li], yi = r.pos[3 * li + 1], zi = r.pos[3 * li + 2];
             int nb = 0, chain = 0, funcy = 0;
             if (POLY) {
                 chain = lchain[li];
@@ -334,7 +332,7 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                         int lo = jb, hi = je;
                         while (lo < hi) {
                             const int mid = (lo + hi) >> 1;
-                            if (r.lx[mid] < xlo) lo = mid + 1;
+                            if (r.pos[3 * mid] < xlo) lo = mid + 1;
                             else hi = mid;
                         }
                         jb = lo;
@@ -342,13 +340,13 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                         hi = je;
                         while (lo < hi) {
                             const int mid = (lo + hi) >> 1;
-                            if (r.lx[mid] <= xhi) lo = mid + 1;
+                            if (r.pos[3 * mid] <= xhi) lo = mid + 1;
                             else hi = mid;
                         }
                         je = lo;
                     }
                     for (int j = jb; j < je; ++j) {
-                        const double ddx = xi - r.lx[j], ddy = yi - r.ly[j], ddz = zi - r.lz[j];
+                        const double ddx = xi - r.pos[3 * j], ddy = yi - r.pos[3 * j + 1], ddz = zi - r.pos[3 * j + 2];
                         const double r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
                         const bool hit = (r2 < a.rn2) && (j != li);
                         const uint32_t sh = (uint32_t)(cnt & 3) << 4;
@@ -422,6 +420,7 @@ struct FbArgs {
     double hp, spacing, shift;
     const double* mie;     // [7][FM_MAX_TYPES^2] + mass[FM_MAX_TYPES]
     const int* order;      // brick order of one-domain launches (brick_of_block)
+    int* halt;             // flag words of a conditional launch (null: unconditional)
     double* cv;            // faces 1..3 of the control-volume tally, or null
     int cvnbo[3], cvnhb[3];
     double cvbs[3], cvhd[3];
@@ -536,7 +535,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
         int rci, li, i;
         imol(r, a.start, q, rci, li, i);
-        const double xi = r.lx[li], yi = r.ly[li], zi = r.lz[li];
+        const double xi = r.pos[3 * li], yi = r.pos[3 * li + 1], zi = r.pos[3 * li + 2];
         const int n = a.nbn[i];
         double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, vir = 0.0;
         double rf[9];
@@ -563,9 +562,9 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 jq[q] = (int)((word >> (16 * q)) & 0xffffu);
-                rxq[q] = xi - r.lx[jq[q]];
-                ryq[q] = yi - r.ly[jq[q]];
-                rzq[q] = zi - r.lz[jq[q]];
+                rxq[q] = xi - r.pos[3 * (jq[q])];
+                ryq[q] = yi - r.pos[3 * (jq[q]) + 1];
+                rzq[q] = zi - r.pos[3 * (jq[q]) + 2];
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -612,7 +611,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                 if (mop || a.cv) {
                     const int rcj = region_cell_of(r, a.g.nreg, j);
                     if (mop_owner(r, a.g, rci, rcj, exti, a.ext[r.lslot[j]])) {
-                        const double ri[3] = {xi, yi, zi}, rj[3] = {r.lx[j], r.ly[j], r.lz[j]}, rij[3] = {rx, ry, rz};
+                        const double ri[3] = {xi, yi, zi}, rj[3] = {r.pos[3 * j], r.pos[3 * j + 1], r.pos[3 * j + 2]}, rij[3] = {rx, ry, rz};
                         if (mop) mop_entry(a, ri, rj, rij, acc, spxy);
                         // interior pairs carry 2 acc rij, entries against a periodic image acc rij
                         // (each such pair is met once from either end)
@@ -695,9 +694,20 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
     }
 }
 
+// A conditional launch does nothing when a rebuild is pending, and passes the halt on to the
+// launches queued after it (flag words are written by earlier launches only, so every workgroup
+// reads the same values; the FLAG_HALT store by workgroup 0 is read by later launches).
+__device__ inline bool halted(int* f) {
+    if (!f) return false;
+    const bool h = f[FLAG_HALT] != 0 || f[FLAG_REBUILD] != 0;
+    if (h && blockIdx.x == 0 && threadIdx.x == 0) f[FLAG_HALT] = 1;
+    return h;
+}
+
 template <bool MIE, bool RECORD, bool EXTRA, bool VIR>
 __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
+    if (halted(a.halt)) return;
     forces_body<MIE, RECORD, EXTRA, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
@@ -732,7 +742,7 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
         int rci, li, i;
         imol(r, a.start, q, rci, li, i);
-        const double xi = r.lx[li], yi = r.ly[li], zi = r.lz[li];
+        const double xi = r.pos[3 * li], yi = r.pos[3 * li + 1], zi = r.pos[3 * li + 2];
         const int n = a.nbn[i];
         float fx = 0.f, fy = 0.f, fz = 0.f;
         double pe = 0.0, vir = 0.0;
@@ -748,9 +758,9 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 jq[q] = (int)((word >> (16 * q)) & 0xffffu);
-                rxq[q] = xi - r.lx[jq[q]];
-                ryq[q] = yi - r.ly[jq[q]];
-                rzq[q] = zi - r.lz[jq[q]];
+                rxq[q] = xi - r.pos[3 * (jq[q])];
+                ryq[q] = yi - r.pos[3 * (jq[q]) + 1];
+                rzq[q] = zi - r.pos[3 * (jq[q]) + 2];
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -777,7 +787,7 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
                 if (mop || a.cv) {
                     const int rcj = region_cell_of(r, a.g.nreg, j);
                     if (mop_owner(r, a.g, rci, rcj, exti, a.ext[r.lslot[j]])) {
-                        const double ri[3] = {xi, yi, zi}, rj[3] = {r.lx[j], r.ly[j], r.lz[j]},
+                        const double ri[3] = {xi, yi, zi}, rj[3] = {r.pos[3 * j], r.pos[3 * j + 1], r.pos[3 * j + 2]},
                                      rij[3] = {rxd, ryd, rzd};
                         if (mop) mop_entry(a, ri, rj, rij, (double)acc, spxy);
                         const bool img_entry = r.sh[rcj] != 13 || r.sh[rci] != 13;
@@ -823,6 +833,7 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
 template <bool RECORD, bool EXTRA, bool VIR>
 __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_mixed(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
+    if (halted(a.halt)) return;
     forces_body_mixed<RECORD, EXTRA, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
@@ -1019,6 +1030,7 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     a.shift = c->mop_shift;
     a.mie = mie ? c->d_mie : nullptr;
     a.order = c->brick_order;
+    a.halt = nullptr;
     a.cv = (a.rec && c->cvface) ? c->cvface : nullptr;
     for (int d = 0; d < 3; ++d) {
         a.cvnbo[d] = c->cv_nbo[d];
@@ -1040,12 +1052,13 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     return 0;
 }
 
-int launch_forces(fm_ctx* c, int iter, bool kick) {
+int launch_forces(fm_ctx* c, int iter, bool kick, bool cond) {
     if (c->np == 0) return 0;
     FbArgs a;
     size_t lds = 0;
     int kind = 0;
     if (int rc = make_fargs(c, iter, kick, &a, &lds, &kind)) return rc;
+    if (cond) a.halt = c->d_flags;
     const bool mixed = kind & 1, mie = kind & 2, extra = kind & 4;
     ktimer_begin(c, KT_FORCES);
     dim3 grid(brick_count(c)), blk(FB_BLOCK);

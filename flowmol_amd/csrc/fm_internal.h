@@ -35,7 +35,9 @@ enum {
     FLAG_NOUT = 5,        // molecules outside the box at sendmols
     FLAG_EDGE = 6,        // molecules exactly on the lower domain face (clamped into cell 1)
     FLAG_MAX_REGION = 7,  // largest LDS region (brick + one-cell halo) in molecules
-    FLAG_COUNT = 8
+    FLAG_HALT = 8,        // a rebuild is pending: conditional launches of later steps do nothing
+    FLAG_RB_STEP = 9,     // step whose displacement check set FLAG_REBUILD
+    FLAG_COUNT = 10
 };
 
 struct KTimer;
@@ -74,6 +76,7 @@ struct fm_ctx {
     int brick[3] = {1, 1, 1};
     int* brick_order = nullptr;           // bricks in 4x4x4-tile order (XCD-aware workgroup mapping)
     int* part = nullptr;                  // per-workgroup maxima (cell, region and list sizes)
+    int kt_step = 0;                      // step number attached to kernel timing records
     int npart = 0;
     int maxreg = 0;                       // largest staged region (molecules) since the last binning
     size_t lds_limit = 65536;             // dynamic LDS per workgroup
@@ -163,8 +166,9 @@ int launch_check_vmax(fm_ctx* c);
 int launch_sendmols(fm_ctx* c);
 int launch_bin(fm_ctx* c);
 int launch_nlist(fm_ctx* c);
-int launch_forces(fm_ctx* c, int iter, bool kick = false);
-int launch_pass1_check_disp(fm_ctx* c);
+// cond: do nothing when a rebuild is pending (FLAG_REBUILD or FLAG_HALT), and raise FLAG_HALT
+int launch_forces(fm_ctx* c, int iter, bool kick = false, bool cond = false);
+int launch_pass1_check_disp(fm_ctx* c, bool cond = false, int step = 0);
 int launch_move_lfv(fm_ctx* c, bool check_disp);
 int launch_max_reduce(fm_ctx* c, int n, int flag);
 int launch_macros(fm_ctx* c);

@@ -178,8 +178,12 @@ __global__ __launch_bounds__(FM_BLOCK) void k_check_disp(St s, double* rd, int n
 
 // simulation_move_particles_vv(1) followed by the displacement update of simulation_checkrebuild,
 // in one pass over the state (same operations as k_pass1 then k_check_disp)
+// cond: a launch queued ahead of the host's check; it does nothing when an earlier step of the
+// queue asked for a rebuild (FLAG_HALT), and records its step when it asks for one itself.
 __global__ __launch_bounds__(FM_BLOCK) void k_pass1_disp(St s, double* rd, int np, int cap, double dt, double hdt2,
-                                                          double hdt, double thresh, int fresh, int* flags) {
+                                                          double hdt, double thresh, int fresh, int* flags, int cond,
+                                                          int step) {
+    if (cond && flags[FLAG_HALT] != 0) return;  // whole workgroup
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool hit = false;
     if (i < np) {
@@ -207,8 +211,10 @@ __global__ __launch_bounds__(FM_BLOCK) void k_pass1_disp(St s, double* rd, int n
     }
     unsigned long long m = __ballot(hit);
     if (m != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1 &&
-        __atomic_load_n(&flags[FLAG_REBUILD], __ATOMIC_RELAXED) == 0)
+        __atomic_load_n(&flags[FLAG_REBUILD], __ATOMIC_RELAXED) == 0) {
         atomicOr(&flags[FLAG_REBUILD], 1);
+        if (cond) atomicExch(&flags[FLAG_RB_STEP], step);
+    }
 }
 
 // ---- rebuild criterion 0 (Rapaport vmax), src/simulation_checkrebuild.f90:84-99 --------------
@@ -869,13 +875,13 @@ int launch_slot_to_ext_i32(fm_ctx* c, int* dst, const int* src, int nf) {
     return 0;
 }
 
-int launch_pass1_check_disp(fm_ctx* c) {
+int launch_pass1_check_disp(fm_ctx* c, bool cond, int step) {
     if (c->np == 0) return 0;
     const double dt = c->p.delta_t;
     ktimer_begin(c, KT_PASS1);
     k_pass1_disp<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(state_of(c->x(), c->cap), c->rdisp, c->np, c->cap, dt,
                                                               0.5 * (dt * dt), 0.5 * dt, 0.5 * c->dmin,
-                                                              c->rdisp_valid ? 0 : 1, c->d_flags);
+                                                              c->rdisp_valid ? 0 : 1, c->d_flags, cond ? 1 : 0, step);
     ktimer_end(c, KT_PASS1);
     LAUNCH_CHECK();
     c->rdisp_valid = true;
