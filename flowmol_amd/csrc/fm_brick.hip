@@ -345,6 +345,33 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                         }
                         je = lo;
                     }
+                    if (!POLY) {
+                        // 32 candidates at a time: a hit mask first (distance tests only), then the
+                        // hits in candidate order into the list words
+                        for (int j0 = jb; j0 < je; j0 += 32) {
+                            const int nn = min(32, je - j0);
+                            uint32_t m = 0;
+                            for (int u = 0; u < nn; ++u) {
+                                const int j = j0 + u;
+                                const double ddx = xi - r.pos[3 * j], ddy = yi - r.pos[3 * j + 1],
+                                             ddz = zi - r.pos[3 * j + 2];
+                                const double r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                                m |= (r2 < a.rn2) ? (1u << u) : 0u;
+                            }
+                            if (li >= j0 && li < j0 + nn) m &= ~(1u << (li - j0));
+                            while (m) {
+                                const int j = j0 + __builtin_ctz(m);
+                                m &= m - 1;
+                                word |= (uint64_t)(uint32_t)j << ((cnt & 3) << 4);
+                                if ((cnt & 3) == 3) {
+                                    if (cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word;
+                                    word = 0;
+                                }
+                                ++cnt;
+                            }
+                        }
+                        continue;
+                    }
                     for (int j = jb; j < je; ++j) {
                         const double ddx = xi - r.pos[3 * j], ddy = yi - r.pos[3 * j + 1], ddz = zi - r.pos[3 * j + 2];
                         const double r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
