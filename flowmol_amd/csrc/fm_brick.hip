@@ -359,14 +359,13 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                                 m |= (r2 < a.rn2) ? (1u << u) : 0u;
                             }
                             if (li >= j0 && li < j0 + nn) m &= ~(1u << (li - j0));
+                            // entries enter the word from the top: after four, the first is in
+                            // bits 0-15
                             while (m) {
                                 const int j = j0 + __builtin_ctz(m);
                                 m &= m - 1;
-                                word |= (uint64_t)(uint32_t)j << ((cnt & 3) << 4);
-                                if ((cnt & 3) == 3) {
-                                    if (cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word;
-                                    word = 0;
-                                }
+                                word = (word >> 16) | ((uint64_t)(uint32_t)j << 48);
+                                if ((cnt & 3) == 3 && cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word;
                                 ++cnt;
                             }
                         }
@@ -391,7 +390,11 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                         }
                     }
                 }
-            if ((cnt & 3) != 0 && cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word;
+            if ((cnt & 3) != 0 && cnt < a.maxnb) {
+                // the last, partial word: the non-polymer path fills words from the top
+                const uint64_t last = POLY ? word : word >> (16 * (4 - (cnt & 3)));
+                a.nbl[(size_t)(cnt >> 2) * cap + slot] = last;
+            }
             a.nbn[slot] = cnt < a.maxnb ? cnt : a.maxnb;
             if (cnt > a.maxnb) set_err(a.flags, FM_EOVERFLOW, a.ext[slot]);
             if (POLY) {
