@@ -71,6 +71,10 @@ class FmParams(C.Structure):
         ("binspercell", C.c_double * 3),
         ("cv_conserve", C.c_int32),
         ("_pad3", C.c_int32),
+        ("sort_flag", C.c_int32),
+        ("sort_freq", C.c_int32),
+        ("sortblocksize", C.c_int32),
+        ("_pad4", C.c_int32),
     ]
 
 

@@ -84,7 +84,8 @@ def make_params(domain, density, rcutoff, delta=0.3, dt=0.005, tplot=10, teval=N
                 rebuild_criteria=FM_REBUILD_DISPLACEMENT, fixed_rebuild=20, precision=FM_PREC_FP64,
                 potential_flag=0, k_c=30.0, R_0=1.5, ensemble=FM_ENSEMBLE_NVE,
                 vflux_outflag=0, nplanes=None, planespacing=None, pressure_outflag=0,
-                max_neighbours=0, binspercell=1.0, integration=None):
+                max_neighbours=0, binspercell=1.0, integration=None, sort_flag=0, sort_freq=1,
+                sortblocksize=2):
     p = FmParams()
     for d in range(3):
         p.domain[d] = float(domain[d])
@@ -121,6 +122,7 @@ def make_params(domain, density, rcutoff, delta=0.3, dt=0.005, tplot=10, teval=N
     for d in range(3):  # BIN2CELLRATIO, used by the control-volume tally (vflux_outflag 4)
         p.binspercell[d] = float(binspercell[d] if hasattr(binspercell, "__len__") else binspercell)
     p.max_neighbours = int(max_neighbours)
+    p.sort_flag, p.sort_freq, p.sortblocksize = int(sort_flag), int(sort_freq), int(sortblocksize)
     return p
 
 

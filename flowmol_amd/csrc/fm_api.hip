@@ -149,7 +149,8 @@ static void free_ctx(fm_ctx* c) {
                     c->cell_count, c->cell_start, c->cell_cursor, c->perm, c->nbl, c->nbn, c->bond, c->bondn,
                     c->pe_lj, c->pe_poly, c->vir, c->rf, c->pxy, c->cvface, c->planes, c->out_flag, c->out_vals,
                     c->out_slot, c->out_count, c->red, c->d_macros, c->d_mie, c->d_vmax, c->d_rmax,
-                    c->d_flags, c->stage, c->cub_tmp, c->brick_order, c->part};
+                    c->d_flags, c->stage, c->cub_tmp, c->brick_order, c->part, c->hcurve, c->sort_key,
+                    c->sort_val, c->sort_key2, c->sort_val2, c->sort_cnt, c->sort_tmp};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->h_flags) hipHostFree(c->h_flags);
@@ -189,6 +190,57 @@ static void choose_brick(const int nc[3], double occ, int out[3]) {
             }
 }
 
+// Block curve of sort_mols: SORT_FLAG 1 numbers the blocks x fastest
+// (src/setup_set_parameters.f90:1141-1153); SORT_FLAG 2 walks them along the 3D Hilbert curve of
+// build_hilbert (src/functions_lib.f90:2555-2636): the curve of order ceiling(log2(max nblocks))
+// in the unit cube, scaled to the block grid, keeping the points that fall inside it.
+static int build_block_curve(int flag, const int nb[3], std::vector<int>& out) {
+    const size_t nblk = (size_t)nb[0] * nb[1] * nb[2];
+    out.assign(nblk, -666);
+    auto at = [&](int i, int j, int k) -> int& { return out[(size_t)(i - 1) + nb[0] * ((size_t)(j - 1) + (size_t)nb[1] * (k - 1))]; };
+    if (flag == 1) {
+        for (int i = 1; i <= nb[0]; ++i)
+            for (int j = 1; j <= nb[1]; ++j)
+                for (int k = 1; k <= nb[2]; ++k) at(i, j, k) = i + nb[0] * (j - 1) + nb[0] * nb[1] * (k - 1);
+        return 0;
+    }
+    int maxcells = std::max(nb[0], std::max(nb[1], nb[2]));
+    const int order = (int)ceil(log((double)maxcells) / log(2.0));
+    // the curve of order n: eight transformed copies of the curve of order n-1
+    std::vector<double> x(1, 0.0), y(1, 0.0), z(1, 0.0);
+    for (int lev = 1; lev <= order; ++lev) {
+        const size_t m = x.size();
+        std::vector<double> X(8 * m), Y(8 * m), Z(8 * m);
+        for (size_t k = 0; k < m; ++k) {
+            const double a = x[k], b = y[k], c = z[k];
+            const double px[8] = {0.5 + c, 0.5 + b, -0.5 + b, -0.5 - a, -0.5 - a, -0.5 - b, 0.5 - b, 0.5 + c};
+            const double py[8] = {0.5 + a, 0.5 + c, 0.5 + c, 0.5 + b, -0.5 + b, -0.5 - c, -0.5 - c, -0.5 - a};
+            const double pz[8] = {0.5 + b, -0.5 + a, -0.5 + a, 0.5 - c, 0.5 - c, -0.5 + a, -0.5 + a, 0.5 - b};
+            for (int q = 0; q < 8; ++q) {
+                X[q * m + k] = 0.5 * px[q];
+                Y[q * m + k] = 0.5 * py[q];
+                Z[q * m + k] = 0.5 * pz[q];
+            }
+        }
+        x.swap(X);
+        y.swap(Y);
+        z.swap(Z);
+    }
+    if (order - log((double)maxcells) / log(2.0) > 0.0001) maxcells = (int)ceil(pow(8.0, (double)order / 3.0));
+    double mn = x[0];
+    for (size_t k = 0; k < x.size(); ++k) mn = std::min(mn, std::min(x[k], std::min(y[k], z[k])));
+    const double shift = 0.00001 - mn;
+    int m = 0;
+    for (size_t k = 0; k < x.size(); ++k) {
+        const int i = (int)ceil((x[k] + shift) * maxcells), j = (int)ceil((y[k] + shift) * maxcells),
+                  kk = (int)ceil((z[k] + shift) * maxcells);
+        if (i <= nb[0] && j <= nb[1] && kk <= nb[2]) at(i, j, kk) = ++m;
+    }
+    for (int v : out)
+        if (v == -666) return fm_set_error(FM_EINVAL, "Error in generating Hilbert curve - change sort flag");
+    return 0;
+}
+
 // ---- lifecycle --------------------------------------------------------------------------------
 extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx** out) {
     if (!p || !out) return fm_set_error(FM_EINVAL, "null argument");
@@ -202,6 +254,13 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
     if (p->rebuild_criteria == FM_REBUILD_FIXED && p->fixed_rebuild < 1) return fm_set_error(FM_EINVAL, "fixed_rebuild must be >= 1");
     if (p->ensemble != FM_ENSEMBLE_NVE && p->ensemble != FM_ENSEMBLE_TAG_MOVE)
         return fm_set_error(FM_EINVAL, "ensemble must be NVE (0) or tag_move (6)");
+    if (p->sort_flag < 0 || p->sort_flag > 2) return fm_set_error(FM_EINVAL, "Incorrect value of sort_flag");
+    if (p->sort_flag != 0 && (p->sort_freq < 1 || p->sortblocksize < 1))
+        return fm_set_error(FM_EINVAL, "sort_freq and sortblocksize must be >= 1");
+    if (p->sort_flag != 0 && p->potential_flag == 1)
+        return fm_set_error(FM_EINVAL, "Sort should be turned off - Not developed for polymers");
+    if (p->sort_flag != 0 && p->mie_potential == 1)
+        return fm_set_error(FM_EINVAL, "Sort should be turned off - Not developed for Mie potential");
     if (p->integration_algorithm != FM_INTEGRATION_VV && p->integration_algorithm != FM_INTEGRATION_LFV)
         return fm_set_error(FM_EINVAL, "integration_algorithm must be leap-frog (0) or velocity Verlet (1)");
     if (p->precision != FM_PREC_FP64 && p->precision != FM_PREC_MIXED) return fm_set_error(FM_EINVAL, "bad precision");
@@ -383,6 +442,30 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
             return rc ? rc : fm_set_error(FM_EHIP, "brick order upload failed");
         }
     }
+    if (p->sort_flag != 0) {
+        for (int d = 0; d < 3; ++d) {
+            c->sort_bsl[d] = p->sortblocksize * c->cs[d];
+            c->sort_nb[d] = (int)ceil(p->domain[d] / c->sort_bsl[d]);
+        }
+        std::vector<int> hc;
+        if ((rc = build_block_curve(p->sort_flag, c->sort_nb, hc))) {
+            free_ctx(c);
+            return rc;
+        }
+        int end_bit = 1;
+        while ((1 << end_bit) <= (int)hc.size()) ++end_bit;
+        c->sort_tmp_bytes = cub_sort_pairs_bytes((int)cap, end_bit);
+        if ((rc = dalloc(&c->hcurve, hc.size())) || (rc = dalloc(&c->sort_key, cap)) ||
+            (rc = dalloc(&c->sort_val, cap)) || (rc = dalloc(&c->sort_key2, cap)) || (rc = dalloc(&c->sort_val2, cap)) ||
+            (rc = dalloc(&c->sort_cnt, hc.size() + 1)) || (rc = dalloc((char**)&c->sort_tmp, c->sort_tmp_bytes))) {
+            free_ctx(c);
+            return rc;
+        }
+        if (hipMemcpy(c->hcurve, hc.data(), hc.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            free_ctx(c);
+            return fm_set_error(FM_EHIP, "block curve upload failed");
+        }
+    }
     {
         std::vector<double> mie(7 * T2 + FM_MAX_TYPES, 0.0);
         for (int t = 0; t < T2; ++t) {
@@ -460,6 +543,7 @@ extern "C" int fm_memory_footprint(const fm_params* p, int32_t np, int64_t* byte
     if (p->ensemble == FM_ENSEMBLE_TAG_MOVE) b += 2 * cap * 4 + 2 * 6 * cap * 8;
     if (p->pressure_outflag == 1) b += 9 * cap * 8;
     if (p->vflux_outflag >= 1 && p->vflux_outflag <= 3) b += 4 * (size_t)std::max(p->nplanes, 0) * 8;
+    if (p->sort_flag != 0) b += 4 * cap * 4 + 2 * (ncell + 1) * 4 + cap * 8;  // keys, values, curve, counts, sort temporary
     if (p->vflux_outflag == 4) {
         size_t nbo = 1;
         for (int d = 0; d < 3; ++d) {
@@ -726,9 +810,18 @@ extern "C" int fm_sendmols(fm_ctx* c) {
     return 0;
 }
 
+// sort_mols, src/linklist.f90:2744-2865.  The state stays in slot (cell) order; only the
+// molecule numbers (ext / slot_of) change, to those the reference's reordering gives.
 extern "C" int fm_sort_mols(fm_ctx* c) {
     CHECK_CTX(c);
-    return 0;  // SORT_FLAG 0: molecule numbering unchanged (the slot order is internal)
+    if (c->sort_nrebuilds != c->p.sort_freq) {
+        c->sort_nrebuilds = c->sort_nrebuilds + 1;
+        return 0;
+    }
+    c->sort_nrebuilds = 1;
+    if (c->p.sort_flag == 0) return 0;
+    int skipped = 0;
+    return launch_sort_mols(c, &skipped);
 }
 
 // largest staged region after binning: sizes the LDS of the brick kernels
@@ -895,6 +988,7 @@ extern "C" int fm_reset_cv_stress(fm_ctx* c) {
 static int rebuild_lists(fm_ctx* c) {
     c->cells_valid = c->nlist_valid = false;
     TRY(launch_sendmols(c));
+    TRY(fm_sort_mols(c));
     TRY(fm_assign_to_cell(c));
     return fm_assign_to_neighbourlist(c);
 }

@@ -48,7 +48,6 @@ struct fm_batch {
     int64_t rebuilds = 0;
 };
 
-int ctx_sync_flags(fm_ctx* c);
 
 static void free_graphs(fm_batch* b) {
     if (b->g1) hipGraphExecDestroy(b->g1);

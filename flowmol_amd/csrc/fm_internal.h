@@ -37,7 +37,8 @@ enum {
     FLAG_MAX_REGION = 7,  // largest LDS region (brick + one-cell halo) in molecules
     FLAG_HALT = 8,        // a rebuild is pending: conditional launches of later steps do nothing
     FLAG_RB_STEP = 9,     // step whose displacement check set FLAG_REBUILD
-    FLAG_COUNT = 10
+    FLAG_SORT_MAX = 10,   // largest sort block (sort_mols)
+    FLAG_COUNT = 11
 };
 
 struct KTimer;
@@ -77,6 +78,17 @@ struct fm_ctx {
     int* brick_order = nullptr;           // bricks in 4x4x4-tile order (XCD-aware workgroup mapping)
     int* part = nullptr;                  // per-workgroup maxima (cell, region and list sizes)
     int kt_step = 0;                      // step number attached to kernel timing records
+
+    // sort_mols: block curve, the routine's saved counter and safety factors
+    int* hcurve = nullptr;                // nblocks^3, 1-based curve positions, x fastest
+    int sort_nb[3] = {0, 0, 0};
+    double sort_bsl[3] = {0.0, 0.0, 0.0};
+    int sort_nrebuilds = 1;
+    double sort_sf1 = 1.5;
+    int sort_sf2 = 5;
+    int *sort_key = nullptr, *sort_val = nullptr, *sort_key2 = nullptr, *sort_val2 = nullptr, *sort_cnt = nullptr;
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
     int npart = 0;
     int maxreg = 0;                       // largest staged region (molecules) since the last binning
     size_t lds_limit = 65536;             // dynamic LDS per workgroup
@@ -205,3 +217,7 @@ int launch_decode_nlist(fm_ctx* c, uint32_t* out);
 int cub_exclusive_scan(fm_ctx* c, const int* in, int* out, int n);
 int cub_select_flagged_index(fm_ctx* c, const int* flags, int* out, int* count, int n);
 size_t cub_temp_bytes(int n_items);
+size_t cub_sort_pairs_bytes(int n_items, int end_bit);
+int cub_sort_pairs(fm_ctx* c, const int* kin, int* kout, const int* vin, int* vout, int n, int end_bit);
+int launch_sort_mols(fm_ctx* c, int* overflow_out);
+int ctx_sync_flags(fm_ctx* c);  // read the flag words, report a device-side error

@@ -723,6 +723,103 @@ int launch_max_reduce(fm_ctx* c, int n, int flag) {
     return 0;
 }
 
+// ---- sort_mols, src/linklist.f90:2744-2865 ------------------------------------------------------
+// key[e] = curve position of molecule e's block, val[e] = e; cnt[key] counts the block
+__global__ __launch_bounds__(FM_BLOCK) void k_sort_keys(St s, const int* slot_of, int np, const int* hcurve,
+                                                         int nb0, int nb1, int nb2, double b0, double b1, double b2,
+                                                         double h0, double h1, double h2, int* key, int* val,
+                                                         int* cnt, int* flags) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= np) return;
+    const int slot = slot_of[e];
+    const int i = (int)ceil((s.x[slot] + h0) / b0), j = (int)ceil((s.y[slot] + h1) / b1),
+              k = (int)ceil((s.z[slot] + h2) / b2);
+    if (i < 1 || i > nb0 || j < 1 || j > nb1 || k < 1 || k > nb2) {
+        set_err(flags, FM_EESCAPE, e);
+        key[e] = 0;
+        val[e] = e;
+        return;
+    }
+    const int c = hcurve[(i - 1) + nb0 * ((j - 1) + nb1 * (k - 1))];
+    key[e] = c;
+    val[e] = e;
+    atomicAdd(&cnt[c], 1);
+}
+
+// order[k] = old number of the molecule that becomes number k -> relabel the slots
+__global__ __launch_bounds__(FM_BLOCK) void k_sort_newof(const int* order, int np, int* newof) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < np) newof[order[k]] = k;
+}
+
+__global__ __launch_bounds__(FM_BLOCK) void k_sort_relabel(St s, int* ext, int* slot_of, const int* newof, int np,
+                                                            const int* tag, double* teth, int cap, double w0,
+                                                            double w1, double w2) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= np) return;
+    const int e = newof[ext[slot]];
+    ext[slot] = e;
+    slot_of[e] = slot;
+    if (tag) {  // read_tag on every renumbered molecule
+        const int tg = tag[slot];
+        const double sg = tg == FM_TAG_TETH_SLIDE ? copysign(1.0, s.y[slot]) : tg == FM_TAG_FIXED_SLIDE ? 1.0 : 0.0;
+        teth[slot + 3 * (size_t)cap] = sg != 0.0 ? w0 * sg : 0.0;
+        teth[slot + 4 * (size_t)cap] = sg != 0.0 ? w1 * sg : 0.0;
+        teth[slot + 5 * (size_t)cap] = sg != 0.0 ? w2 * sg : 0.0;
+    }
+}
+
+// flags[flag] = max(flags[flag], v[0..n)) for an arbitrary array
+__global__ __launch_bounds__(FM_BLOCK) void k_max_of(const int* v, int n, int* flag) {
+    __shared__ int sm[FM_BLOCK / 64];
+    int m = INT_MIN;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) m = max(m, v[k]);
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < FM_BLOCK / 64; ++w) m = max(m, sm[w]);
+        m = max(m, sm[0]);
+        *flag = m;
+    }
+}
+
+// One sort_mols renumbering (the caller handles the call counter).  *overflow_out = 1 when a
+// block exceeds the estimate and the round is skipped, as the reference does.
+int launch_sort_mols(fm_ctx* c, int* overflow_out) {
+    *overflow_out = 0;
+    if (c->np == 0) return 0;
+    const int blocks = c->sort_nb[0] * c->sort_nb[1] * c->sort_nb[2];
+    int ave = c->np / blocks;
+    ave = (int)ceil(ave * c->sort_sf1 + c->sort_sf2);
+    if (int rc = launch_fill_i32(c, c->sort_cnt, 0, (size_t)blocks + 1)) return rc;
+    St s = state_of(c->x(), c->cap);
+    k_sort_keys<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(
+        s, c->slot_of, c->np, c->hcurve, c->sort_nb[0], c->sort_nb[1], c->sort_nb[2], c->sort_bsl[0], c->sort_bsl[1],
+        c->sort_bsl[2], c->half[0], c->half[1], c->half[2], c->sort_key, c->sort_val, c->sort_cnt, c->d_flags);
+    LAUNCH_CHECK();
+    k_max_of<<<1, FM_BLOCK, 0, c->stream>>>(c->sort_cnt, blocks + 1, c->d_flags + FLAG_SORT_MAX);
+    LAUNCH_CHECK();
+    if (int rc = ctx_sync_flags(c)) return rc;
+    if (c->h_flags[FLAG_SORT_MAX] > ave) {  // "Miss this round of sorting"
+        c->sort_sf1 = c->sort_sf1 + 0.1;
+        c->sort_sf2 = c->sort_sf2 + 5;
+        *overflow_out = 1;
+        return 0;
+    }
+    int end_bit = 1;
+    while ((1 << end_bit) <= blocks) ++end_bit;
+    if (int rc = cub_sort_pairs(c, c->sort_key, c->sort_key2, c->sort_val, c->sort_val2, c->np, end_bit)) return rc;
+    k_sort_newof<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(c->sort_val2, c->np, c->sort_key);
+    LAUNCH_CHECK();
+    const bool tm = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && c->has_tags;
+    k_sort_relabel<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(
+        s, c->ext[c->cur], c->slot_of, c->sort_key, c->np, tm ? c->tag[c->cur] : nullptr,
+        tm ? c->teth[c->cur] : nullptr, c->cap, c->p.wallslidev[0], c->p.wallslidev[1], c->p.wallslidev[2]);
+    LAUNCH_CHECK();
+    return 0;
+}
+
 // simulation_move_particles_lfv; check_disp: fused displacement update of rebuild criterion 2
 int launch_move_lfv(fm_ctx* c, bool check_disp) {
     if (c->np == 0) return 0;

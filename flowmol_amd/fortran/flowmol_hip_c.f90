@@ -41,6 +41,7 @@ module flowmol_hip_c
         integer(c_int32_t) :: brick(3), pad2
         real(c_double)     :: binspercell(3)
         integer(c_int32_t) :: cv_conserve, pad3
+        integer(c_int32_t) :: sort_flag, sort_freq, sortblocksize, pad4
     end type fm_params
 
     type, bind(C) :: fm_macros

@@ -32,7 +32,8 @@ contains
                                                rebuild_criteria, fixed_rebuild, potential_flag, &
                                                Mie_potential, vflux_outflag, pressure_outflag, &
                                                integration_algorithm, ensemble, tag_move, &
-                                               teth_k2, teth_k4, teth_k6, binspercell, cv_conserve
+                                               teth_k2, teth_k4, teth_k6, binspercell, cv_conserve, &
+                                               sort_flag, sort_freq, sortblocksize
         use arrays_MD, only : r, v, a, tag, rtether
         type(fm_params) :: p
         p%domain = globaldomain
@@ -58,6 +59,9 @@ contains
         p%wallslidev = wallslidev
         p%binspercell = binspercell
         p%cv_conserve = cv_conserve
+        p%sort_flag = sort_flag
+        p%sort_freq = sort_freq
+        p%sortblocksize = sortblocksize
         call fm_check(fm_init(p, int(np, c_int32_t), int(np, c_int32_t), fm_ctx))
         call fm_check(fm_upload_state(fm_ctx, int(np, c_int32_t), r, v, a))
         if (ensemble == tag_move) call fm_check(fm_upload_tags(fm_ctx, tag, rtether))

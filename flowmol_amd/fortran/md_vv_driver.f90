@@ -73,6 +73,11 @@ program md_vv_driver
     p%max_neighbours = 0
     p%max_cell_np = 0
     p%brick = 0
+    p%binspercell = 1.d0
+    p%cv_conserve = 0
+    p%sort_flag = 0
+    p%sort_freq = 1
+    p%sortblocksize = 1
 
     call check(fm_init(p, int(np, c_int32_t), int(np, c_int32_t), ctx))
     call check(fm_upload_state(ctx, int(np, c_int32_t), r, v, acc))

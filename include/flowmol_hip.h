@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FM_ABI_VERSION 2
+#define FM_ABI_VERSION 3
 
 #define FM_MAX_TYPES 4      /* moltype values 0..FM_MAX_TYPES-1 (Mie lookup tables) */
 #define FM_MAX_FUNCY 4      /* max bonds per monomer, src/modules.f90:468 */
@@ -121,6 +121,13 @@ typedef struct fm_params {
     double binspercell[3];
     int32_t cv_conserve;         /* CV_conserve: 0 tally on record steps only; 1 (every step) is rejected */
     int32_t _pad3;
+
+    /* SORT_FLAG, SORT_FREQ, SORTBLOCKSIZE of sort_mols (src/linklist.f90:2744-2865): renumber the
+       molecules block by block along a curve through blocks of sortblocksize^3 cells */
+    int32_t sort_flag;           /* 0 none; 1 blocks in x-fastest order; 2 blocks along a 3D Hilbert curve */
+    int32_t sort_freq;           /* sort at every sort_freq-th call (rebuild) */
+    int32_t sortblocksize;
+    int32_t _pad4;
 } fm_params;
 
 typedef struct fm_ctx fm_ctx;   /* opaque */

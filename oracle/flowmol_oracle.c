@@ -85,6 +85,13 @@ struct orc_state {
     int nbins[3], nhb[3], nbinso[3];
     double binsize[3];
     double* Pxyface;             /* (nbinso(1),nbinso(2),nbinso(3),3,6) column-major */
+    /* sort_mols */
+    int sort_nblocks[3];
+    double sort_bsl[3];
+    int32_t* hcurve;             /* Hcurve(nblocks(1),nblocks(2),nblocks(3)), 1-based curve positions */
+    int sort_nrebuilds;          /* the routine's saved counter, starts at 1 */
+    double sort_sf1;             /* saved safety factors, 1.5 and 5 */
+    int sort_sf2;
     double simtime;
 };
 
@@ -102,12 +109,100 @@ void orc_destroy(orc_state* s) {
     free(s->cell_head); free(s->cell_np); free(s->mol_next);
     free(s->halo_src); free(s->halo_shift);
     free(s->nb_head); free(s->nb_n); free(s->nodes);
-    free(s->rdisp); free(s->planes); free(s->Pxy_plane); free(s->Pxyface);
+    free(s->rdisp); free(s->planes); free(s->Pxy_plane); free(s->Pxyface); free(s->hcurve);
     free(s);
 }
 
 /* set_parameters_cells (src/setup_set_parameters.f90:1848-1903), potshift (1222), MOP planes
    (2298-2340) */
+/* hilbert3 of build_hilbert (src/functions_lib.f90:2610-2636): the 8^n points of the curve of
+   order n, built by concatenating eight transformed copies of the curve of order n-1 */
+static int hilbert3(int n, double** x, double** y, double** z, size_t* len) {
+    if (n <= 0) {
+        *x = calloc(1, sizeof(double));
+        *y = calloc(1, sizeof(double));
+        *z = calloc(1, sizeof(double));
+        *len = 1;
+        return !*x || !*y || !*z;
+    }
+    double *xo, *yo, *zo;
+    size_t m;
+    if (hilbert3(n - 1, &xo, &yo, &zo, &m)) return 1;
+    *len = 8 * m;
+    *x = malloc(*len * sizeof(double));
+    *y = malloc(*len * sizeof(double));
+    *z = malloc(*len * sizeof(double));
+    if (!*x || !*y || !*z) return 1;
+    for (size_t k = 0; k < m; ++k) {
+        const double X = xo[k], Y = yo[k], Z = zo[k];
+        const double px[8] = {0.5 + Z, 0.5 + Y, -0.5 + Y, -0.5 - X, -0.5 - X, -0.5 - Y, 0.5 - Y, 0.5 + Z};
+        const double py[8] = {0.5 + X, 0.5 + Z, 0.5 + Z, 0.5 + Y, -0.5 + Y, -0.5 - Z, -0.5 - Z, -0.5 - X};
+        const double pz[8] = {0.5 + Y, -0.5 + X, -0.5 + X, 0.5 - Z, 0.5 - Z, -0.5 + X, -0.5 + X, 0.5 - Y};
+        for (int b = 0; b < 8; ++b) {
+            (*x)[b * m + k] = 0.5 * px[b];
+            (*y)[b * m + k] = 0.5 * py[b];
+            (*z)[b * m + k] = 0.5 * pz[b];
+        }
+    }
+    free(xo); free(yo); free(zo);
+    return 0;
+}
+
+/* Hcurve for SORT_FLAG 1 (src/setup_set_parameters.f90:1141-1153) and 2 (build_hilbert,
+   src/functions_lib.f90:2555-2604) over blocks of sortblocksize cells */
+static int orc_setup_sort(orc_state* s) {
+    const fm_params* p = &s->p;
+    if (p->sort_flag != 1 && p->sort_flag != 2) return fail("Incorrect value of sort_flag");
+    if (p->sortblocksize < 1 || p->sort_freq < 1) return fail("sortblocksize and sort_freq must be >= 1");
+    for (int d = 0; d < 3; ++d) {
+        s->sort_bsl[d] = p->sortblocksize * s->cellside[d];
+        s->sort_nblocks[d] = f_ceiling(p->domain[d] / s->sort_bsl[d]);
+    }
+    const int *nb = s->sort_nblocks;
+    const size_t nblk = (size_t)nb[0] * nb[1] * nb[2];
+    s->hcurve = malloc(nblk * sizeof(int32_t));
+    if (!s->hcurve) return fail("out of memory");
+#define HC(i, j, k) s->hcurve[((size_t)(k) - 1) * nb[0] * nb[1] + ((size_t)(j) - 1) * nb[0] + ((i) - 1)]
+    if (p->sort_flag == 1) {
+        for (int i = 1; i <= nb[0]; ++i)
+            for (int j = 1; j <= nb[1]; ++j)
+                for (int k = 1; k <= nb[2]; ++k) HC(i, j, k) = i + nb[0] * (j - 1) + nb[0] * nb[1] * (k - 1);
+        return 0;
+    }
+    int maxcells = nb[0] > nb[1] ? nb[0] : nb[1];
+    if (nb[2] > maxcells) maxcells = nb[2];
+    const int hindex = f_ceiling(log((double)maxcells) / log(2.0));
+    double *x, *y, *z;
+    size_t n;
+    if (hilbert3(hindex, &x, &y, &z, &n)) return fail("out of memory");
+    if (hindex - log((double)maxcells) / log(2.0) > 0.0001) maxcells = f_ceiling(pow(8.0, (double)hindex / 3.0));
+    double mn = x[0];
+    for (size_t k = 0; k < n; ++k) {
+        if (x[k] < mn) mn = x[k];
+        if (y[k] < mn) mn = y[k];
+        if (z[k] < mn) mn = z[k];
+    }
+    const double shift = 0.00001 - mn;
+    for (size_t k = 0; k < nblk; ++k) s->hcurve[k] = -666;
+    int m = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const int ic = f_ceiling((x[k] + shift) * maxcells), jc = f_ceiling((y[k] + shift) * maxcells),
+                  kc = f_ceiling((z[k] + shift) * maxcells);
+        if (ic <= nb[0] && jc <= nb[1] && kc <= nb[2]) HC(ic, jc, kc) = ++m;
+    }
+    free(x); free(y); free(z);
+    for (size_t k = 0; k < nblk; ++k)
+        if (s->hcurve[k] == -666) return fail("Error in generating Hilbert curve - change sort flag");
+    return 0;
+#undef HC
+}
+
+int orc_get_hcurve(const orc_state* s, int32_t* out) {
+    if (!s->hcurve) return fail("sort_flag is 0");
+    memcpy(out, s->hcurve, (size_t)s->sort_nblocks[0] * s->sort_nblocks[1] * s->sort_nblocks[2] * sizeof(int32_t));
+    return 0;
+}
+
 orc_state* orc_create(const fm_params* p, int32_t np, int32_t capacity) {
     orc_state* s = (orc_state*)calloc(1, sizeof(orc_state));
     if (!s) { fail("out of memory"); return NULL; }
@@ -191,6 +286,15 @@ orc_state* orc_create(const fm_params* p, int32_t np, int32_t capacity) {
         double shift = 0.5 * (p->domain[pn] - p->planespacing * (s->nplanes - 1));
         for (int n = 1; n <= s->nplanes; ++n)
             s->planes[n - 1] = p->planespacing * (n - 1) + shift - s->halfdomain[pn];
+    }
+    s->sort_nrebuilds = 1;
+    s->sort_sf1 = 1.5;
+    s->sort_sf2 = 5;
+    if (p->sort_flag != 0) {
+        if (orc_setup_sort(s)) {
+            orc_destroy(s);
+            return NULL;
+        }
     }
     if (p->vflux_outflag == 4) {
         /* src/setup_set_parameters.f90:1990-2011 with one process per direction: gnbins is the
@@ -975,6 +1079,74 @@ int orc_move_particles_lfv(orc_state* s) {
     return 0;
 }
 
+/* sort_mols (src/linklist.f90:2744-2865): every sort_freq-th call, renumber the molecules block
+   by block in Hcurve order, keeping their relative order inside a block; a block with more
+   molecules than the estimate raises the saved safety factors and skips this round.  r, v, tag
+   and rtether move with their molecules (a is recomputed before it is used again). */
+int orc_sort_mols(orc_state* s) {
+    if (s->sort_nrebuilds != s->p.sort_freq) {
+        s->sort_nrebuilds = s->sort_nrebuilds + 1;
+        return 0;
+    }
+    s->sort_nrebuilds = 1;
+    if (s->p.sort_flag == 0) return 0;
+    if (s->p.potential_flag == 1) return fail("Sort should be turned off - Not developed for polymers");
+    if (s->p.mie_potential == 1) return fail("Sort should be turned off - Not developed for Mie potential");
+    const int* nb = s->sort_nblocks;
+    const int blocks = nb[0] * nb[1] * nb[2];
+    int ave = s->np / blocks;
+    ave = f_ceiling(ave * s->sort_sf1 + s->sort_sf2);
+    int* blk = malloc((size_t)(s->np > 0 ? s->np : 1) * sizeof(int));
+    int* cnt = calloc((size_t)blocks + 2, sizeof(int));
+    if (!blk || !cnt) { free(blk); free(cnt); return fail("out of memory"); }
+    for (int n = 0; n < s->np; ++n) {
+        int b[3];
+        for (int d = 0; d < 3; ++d) {
+            b[d] = f_ceiling((s->r[3 * n + d] + s->halfdomain[d]) / s->sort_bsl[d]);
+            if (b[d] < 1 || b[d] > nb[d]) { free(blk); free(cnt); return fail("sort_mols: molecule %d outside the blocks", n + 1); }
+        }
+        const int i = s->hcurve[((size_t)b[2] - 1) * nb[0] * nb[1] + ((size_t)b[1] - 1) * nb[0] + (b[0] - 1)];
+        blk[n] = i;
+        cnt[i] = cnt[i] + 1;
+        if (cnt[i] > ave) {
+            s->sort_sf1 = s->sort_sf1 + 0.1;
+            s->sort_sf2 = s->sort_sf2 + 5;
+            free(blk); free(cnt);
+            return 0;
+        }
+    }
+    /* start of each block in the new numbering, then the molecules in their old order */
+    int* pos = malloc((size_t)(blocks + 2) * sizeof(int));
+    int* newof = malloc((size_t)(s->np > 0 ? s->np : 1) * sizeof(int));
+    if (!pos || !newof) { free(blk); free(cnt); free(pos); free(newof); return fail("out of memory"); }
+    pos[1] = 0;
+    for (int i = 1; i <= blocks; ++i) pos[i + 1] = pos[i] + cnt[i];
+    for (int n = 0; n < s->np; ++n) newof[n] = pos[blk[n]]++;
+    double* r = malloc(3 * (size_t)(s->np > 0 ? s->np : 1) * sizeof(double));
+    double* v = malloc(3 * (size_t)(s->np > 0 ? s->np : 1) * sizeof(double));
+    double* rt = malloc(3 * (size_t)(s->np > 0 ? s->np : 1) * sizeof(double));
+    int32_t* tg = malloc((size_t)(s->np > 0 ? s->np : 1) * sizeof(int32_t));
+    if (!r || !v || !rt || !tg) { free(blk); free(cnt); free(pos); free(newof); free(r); free(v); free(rt); free(tg); return fail("out of memory"); }
+    for (int n = 0; n < s->np; ++n) {
+        const int m = newof[n];
+        for (int d = 0; d < 3; ++d) {
+            r[3 * m + d] = s->r[3 * n + d];
+            v[3 * m + d] = s->v[3 * n + d];
+            rt[3 * m + d] = s->rtether[3 * n + d];
+        }
+        tg[m] = s->tag[n];
+    }
+    memcpy(s->r, r, 3 * (size_t)s->np * sizeof(double));
+    memcpy(s->v, v, 3 * (size_t)s->np * sizeof(double));
+    if (s->p.ensemble == FM_ENSEMBLE_TAG_MOVE) {
+        memcpy(s->rtether, rt, 3 * (size_t)s->np * sizeof(double));
+        memcpy(s->tag, tg, (size_t)s->np * sizeof(int32_t));
+        for (int n = 0; n < s->np; ++n) read_tag(s, n);
+    }
+    free(blk); free(cnt); free(pos); free(newof); free(r); free(v); free(rt); free(tg);
+    return 0;
+}
+
 /* simulation_checkrebuild (src/simulation_checkrebuild.f90:17-174), criteria 0, 1, 2 */
 int orc_checkrebuild(orc_state* s, int32_t iter, int32_t* rebuild) {
     int rb = 0;
@@ -1109,6 +1281,7 @@ int orc_get_planes(const orc_state* s, double* planes) {
 static int rebuild_all(orc_state* s) {
     if (orc_linklist_deallocateall(s)) return 1;
     if (orc_sendmols(s)) return 1;
+    if (orc_sort_mols(s)) return 1;
     if (orc_assign_to_cell(s)) return 1;
     if (orc_updateborders(s, 1)) return 1;
     return orc_assign_to_neighbourlist(s);

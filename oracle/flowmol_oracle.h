@@ -54,6 +54,8 @@ int32_t orc_np(const orc_state* s);
 /* reference routines */
 int orc_linklist_deallocateall(orc_state* s);
 int orc_sendmols(orc_state* s);
+int orc_sort_mols(orc_state* s);
+int orc_get_hcurve(const orc_state* s, int32_t* hcurve);
 int orc_assign_to_cell(orc_state* s);
 int orc_updateborders(orc_state* s, int32_t rebuild);
 int orc_assign_to_neighbourlist(orc_state* s);

@@ -39,6 +39,8 @@ def lib():
             "orc_halo_np": (i32, [vp]),
             "orc_linklist_deallocateall": (C.c_int, [vp]),
             "orc_sendmols": (C.c_int, [vp]),
+            "orc_sort_mols": (C.c_int, [vp]),
+            "orc_get_hcurve": (C.c_int, [vp, ip]),
             "orc_assign_to_cell": (C.c_int, [vp]),
             "orc_updateborders": (C.c_int, [vp, i32]),
             "orc_assign_to_neighbourlist": (C.c_int, [vp]),
@@ -160,6 +162,15 @@ class Oracle:
     def sendmols(self):
         _check(lib().orc_sendmols(self._h))
 
+    def sort_mols(self):
+        _check(lib().orc_sort_mols(self._h))
+
+    def hcurve(self, nblocks):
+        """Hcurve indexed [i, j, k] (0-based block indices; 1-based curve positions)."""
+        out = np.zeros(int(np.prod(nblocks)), np.int32)
+        _check(lib().orc_get_hcurve(self._h, iptr(out)))
+        return out.reshape((nblocks[2], nblocks[1], nblocks[0])).transpose(2, 1, 0)
+
     def assign_to_cell(self):
         _check(lib().orc_assign_to_cell(self._h))
 
@@ -233,6 +244,7 @@ class Oracle:
     def rebuild(self):
         self.linklist_deallocateall()
         self.sendmols()
+        self.sort_mols()
         self.assign_to_cell()
         self.updateborders(1)
         self.assign_to_neighbourlist()

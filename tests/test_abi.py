@@ -48,10 +48,10 @@ def _c_layout():
 #include <stdio.h>
 #include "flowmol_hip.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(fm_params), sizeof(fm_macros),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(fm_params), sizeof(fm_macros),
          offsetof(fm_params, k_c), offsetof(fm_params, planespacing),
          offsetof(fm_params, max_cell_np), offsetof(fm_macros, max_potenergymol),
-         offsetof(fm_params, binspercell), offsetof(fm_params, cv_conserve));
+         offsetof(fm_params, binspercell), offsetof(fm_params, cv_conserve), offsetof(fm_params, sortblocksize));
   return 0;
 }
 """
@@ -64,7 +64,7 @@ int main(void) {
 
 
 def test_struct_layout_matches_header():
-    sp, sm, okc, ops, omc, omax, obin, ocv = _c_layout()
+    sp, sm, okc, ops, omc, omax, obin, ocv, osb = _c_layout()
     assert C.sizeof(_abi.FmParams) == sp
     assert C.sizeof(_abi.FmMacros) == sm
     assert _abi.FmParams.k_c.offset == okc
@@ -73,3 +73,4 @@ def test_struct_layout_matches_header():
     assert _abi.FmMacros.max_potenergymol.offset == omax
     assert _abi.FmParams.binspercell.offset == obin
     assert _abi.FmParams.cv_conserve.offset == ocv
+    assert _abi.FmParams.sortblocksize.offset == osb

@@ -355,3 +355,72 @@ def test_mixed_precision_error_vs_fp64():
     assert abs(pe32.sum() - pe64.sum()) < 1e-5 * abs(pe64.sum())
     g64.close()
     g32.close()
+
+
+# 2-cell blocks on 8 lattice units: 4 cells a side, so the 2 blocks a side split the box evenly.
+# On 6 units (3 cells a side) the first block holds 8/27 of the molecules, above the estimate
+# ceil(1.5*108 + 5) = 167, so the first calls skip and raise the safety factors instead.
+@pytest.mark.parametrize("flag,freq,bs,nu,moves", [(1, 1, 1, 6, True), (2, 2, 1, 6, False),
+                                                   (2, 1, 2, 8, True), (2, 1, 2, 6, False)])
+def test_sort_mols_trajectory(flag, freq, bs, nu, moves):
+    """sort_mols inside the rebuild branch renumbers the molecules along the block curve
+    (src/linklist.f90:2744-2865): both sides must agree on the new numbering, so positions
+    compare molecule by molecule after every rebuild; the fused loop repeats the routine one."""
+    p, r, v, a = _lj((nu, nu, nu), seed=29, steps=100, tplot=1000, sort_flag=flag, sort_freq=freq,
+                     sortblocksize=bs)
+    # start from a shuffled numbering so that the first sort has work to do
+    perm = np.random.default_rng(31).permutation(r.shape[0])
+    r, v, a = r[perm].copy(), v[perm].copy(), a[perm].copy()
+    say(f"case: sort_mols flag {flag} freq {freq} block {bs}, {r.shape[0]} LJ, 60 steps")
+    o = Oracle(p, r, v, a)
+    g = MDContext(p, r, v, a)
+    o.setup()
+    o.compute_forces(0)
+    g.setup(0)
+    # a direct call right after setup: the counter starts at 1, so freq 1 sorts here
+    o.sort_mols()
+    g.sort_mols()
+    assert moves == (not np.array_equal(o.state()[0], r))
+    for x, y in zip(o.state()[:2], g.state()[:2]):
+        assert np.array_equal(x, y)
+    o.rebuild()
+    g.rebuild()
+    o.compute_forces(0)
+    g.simulation_compute_forces(0)
+    ro, rg = drop_in_steps(o, g, 1, 60)
+    assert ro == rg and len(ro) > 2
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-10
+    assert np.abs(g.state()[1] - o.state()[1]).max() < 1e-10
+    f = MDContext(p, r, v, a)
+    f.setup(0)
+    f.sort_mols()
+    f.rebuild()
+    f.simulation_compute_forces(0)
+    _, nrb = f.run_vv(1, 60)
+    assert nrb == len(rg)
+    for x, y in zip(f.state(), g.state()):
+        assert np.array_equal(x, y)
+    f.close()
+    g.close()
+
+
+def test_sort_mols_tethered_walls_lfv():
+    """Tags, tether sites and sliding velocities travel with the renumbered molecules."""
+    p, r, v, tag, rt = config.config_couette(nunits=(8, 8, 8), tplot=5, nplanes=16, sort_flag=2,
+                                             sort_freq=1, sortblocksize=1)
+    p = _lfv(p)
+    say("case: sort_mols flag 2 with leap-frog tethered walls, 40 steps")
+    o = Oracle(p, r, v)
+    o.set_tags(tag, rt)
+    g = MDContext(p, r, v)
+    g.set_tags(tag, rt)
+    o.setup()
+    o.compute_forces(0)
+    g.setup(0)
+    ro, rg = lfv_steps(o, g, 0, 40)
+    assert ro == rg and len(ro) > 0
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-9
+    assert np.abs(g.rtether() - o.rtether()).max() < 1e-12
+    Po, Pg = o.mop(), g.mop()
+    assert np.abs(Pg - Po).max() <= 1e-10 * max(1.0, np.abs(Po).max())
+    g.close()

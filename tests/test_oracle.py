@@ -426,6 +426,75 @@ def test_lfv_fixed_and_fixed_slide_tags():
         o.set_tags(np.array([4, 0, 0, 0], np.int32))
 
 
+# ---- sort_mols -----------------------------------------------------------------------------------
+def _sort_setup(L, pos, flag, freq=1, bs=1):
+    p = config.make_params((L, L, L), 0.5, 2.5, sort_flag=flag, sort_freq=freq, sortblocksize=bs, tplot=1000)
+    return p, Oracle(p, pos, np.zeros_like(pos))
+
+
+def test_sort_hilbert_curve_visits_every_block_through_faces():
+    """build_hilbert on a 4^3 block grid (src/functions_lib.f90:2555-2604): a permutation of 1..64
+    in which consecutive positions are face neighbours; on 5x3x4 blocks the curve restricted to
+    the grid still numbers every block once."""
+    pos = np.zeros((1, 3))
+    for L, nb, adjacent in [(12.0, (4, 4, 4), True), (15.0, (5, 5, 5), False)]:
+        p, o = _sort_setup(L, pos, 2)
+        H = o.hcurve(nb)
+        assert sorted(H.ravel()) == list(range(1, int(np.prod(nb)) + 1))
+        if adjacent:
+            where = {int(H[i, j, k]): np.array([i, j, k]) for i in range(nb[0]) for j in range(nb[1]) for k in range(nb[2])}
+            steps = [np.abs(where[m + 1] - where[m]).sum() for m in range(1, int(np.prod(nb)))]
+            assert all(s_ == 1 for s_ in steps)
+    # SORT_FLAG 1: blocks numbered x fastest
+    p, o = _sort_setup(12.0, pos, 1)
+    H = o.hcurve((4, 4, 4))
+    i, j, k = np.meshgrid(np.arange(4), np.arange(4), np.arange(4), indexing="ij")
+    assert np.array_equal(H, 1 + i + 4 * j + 16 * k)
+
+
+@pytest.mark.parametrize("flag", [1, 2])
+def test_sort_mols_renumbers_block_by_block(flag):
+    """The new numbering is a stable sort of the old one by the curve position of each molecule's
+    block; positions and velocities move with their molecules."""
+    rng = np.random.default_rng(7)
+    L = 12.0
+    pos = rng.uniform(-L / 2, L / 2, (300, 3))
+    vel = rng.normal(size=(300, 3))
+    p = config.make_params((L, L, L), 0.5, 2.5, sort_flag=flag, sort_freq=1, sortblocksize=1, tplot=1000)
+    o = Oracle(p, pos, vel)
+    H = o.hcurve((4, 4, 4))
+    b = np.ceil((pos + L / 2) / 3.0).astype(int) - 1
+    key = H[b[:, 0], b[:, 1], b[:, 2]]
+    order = np.argsort(key, kind="stable")
+    o.sort_mols()
+    r, v, _ = o.state()
+    assert np.array_equal(r, pos[order]) and np.array_equal(v, vel[order])
+
+
+def test_sort_mols_frequency_and_skipped_rounds():
+    """Only every sort_freq-th call sorts (the saved counter starts at 1); a block holding more
+    molecules than ceiling(int(np/blocks)*sf1 + sf2) skips the round and raises the saved safety
+    factors by 0.1 and 5, so a crowded block is sorted once they allow it."""
+    rng = np.random.default_rng(3)
+    L = 9.0
+    pos = rng.uniform(-L / 2, L / 2, (60, 3))
+    p, o = _sort_setup(L, pos, 1, freq=3)
+    for call in range(1, 7):
+        before = o.state()[0]
+        o.sort_mols()
+        moved = not np.array_equal(o.state()[0], before)
+        assert moved == (call == 3), call   # the 6th call finds the order already sorted
+    # 40 of 41 molecules in one of 27 blocks: int(41/27) = 1, so the estimate is ceil(1.5 + 5) = 7,
+    # then ceil(1.6 + 10) = 12, 17, 22, 27, 32, 38, and ceil(2.2 + 40) = 43 >= 40 at the eighth call
+    crowd = np.concatenate([rng.uniform(-4.4, -1.6, (40, 3)), rng.uniform(1.6, 4.4, (1, 3))])
+    crowd = crowd[::-1].copy()
+    p, o = _sort_setup(L, crowd, 1, freq=1)
+    for call in range(1, 9):
+        o.sort_mols()
+        moved = not np.array_equal(o.state()[0], crowd)
+        assert moved == (call == 8), call
+
+
 # ---- MOP -----------------------------------------------------------------------------------------
 def test_mop_single_crossing():
     L = 12.0
