@@ -23,6 +23,7 @@ SIGNATURES = {
     "fm_last_error": (C.c_char_p, []),
     "fm_abi_version": (C.c_int, []),
     "fm_get_geometry": (C.c_int, [_vp, _ip, _dp, _dp, _dp, _dp]),
+    "fm_get_capacity": (C.c_int, [_vp, _ip, _ip]),
     "fm_upload_state": (C.c_int, [_vp, _i32, _dp, _dp, _dp]),
     "fm_upload_moltype": (C.c_int, [_vp, _ip]),
     "fm_upload_tags": (C.c_int, [_vp, _ip, _dp]),

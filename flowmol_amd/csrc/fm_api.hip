@@ -5,6 +5,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -149,7 +150,7 @@ static void free_ctx(fm_ctx* c) {
                     c->cell_count, c->cell_start, c->cell_cursor, c->perm, c->nbl, c->nbn, c->bond, c->bondn,
                     c->pe_lj, c->pe_poly, c->vir, c->rf, c->pxy, c->cvface, c->planes, c->out_flag, c->out_vals,
                     c->out_slot, c->out_count, c->red, c->d_macros, c->d_mie, c->d_vmax, c->d_rmax,
-                    c->d_flags, c->stage, c->cub_tmp, c->brick_order, c->part, c->hcurve, c->sort_key,
+                    c->d_flags, c->stage, c->cub_tmp, c->brick_order, c->hcurve, c->sort_key,
                     c->sort_val, c->sort_key2, c->sort_val2, c->sort_cnt, c->sort_tmp};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -168,6 +169,9 @@ static int alloc_nlist(fm_ctx* c, int maxnb) {
     c->maxnb = (maxnb + 3) & ~3;
     return dalloc(&c->nbl, (size_t)(c->maxnb / 4) * c->cap);
 }
+
+// a build met a list of `need` entries: room for a quarter more
+static int grow_nlist(fm_ctx* c, int need) { return alloc_nlist(c, need + need / 4 + 8); }
 
 // Brick shape: the largest number of cells whose expected molecule count fits one pass of the
 // workgroup (15/16 of its threads), preferring the shape with the least staged halo per cell.
@@ -319,7 +323,12 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
     }
     {
         hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) c->lds_limit = (size_t)prop.sharedMemPerBlock;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+            c->lds_limit = (size_t)prop.sharedMemPerBlock;
+            c->lds_cu = std::max((size_t)prop.maxSharedMemoryPerMultiProcessor, c->lds_limit);
+        }
+        const char* tight = getenv("FLOWMOL_TIGHT_REGIONS");
+        c->tight_regions = tight && tight[0] == '1';
     }
 
     // neighbour capacity: 1.5x the expected count at the mean density, plus margin
@@ -431,11 +440,6 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
                         for (int y = ty; y < std::min(ty + T, nb[1]); ++y)
                             for (int x = tx; x < std::min(tx + T, nb[0]); ++x)
                                 order.push_back(x + nb[0] * (y + nb[1] * z));
-        c->npart = (int)std::max<size_t>(order.size(), ((size_t)c->ncell + 3) / 4) + 1;
-        if ((rc = dalloc(&c->part, (size_t)c->npart))) {
-            free_ctx(c);
-            return rc;
-        }
         if ((rc = dalloc(&c->brick_order, order.size())) ||
             hipMemcpy(c->brick_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
             free_ctx(c);
@@ -571,6 +575,13 @@ extern "C" int fm_get_geometry(const fm_ctx* c, int32_t ncells[3], double cellsi
     if (rneighbr2) *rneighbr2 = c->rn2;
     if (rcutoff2) *rcutoff2 = c->rc2;
     if (potshift) *potshift = c->potshift;
+    return 0;
+}
+
+extern "C" int fm_get_capacity(const fm_ctx* c, int32_t* list_entries, int32_t* region_molecules) {
+    CHECK_CTX(c);
+    if (list_entries) *list_entries = c->maxnb;
+    if (region_molecules) *region_molecules = c->maxreg;
     return 0;
 }
 
@@ -824,27 +835,46 @@ extern "C" int fm_sort_mols(fm_ctx* c) {
     return launch_sort_mols(c, &skipped);
 }
 
-// largest staged region after binning: sizes the LDS of the brick kernels
-static int size_regions(fm_ctx* c) {
-    const int maxreg = c->h_flags[FLAG_MAX_REGION];
-    if (maxreg > 65535)
-        return fm_set_error(FM_EOVERFLOW, "brick region holds %d molecules (16-bit list indices); use a smaller brick", maxreg);
-    c->maxreg = std::max(maxreg, 1);
-    size_t need = std::max(brick_lds_bytes(c, c->maxreg, 0), brick_lds_bytes(c, c->maxreg, 2));
+// LDS of a brick kernel staging m molecules: the largest over the kernel kinds
+static size_t region_bytes(const fm_ctx* c, int m) {
+    return std::max(brick_lds_bytes(c, m, 0), brick_lds_bytes(c, m, 2));
+}
+
+// Region capacity from the largest region of a binning (molecules): the LDS those need sets how
+// many workgroups share a CU; the capacity is raised to what that share of the CU's LDS holds
+// (occupancy unchanged), capped by the 16-bit list indices.
+static int size_regions(fm_ctx* c, int measured) {
+    if (measured > 65536)
+        return fm_set_error(FM_EOVERFLOW, "brick region holds %d molecules (16-bit list indices); use a smaller brick", measured);
+    const int m0 = std::max(measured, 1);
+    const size_t need = region_bytes(c, m0);
     if (need > c->lds_limit)
         return fm_set_error(FM_EOVERFLOW, "brick region needs %zu bytes of LDS (limit %zu); use a smaller brick", need,
                             c->lds_limit);
+    if (c->tight_regions) {
+        c->maxreg = m0;
+        return 0;
+    }
+    const size_t wgs = std::max<size_t>(c->lds_cu / need, 1);
+    const size_t budget = std::min(c->lds_limit, (c->lds_cu / wgs) & ~(size_t)1023);
+    int lo = m0, hi = 65536;  // largest m in [m0, 65536] with region_bytes(m) <= budget
+    while (lo < hi) {
+        const int mid = lo + (hi - lo + 1) / 2;
+        if (region_bytes(c, mid) <= budget) lo = mid;
+        else hi = mid - 1;
+    }
+    c->maxreg = lo;
     return 0;
 }
 
 extern "C" int fm_assign_to_cell(fm_ctx* c) {
     CHECK_CTX(c);
-    TRY(launch_fill_i32(c, c->d_flags + FLAG_MAX_CELL, 0, 1));
-    TRY(launch_fill_i32(c, c->d_flags + FLAG_MAX_REGION, 0, 1));
+    static const int reset[] = {FLAG_MAX_REGION, FLAG_NEED, FLAG_HALT};
+    TRY(launch_reset_flags(c, reset, 3));
     TRY(launch_bin(c));
     TRY(launch_region_max(c));
     TRY(sync_flags(c));
-    TRY(size_regions(c));
+    TRY(size_regions(c, c->h_flags[FLAG_MAX_REGION]));
     c->moved_since_bin = false;
     c->cells_valid = true;
     c->nlist_valid = false;
@@ -860,18 +890,19 @@ extern "C" int fm_assign_to_neighbourlist(fm_ctx* c) {
     CHECK_CTX(c);
     if (!c->cells_valid) return fm_set_error(FM_ESTATE, "assign_to_neighbourlist before assign_to_cell");
     if (c->p.potential_flag == 1 && !c->has_mono) return fm_set_error(FM_ESTATE, "potential_flag 1 needs monomer data (fm_upload_monomers)");
-    for (int attempt = 0; attempt < 3; ++attempt) {
-        TRY(launch_fill_i32(c, c->d_flags + FLAG_MAX_NB, 0, 1));
+    for (int attempt = 0;; ++attempt) {
+        static const int reset[] = {FLAG_MAX_NB, FLAG_NEED, FLAG_HALT};
+        TRY(launch_reset_flags(c, reset, 3));
         TRY(launch_nlist(c));
-        FM_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, FLAG_COUNT * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        FM_HIP(hipStreamSynchronize(c->stream));
-        int need = c->h_flags[FLAG_MAX_NB];
-        if (need <= c->maxnb || c->h_flags[FLAG_ERROR] != FM_EOVERFLOW) break;
-        // grow the list and rebuild
-        TRY(launch_fill_i32(c, c->d_flags + FLAG_ERROR, 0, 2));
-        TRY(alloc_nlist(c, ((need + need / 4 + 8) + 3) & ~3));
+        TRY(sync_flags(c));
+        const int need = c->h_flags[FLAG_NEED];
+        if (need == 0) break;
+        if (attempt == 2 || (need & NEED_REGION))
+            return fm_set_error(FM_EOVERFLOW, "neighbour list build needs %d entries, region %d molecules",
+                                c->h_flags[FLAG_MAX_NB], c->h_flags[FLAG_MAX_REGION]);
+        // grow the list and build again
+        TRY(grow_nlist(c, c->h_flags[FLAG_MAX_NB]));
     }
-    TRY(sync_flags(c));
     c->nlist_valid = true;
     return 0;
 }
@@ -995,6 +1026,42 @@ static int rebuild_lists(fm_ctx* c) {
 
 int rebuild_lists_internal(fm_ctx* c) { return rebuild_lists(c); }
 
+// The same rebuild queued without waiting on the device (fused loops): the list build uses the
+// region and list capacities of the last measured build, and a region or list beyond them raises
+// FLAG_NEED and FLAG_HALT, so the conditional launches queued after it do nothing until
+// redo_lists() has grown the capacities at the next synchronisation.
+static int rebuild_lists_queued(fm_ctx* c) {
+    if (c->maxreg <= 0) return rebuild_lists(c);
+    c->cells_valid = c->nlist_valid = false;
+    TRY(launch_sendmols(c));
+    TRY(fm_sort_mols(c));
+    static const int reset[] = {FLAG_REBUILD, FLAG_HALT, FLAG_NEED, FLAG_MAX_REGION, FLAG_MAX_NB};
+    TRY(launch_reset_flags(c, reset, 5));
+    TRY(launch_bin(c));
+    c->moved_since_bin = false;
+    c->cells_valid = true;
+    TRY(launch_nlist(c));
+    c->nlist_valid = true;
+    return 0;
+}
+
+// A synchronisation found FLAG_NEED set by the queued list build: grow the region and list
+// capacities to what it met and build the lists again (the cells are unchanged, and every launch
+// queued after the build did nothing).
+static int redo_lists(fm_ctx* c) {
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        const int need = c->h_flags[FLAG_NEED];
+        if (need == 0) return 0;
+        if (need & NEED_REGION) TRY(size_regions(c, std::max(c->h_flags[FLAG_MAX_REGION], c->maxreg + 1)));
+        if (need & NEED_LIST) TRY(grow_nlist(c, std::max(c->h_flags[FLAG_MAX_NB], c->maxnb + 1)));
+        static const int reset[] = {FLAG_HALT, FLAG_NEED, FLAG_MAX_REGION, FLAG_MAX_NB};
+        TRY(launch_reset_flags(c, reset, 4));
+        TRY(launch_nlist(c));
+        TRY(sync_flags(c));
+    }
+    return c->h_flags[FLAG_NEED] ? fm_set_error(FM_EOVERFLOW, "list build still beyond its capacities") : 0;
+}
+
 // bookkeeping of a completed step of the fused loops
 static int finish_step(fm_ctx* c, int iter, fm_macros* macros, int n_macros, int* nm) {
     const bool rec = (iter % c->p.tplot) == 0;
@@ -1010,61 +1077,77 @@ static int finish_step(fm_ctx* c, int iter, fm_macros* macros, int n_macros, int
 }
 
 // md_advance_vv without a host synchronisation per step (rebuild criteria 1 and 2, no bonds or
-// tethers).  Steps are queued up to the next step whose rebuild the host knows in advance (forced
-// at mod(iter+1,tplot)==0, or the fixed interval), a record step whose sums are wanted, or 16
-// steps.  With the displacement criterion the queued launches are conditional: the first step
-// whose check fires records its number and the launches after it do nothing; the host then
-// rebuilds, repeats that step's force call and continues from the next step.  The arithmetic of
-// every completed step is that of the step-by-step loop.
+// tethers).  Steps are queued in windows that end before the next step whose rebuild the host
+// knows in advance (forced at mod(iter+1,tplot)==0, or the fixed interval), after a record step
+// whose sums are wanted, or after 16 steps; the host synchronises once per window.  With the
+// displacement criterion the queued launches are conditional: the first step whose check fires
+// records its number and the launches after it do nothing; the next window then starts with that
+// step's rebuild and force call.  A window that starts with a rebuild queues it too (the list
+// capacities are checked at the window's synchronisation, and a build beyond them is repeated
+// after growing them).  The arithmetic of every completed step is that of the step-by-step loop.
 static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, int n_macros, int* nrb, int* nm) {
     const int end = iter0 + nsteps, W = 16;
     const bool disp = c->p.rebuild_criteria == FM_REBUILD_DISPLACEMENT;
     auto host_rebuild = [&](int it) {
         return ((it + 1) % c->p.tplot) == 0 || (!disp && it % c->p.fixed_rebuild == 0);
     };
-    TRY(launch_fill_i32(c, c->d_flags + FLAG_REBUILD, 0, 1));
-    TRY(launch_fill_i32(c, c->d_flags + FLAG_HALT, 0, 1));
+    auto sums_wanted = [&](int it) { return macros && *nm < n_macros && (it % c->p.tplot) == 0; };
+    static const int reset[] = {FLAG_REBUILD, FLAG_HALT};
+    TRY(launch_reset_flags(c, reset, 2));
     int iter = iter0;
+    int rb = -1;  // step whose displacement check fired (its pass 1 has run)
     while (iter < end) {
-        if (host_rebuild(iter)) {
+        ktimer_make_room(c, 4 * W + 64);
+        int wend = iter, lists = -1;
+        if (rb == iter || host_rebuild(iter)) {
             c->kt_step = iter;
-            c->moved_since_bin = true;
-            TRY(launch_pass1(c));
+            if (rb != iter) {
+                c->moved_since_bin = true;
+                TRY(launch_pass1(c));
+            }
             c->rdisp_valid = false;
-            TRY(rebuild_lists(c));
+            TRY(rebuild_lists_queued(c));
             ++*nrb;
-            TRY(launch_forces(c, iter, true));
-            TRY(finish_step(c, iter, macros, n_macros, nm));
-            ++iter;
-            continue;
+            TRY(launch_forces(c, iter, true, true));
+            lists = iter;
+            wend = iter + 1;
         }
-        ktimer_make_room(c, 4 * W + 16);
-        int wend = iter;
-        while (wend < end && wend - iter < W && !host_rebuild(wend)) {
+        rb = -1;
+        while (wend < end && wend - iter < W && !host_rebuild(wend) && !(wend > iter && sums_wanted(wend - 1))) {
             c->kt_step = wend;
             c->moved_since_bin = true;
             if (disp) TRY(launch_pass1_check_disp(c, true, wend));
-            else TRY(launch_pass1(c));
-            TRY(launch_forces(c, wend, true, disp));
+            else TRY(launch_pass1(c, lists >= 0));
+            TRY(launch_forces(c, wend, true, disp || lists >= 0));
             ++wend;
-            if (macros && *nm < n_macros && ((wend - 1) % c->p.tplot) == 0) break;
         }
         TRY(sync_flags(c));
+        if (c->h_flags[FLAG_NEED]) {
+            if (lists < 0) return fm_set_error(FM_ESTATE, "capacity request without a queued list build");
+            ktimer_drop(c, KT_NLIST, lists);
+            ktimer_drop(c, KT_FORCES, lists);
+            ktimer_drop(c, KT_PASS1, lists + 1);
+            // the launches after the build did nothing: positions are those of the binning, and
+            // the displacement sums start again after the rebuild
+            c->kt_step = lists;
+            c->moved_since_bin = false;
+            c->rdisp_valid = false;
+            TRY(redo_lists(c));
+            TRY(launch_forces(c, lists, true));
+            TRY(finish_step(c, lists, macros, n_macros, nm));
+            iter = lists + 1;
+            continue;
+        }
         if (disp && c->h_flags[FLAG_REBUILD]) {
             const int s = c->h_flags[FLAG_RB_STEP];
-            if (s < iter || s >= wend) return fm_set_error(FM_ESTATE, "rebuild requested at step %d outside %d..%d", s, iter, wend - 1);
+            if (s < iter || s >= wend || s == lists)
+                return fm_set_error(FM_ESTATE, "rebuild requested at step %d outside %d..%d", s, iter, wend - 1);
             ktimer_drop(c, KT_FORCES, s);
             ktimer_drop(c, KT_PASS1, s + 1);
-            TRY(launch_fill_i32(c, c->d_flags + FLAG_REBUILD, 0, 1));
-            TRY(launch_fill_i32(c, c->d_flags + FLAG_HALT, 0, 1));
+            TRY(launch_reset_flags(c, reset, 2));
             for (int k = iter; k < s; ++k) TRY(finish_step(c, k, macros, n_macros, nm));
-            c->kt_step = s;
-            c->rdisp_valid = false;
-            TRY(rebuild_lists(c));
-            ++*nrb;
-            TRY(launch_forces(c, s, true));
-            TRY(finish_step(c, s, macros, n_macros, nm));
-            iter = s + 1;
+            rb = s;
+            iter = s;
         } else {
             for (int k = iter; k < wend; ++k) TRY(finish_step(c, k, macros, n_macros, nm));
             iter = wend;

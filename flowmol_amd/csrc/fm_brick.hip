@@ -52,6 +52,11 @@ __device__ inline double recip(double x) {
     return __builtin_fma(r, e, r);
 }
 __device__ inline double fmaf64(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// one Newton step: relative error below 3e-15 (v_rcp_f64 alone is good to about 5e-8)
+__device__ inline double recip1(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
 
 __device__ inline double p3(double x) { return x * (x * x); }
 __device__ inline double p4(double x) { double x2 = x * x; return x2 * x2; }
@@ -137,9 +142,11 @@ __device__ inline void wave0_scan(int* off, int n) {
     }
 }
 
-// Region metadata + interior-cell list + staged positions.  Ends with __syncthreads().
-__device__ void stage_region(Region& r, const BG& g, int brick, const int* start, const St& s, bool stage_pos,
-                             const int* mono_src, int cap, int* lchain, int* lsub) {
+// Region metadata + interior-cell list + staged positions.  Ends with __syncthreads().  A region
+// of more than capm molecules is not staged: the workgroup raises NEED_REGION (and the size in
+// FLAG_MAX_REGION) and FLAG_HALT, and the call returns false on every thread.
+__device__ bool stage_region(Region& r, const BG& g, int brick, const int* start, const St& s, bool stage_pos,
+                             const int* mono_src, int cap, int* lchain, int* lsub, int capm, int* flags) {
     const int bx = brick % g.nb[0], by = (brick / g.nb[0]) % g.nb[1], bz = brick / (g.nb[0] * g.nb[1]);
     const int org[3] = {bx * g.B[0], by * g.B[1], bz * g.B[2]};
     for (int t = threadIdx.x; t < g.nreg; t += blockDim.x) {
@@ -184,6 +191,14 @@ __device__ void stage_region(Region& r, const BG& g, int brick, const int* start
     r.total = r.off[g.nreg];
     r.nic = min(g.B[0], g.nc[0] - org[0]) * min(g.B[1], g.nc[1] - org[1]) * min(g.B[2], g.nc[2] - org[2]);
     r.ni = r.ic_off[r.nic];
+    if (r.total > capm) {
+        if (threadIdx.x == 0 && flags) {
+            atomicOr(&flags[FLAG_NEED], NEED_REGION);
+            atomicMax(&flags[FLAG_MAX_REGION], r.total);
+            flags[FLAG_HALT] = 1;
+        }
+        return false;
+    }
     if (stage_pos || r.lslot || lchain) {
         for (int q = threadIdx.x; q < r.total; q += blockDim.x) {
             int lo = 0, hi = g.nreg - 1;  // last rc with off[rc] <= q
@@ -209,6 +224,7 @@ __device__ void stage_region(Region& r, const BG& g, int brick, const int* start
         }
     }
     __syncthreads();
+    return true;
 }
 
 // i-molecule q of the brick -> (region cell, LDS index, slot)
@@ -261,7 +277,6 @@ struct NlbArgs {
     double xwin;     // window half-width: above sqrt(rn2), so no pair inside rn2 is skipped
     double cs[3], hd[3];  // cell side and half domain (cell faces in image coordinates)
     const int* order;     // brick order (brick_of_block)
-    int* part;            // per-workgroup longest list
 };
 
 __device__ inline int get_bondflag(const int* mono, int cap, int i, int scID) {
@@ -286,7 +301,7 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
         lsub = lchain + a.maxreg;
     }
     const int brick = brick_of_block(a.order, blockIdx.x, gridDim.x);
-    stage_region(r, a.g, brick, a.start, a.s, true, a.mono, a.cap, lchain, lsub);
+    if (!stage_region(r, a.g, brick, a.start, a.s, true, a.mono, a.cap, lchain, lsub, a.maxreg, a.flags)) return;
     int wmax = 0;  // longest list of this wavefront
     const size_t cap = a.cap;
     const int R0 = a.g.R[0], R1 = a.g.R[1];
@@ -396,7 +411,6 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                 a.nbl[(size_t)(cnt >> 2) * cap + slot] = last;
             }
             a.nbn[slot] = cnt < a.maxnb ? cnt : a.maxnb;
-            if (cnt > a.maxnb) set_err(a.flags, FM_EOVERFLOW, a.ext[slot]);
             if (POLY) {
                 a.bondn[slot] = nb < funcy ? nb : funcy;
                 if (nb > funcy) set_err(a.flags, FM_EBOND, a.ext[slot]);
@@ -406,15 +420,19 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
         for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
         wmax = max(wmax, m);
     }
-    // longest list of the workgroup into part[blockIdx.x] (reduced by k_max_reduce); the region
-    // tables are no longer read, so their first words hold the per-wave values
+    // a list longer than the capacity: the workgroup's longest into FLAG_MAX_NB, NEED_LIST and
+    // FLAG_HALT (the region tables are no longer read, so their first words hold the per-wave values)
     __syncthreads();
     if ((threadIdx.x & 63) == 0) r.off[threadIdx.x >> 6] = wmax;
     __syncthreads();
     if (threadIdx.x == 0) {
         int m = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = max(m, r.off[w]);
-        a.part[blockIdx.x] = m;
+        if (m > a.maxnb) {
+            atomicMax(&a.flags[FLAG_MAX_NB], m);
+            atomicOr(&a.flags[FLAG_NEED], NEED_LIST);
+            a.flags[FLAG_HALT] = 1;
+        }
     }
 }
 
@@ -545,6 +563,72 @@ __device__ inline bool mop_owner(const Region& r, const BG& g, int rci, int rcj,
     return is_forward(dx, dy, dz);
 }
 
+// LJ list loop of a step without tallies.  Four entries per word, evaluated without branches: an
+// entry beyond rc, or past the end of the list (index 0 pads a last, partial word), gets a zero
+// magnitude.  Two words are in flight: the next word's positions are read from LDS while this
+// word is evaluated, and the word after it from HBM.  The pair law 48 x^4 (x^3 - 1/2) is summed
+// as x^4 (2 x^3 - 1) r_ij and scaled by 24 at the end, with x = 1/r^2 from one Newton step;
+// within 2e-14 relative of the reference's 48 (x^7 - x^4/2) per pair.
+struct P4 {
+    double x[4], y[4], z[4];
+};
+__device__ __forceinline__ void lj_stage(P4& p, const double* pos, uint64_t w) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int j = (int)((w >> (16 * q)) & 0xffffu);
+        p.x[q] = pos[3 * j];
+        p.y[q] = pos[3 * j + 1];
+        p.z[q] = pos[3 * j + 2];
+    }
+}
+__device__ __forceinline__ void lj_word(const P4& p, int left, double xi, double yi, double zi, double rc2,
+                                        double& fx, double& fy, double& fz) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double rx = xi - p.x[q], ry = yi - p.y[q], rz = zi - p.z[q];
+        const double r2 = (rx * rx + ry * ry) + rz * rz;
+        double x = recip1(r2);
+        // an entry that does not count keeps only the low word of x: at most a denormal, whose
+        // square underflows to zero, so its magnitude below is exactly -0
+        x = (r2 < rc2 && q < left) ? x : __hiloint2double(0, __double2loint(x));
+        const double x2 = x * x;
+        const double x4 = x2 * x2;
+        const double acc = x4 * fmaf64(2.0, x2 * x, -1.0);
+        fx = fmaf64(acc, rx, fx);
+        fy = fmaf64(acc, ry, fy);
+        fz = fmaf64(acc, rz, fz);
+    }
+}
+__device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row, int n, size_t cap, double xi,
+                                         double yi, double zi, double rc2, double& fx, double& fy, double& fz) {
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    P4 pa, pb;
+    // At the top of each pass pa holds the positions of word k = kw/4, w1 is word k+1 and w2
+    // word k+2 (in flight).  Word loads past the end re-read the last word (the row always has
+    // at least one word), so no load sits behind a branch and each wait is for the oldest load;
+    // a pass always evaluates both words (the second one masked off past the end) so that the
+    // loop body is one block and the loads stay ahead of their uses.
+    const int lastw = n > 0 ? (n - 1) >> 2 : 0;
+    uint64_t w0 = row[0];
+    lj_stage(pa, pos, n > 0 ? w0 : 0);
+    uint64_t w1 = row[(size_t)min(1, lastw) * cap];
+    uint64_t w2 = row[(size_t)min(2, lastw) * cap];
+    for (int kw = 0; kw < n; kw += 8) {
+        const int k = kw >> 2;
+        lj_stage(pb, pos, kw + 4 < n ? w1 : 0);
+        w1 = row[(size_t)min(k + 3, lastw) * cap];
+        __builtin_amdgcn_sched_barrier(0);  // keep the reads above the evaluation
+        lj_word(pa, n - kw, xi, yi, zi, rc2, sx, sy, sz);
+        lj_stage(pa, pos, kw + 8 < n ? w2 : 0);
+        w2 = row[(size_t)min(k + 4, lastw) * cap];
+        __builtin_amdgcn_sched_barrier(0);
+        lj_word(pb, n - kw - 4, xi, yi, zi, rc2, sx, sy, sz);
+    }
+    fx = 24.0 * sx;
+    fy = 24.0 * sy;
+    fz = 24.0 * sz;
+}
+
 // VIR: virial on a non-record step (teval); record steps always tally it
 template <bool MIE, bool RECORD, bool EXTRA, bool VIR = RECORD>
 __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* smem) {
@@ -555,7 +639,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
     carve(r, smem, a.g, a.maxreg, mop || MIE || cvrec);
     double* spxy = nullptr;
     if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
-    stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr);
+    if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags)) return;
     if (mop) {
         for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
         __syncthreads();
@@ -580,11 +664,12 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         }
         const int exti = (mop || (RECORD && a.cv)) ? a.ext[i] : 0;
         const uint64_t* row = a.nbl + i;
+        if constexpr (!MIE && !RECORD && !VIR) lj_plain(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz);
         // Four 16-bit entries per word.  The next word is loaded while this one is worked on,
         // and the four entries' positions are read from LDS together (the unused entries of a
         // last, partial word are 0, a valid region index).
         uint64_t wnext = n > 0 ? row[0] : 0;
-        for (int kw = 0; kw < n; kw += 4) {
+        for (int kw = 0; (MIE || RECORD || VIR) && kw < n; kw += 4) {
             const uint64_t word = wnext;
             if (kw + 4 < n) wnext = row[(size_t)((kw >> 2) + 1) * cap];
             int jq[4];
@@ -735,7 +820,7 @@ __device__ inline bool halted(int* f) {
 }
 
 template <bool MIE, bool RECORD, bool EXTRA, bool VIR>
-__global__ __launch_bounds__(FB_BLOCK) void k_forces_brick(FbArgs a) {
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_forces_brick(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt)) return;
     forces_body<MIE, RECORD, EXTRA, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
@@ -762,7 +847,7 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
     carve(r, smem, a.g, a.maxreg, mop || cvrec);
     double* spxy = nullptr;
     if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
-    stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr);
+    if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags)) return;
     if (mop) {
         for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
         __syncthreads();
@@ -883,7 +968,7 @@ __global__ __launch_bounds__(FM_BLOCK) void k_decode_brick(St s, BG g, const int
     extern __shared__ __align__(16) char smem[];
     Region r;
     carve(r, smem, g, maxreg, true);
-    stage_region(r, g, blockIdx.x, start, s, false, nullptr, cap, nullptr, nullptr);
+    if (!stage_region(r, g, blockIdx.x, start, s, false, nullptr, cap, nullptr, nullptr, maxreg, nullptr)) return;
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
         int rci, li, i;
         imol(r, start, q, rci, li, i);
@@ -896,27 +981,36 @@ __global__ __launch_bounds__(FM_BLOCK) void k_decode_brick(St s, BG g, const int
     }
 }
 
-// staged region sizes (LDS sizing): one wavefront per brick sums its region cells into part[brick]
-__global__ __launch_bounds__(FM_BLOCK) void k_region_max(BG g, const int* start, int nbrick, int* part) {
+// staged region sizes (LDS sizing): one wavefront per brick sums its region cells; the largest
+// of each workgroup goes into flags[FLAG_MAX_REGION]
+__global__ __launch_bounds__(FM_BLOCK) void k_region_max(BG g, const int* start, int nbrick, int* flags) {
+    __shared__ int wm[FM_BLOCK / 64];
     const int b = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
-    if (b >= nbrick) return;  // whole wavefront
-    const int bx = b % g.nb[0], by = (b / g.nb[0]) % g.nb[1], bz = b / (g.nb[0] * g.nb[1]);
-    const int org[3] = {bx * g.B[0], by * g.B[1], bz * g.B[2]};
     int tot = 0;
-    for (int t = lane; t < g.nreg; t += 64) {
-        const int rr[3] = {t % g.R[0], (t / g.R[0]) % g.R[1], t / (g.R[0] * g.R[1])};
-        int v[3];
-        for (int d = 0; d < 3; ++d) {
-            v[d] = org[d] + rr[d] - 1;
-            if (v[d] < 0) v[d] += g.nc[d];
-            else if (v[d] >= g.nc[d]) v[d] -= g.nc[d];
+    if (b < nbrick) {
+        const int bx = b % g.nb[0], by = (b / g.nb[0]) % g.nb[1], bz = b / (g.nb[0] * g.nb[1]);
+        const int org[3] = {bx * g.B[0], by * g.B[1], bz * g.B[2]};
+        for (int t = lane; t < g.nreg; t += 64) {
+            const int rr[3] = {t % g.R[0], (t / g.R[0]) % g.R[1], t / (g.R[0] * g.R[1])};
+            int v[3];
+            for (int d = 0; d < 3; ++d) {
+                v[d] = org[d] + rr[d] - 1;
+                if (v[d] < 0) v[d] += g.nc[d];
+                else if (v[d] >= g.nc[d]) v[d] -= g.nc[d];
+            }
+            const int c = v[0] + g.nc[0] * (v[1] + g.nc[1] * v[2]);
+            tot += start[c + 1] - start[c];
         }
-        const int c = v[0] + g.nc[0] * (v[1] + g.nc[1] * v[2]);
-        tot += start[c + 1] - start[c];
+        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
     }
-    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
-    if (lane == 0) part[b] = tot;
+    if (lane == 0) wm[threadIdx.x >> 6] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = wm[0];
+        for (int w = 1; w < FM_BLOCK / 64; ++w) m = max(m, wm[w]);
+        atomicMax(&flags[FLAG_MAX_REGION], m);
+    }
 }
 
 inline int nblk(size_t n) { return (int)((n + FM_BLOCK - 1) / FM_BLOCK); }
@@ -963,9 +1057,8 @@ size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind) {
 int launch_region_max(fm_ctx* c) {
     BG g = make_bg(c);
     int nbrick = g.nb[0] * g.nb[1] * g.nb[2];
-    k_region_max<<<nblk((size_t)nbrick * 64), FM_BLOCK, 0, c->stream>>>(g, c->cell_start, nbrick, c->part);
+    k_region_max<<<nblk((size_t)nbrick * 64), FM_BLOCK, 0, c->stream>>>(g, c->cell_start, nbrick, c->d_flags);
     LAUNCH_CHECK();
-    if (int rc = launch_max_reduce(c, nbrick, FLAG_MAX_REGION)) return rc;
     return 0;
 }
 
@@ -990,7 +1083,6 @@ int launch_nlist(fm_ctx* c) {
     a.xsorted = c->moved_since_bin ? 0 : 1;
     a.xwin = sqrt(c->rn2) * (1.0 + 1e-9) + 1e-12;
     a.order = c->brick_order;
-    a.part = c->part;
     for (int d = 0; d < 3; ++d) {
         a.cs[d] = c->cs[d];
         a.hd[d] = c->half[d];
@@ -1000,7 +1092,6 @@ int launch_nlist(fm_ctx* c) {
     if (a.mono) k_nlist_brick<true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
     else k_nlist_brick<false><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
     LAUNCH_CHECK();
-    if (int rc = launch_max_reduce(c, brick_count(c), FLAG_MAX_NB)) return rc;
     ktimer_end(c, KT_NLIST);
     return 0;
 }

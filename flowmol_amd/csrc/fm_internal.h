@@ -30,7 +30,7 @@ enum {
     FLAG_REBUILD = 0,     // displacement / vmax criterion fired
     FLAG_ERROR = 1,       // first FM_E* error code
     FLAG_ERROR_MOL = 2,   // molecule number of the first error
-    FLAG_MAX_CELL = 3,    // largest cell occupancy seen at the last binning
+    FLAG_MAX_CELL = 3,    // (not tracked)
     FLAG_MAX_NB = 4,      // largest neighbour count at the last build
     FLAG_NOUT = 5,        // molecules outside the box at sendmols
     FLAG_EDGE = 6,        // molecules exactly on the lower domain face (clamped into cell 1)
@@ -38,8 +38,10 @@ enum {
     FLAG_HALT = 8,        // a rebuild is pending: conditional launches of later steps do nothing
     FLAG_RB_STEP = 9,     // step whose displacement check set FLAG_REBUILD
     FLAG_SORT_MAX = 10,   // largest sort block (sort_mols)
-    FLAG_COUNT = 11
+    FLAG_NEED = 11,       // capacity requests of queued list builds: NEED_REGION, NEED_LIST bits
+    FLAG_COUNT = 12
 };
+enum { NEED_REGION = 1, NEED_LIST = 2 };
 
 struct KTimer;
 
@@ -76,7 +78,6 @@ struct fm_ctx {
     // LDS bricks
     int brick[3] = {1, 1, 1};
     int* brick_order = nullptr;           // bricks in 4x4x4-tile order (XCD-aware workgroup mapping)
-    int* part = nullptr;                  // per-workgroup maxima (cell, region and list sizes)
     int kt_step = 0;                      // step number attached to kernel timing records
 
     // sort_mols: block curve, the routine's saved counter and safety factors
@@ -89,9 +90,14 @@ struct fm_ctx {
     int *sort_key = nullptr, *sort_val = nullptr, *sort_key2 = nullptr, *sort_val2 = nullptr, *sort_cnt = nullptr;
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
-    int npart = 0;
-    int maxreg = 0;                       // largest staged region (molecules) since the last binning
+    // LDS capacity of a staged region (molecules): at least the largest region of the last
+    // measured binning, raised to what the LDS of as many workgroups per CU still holds, so that
+    // queued rebuilds run without a measurement; a kernel meeting a larger region raises
+    // NEED_REGION and FLAG_HALT instead of staging it
+    int maxreg = 0;
     size_t lds_limit = 65536;             // dynamic LDS per workgroup
+    size_t lds_cu = 65536;                // LDS per compute unit
+    bool tight_regions = false;           // capacity = measured largest region (no headroom)
 
     // neighbour list: 16-bit indices into the brick's LDS region layout, four per 64-bit word:
     // entries 4c..4c+3 of slot i in nbl[c*cap + i]
@@ -171,18 +177,18 @@ extern const char* const kt_names[KT_NCLASS];
 // ---- launchers (fm_kernels.hip, fm_brick.hip) ----------------------------------------------------
 int launch_fill_i32(fm_ctx* c, int* p, int v, size_t n);
 int launch_fill_f64(fm_ctx* c, double* p, double v, size_t n);
-int launch_pass1(fm_ctx* c);
+int launch_pass1(fm_ctx* c, bool cond = false);  // cond: do nothing while FLAG_HALT is set
 int launch_pass2(fm_ctx* c);
 int launch_check_disp(fm_ctx* c);
 int launch_check_vmax(fm_ctx* c);
 int launch_sendmols(fm_ctx* c);
 int launch_bin(fm_ctx* c);
+int launch_reset_flags(fm_ctx* c, const int* which, int n);  // zero the listed flag words
 int launch_nlist(fm_ctx* c);
 // cond: do nothing when a rebuild is pending (FLAG_REBUILD or FLAG_HALT), and raise FLAG_HALT
 int launch_forces(fm_ctx* c, int iter, bool kick = false, bool cond = false);
 int launch_pass1_check_disp(fm_ctx* c, bool cond = false, int step = 0);
 int launch_move_lfv(fm_ctx* c, bool check_disp);
-int launch_max_reduce(fm_ctx* c, int n, int flag);
 int launch_macros(fm_ctx* c);
 int launch_scatter_state_to_host_order(fm_ctx* c, double* d_out9, double* d_tally);
 int launch_from_host(fm_ctx* c, const double* d_in9);

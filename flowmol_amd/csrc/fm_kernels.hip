@@ -63,7 +63,9 @@ __global__ void k_fill_f64(double* p, double v, size_t n) {
 }
 
 // ---- velocity Verlet, src/simulation_move_particles_vv.f90:64-69 (pass 1), 132-135 (pass 2) ---
-__global__ __launch_bounds__(FM_BLOCK) void k_pass1(St s, int np, double dt, double hdt2, double hdt) {
+// halt: flag words of a conditional launch (does nothing while FLAG_HALT is set), or null
+__global__ __launch_bounds__(FM_BLOCK) void k_pass1(St s, int np, double dt, double hdt2, double hdt, const int* halt) {
+    if (halt && halt[FLAG_HALT] != 0) return;  // whole workgroup
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     double ax = s.ax[i], ay = s.ay[i], az = s.az[i];
@@ -396,10 +398,19 @@ __global__ __launch_bounds__(FM_BLOCK) void k_bin_count(St s, const int* ext, in
             cell = c[0] + nx * (c[1] + ny * c[2]);
             cell_of[i] = cell;
         } else {
+            // counted in cell 0 so that the permutation stays complete; the error stops the run
             set_err(flags, FM_EESCAPE, ext[i]);
+            cell = 0;
+            cell_of[i] = 0;
         }
     }
     wave_aggregated_add(count, cell, cell >= 0);
+}
+
+__global__ __launch_bounds__(FM_BLOCK) void k_zero2(int* a, size_t na, int* b, size_t nb) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < na) a[i] = 0;
+    if (i < nb) b[i] = 0;
 }
 
 __global__ __launch_bounds__(FM_BLOCK) void k_bin_scatter(const int* cell_of, const int* start, int* cursor,
@@ -416,22 +427,13 @@ __global__ __launch_bounds__(FM_BLOCK) void k_bin_scatter(const int* cell_of, co
 // molecules fall back to an insertion sort by lane 0
 __device__ inline bool xkey_less(double xa, int ea, double xb, int eb) { return xa < xb || (xa == xb && ea < eb); }
 
-// one wavefront per cell; part[blockIdx.x] = the largest cell of the workgroup
+// one wavefront per cell
 __global__ __launch_bounds__(FM_BLOCK) void k_bin_sort(const int* start, int* perm, const int* ext, const double* x,
-                                                       int ncell, int maxcell, int* flags, int* part) {
-    __shared__ int bmax[FM_BLOCK / 64];
+                                                       int ncell, int maxcell, int* flags) {
     const int lane = threadIdx.x & 63;
     const int c = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
-    const bool valid = c < ncell;
-    const int b = valid ? start[c] : 0, e = valid ? start[c + 1] : 0, n = e - b;
-    if (lane == 0) bmax[threadIdx.x >> 6] = n;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int m = bmax[0];
-        for (int w = 1; w < FM_BLOCK / 64; ++w) m = max(m, bmax[w]);
-        part[blockIdx.x] = m;
-    }
-    if (!valid) return;  // whole wavefront
+    if (c >= ncell) return;  // whole wavefront
+    const int b = start[c], e = start[c + 1], n = e - b;
     if (n > maxcell && lane == 0) set_err(flags, FM_EOVERFLOW, c);
     if (n <= 1) return;
     if (n <= 64) {
@@ -669,12 +671,12 @@ int launch_fill_f64(fm_ctx* c, double* p, double v, size_t n) {
     return 0;
 }
 
-int launch_pass1(fm_ctx* c) {
+int launch_pass1(fm_ctx* c, bool cond) {
     if (c->np == 0) return 0;
     double dt = c->p.delta_t;
     ktimer_begin(c, KT_PASS1);
     k_pass1<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(state_of(c->x(), c->cap), c->np, dt, 0.5 * (dt * dt),
-                                                         0.5 * dt);
+                                                         0.5 * dt, cond ? c->d_flags : nullptr);
     ktimer_end(c, KT_PASS1);
     LAUNCH_CHECK();
     return 0;
@@ -701,24 +703,21 @@ int launch_check_disp(fm_ctx* c) {
     return 0;
 }
 
-// flags[flag] = max(flags[flag], part[0..n)): one workgroup, the only writer of that flag
-__global__ __launch_bounds__(FM_BLOCK) void k_max_reduce(const int* part, int n, int* flag) {
-    __shared__ int sm[FM_BLOCK / 64];
-    int m = INT_MIN;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) m = max(m, part[k]);
-    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < FM_BLOCK / 64; ++w) m = max(m, sm[w]);
-        m = max(m, sm[0]);
-        if (m > *flag) *flag = m;
-    }
+// zero the listed flag words (one launch for the resets before a queued stage)
+struct FlagList {
+    int n;
+    int w[8];
+};
+__global__ void k_reset_flags(int* flags, FlagList l) {
+    if ((int)threadIdx.x < l.n) flags[l.w[threadIdx.x]] = 0;
 }
 
-int launch_max_reduce(fm_ctx* c, int n, int flag) {
-    if (n > c->npart) return fm_set_error(FM_EINVAL, "partial maxima exceed their buffer");
-    k_max_reduce<<<1, FM_BLOCK, 0, c->stream>>>(c->part, n, c->d_flags + flag);
+int launch_reset_flags(fm_ctx* c, const int* which, int n) {
+    if (n < 1 || n > 8) return fm_set_error(FM_EINVAL, "reset of %d flag words", n);
+    FlagList l;
+    l.n = n;
+    for (int k = 0; k < n; ++k) l.w[k] = which[k];
+    k_reset_flags<<<1, 64, 0, c->stream>>>(c->d_flags, l);
     LAUNCH_CHECK();
     return 0;
 }
@@ -869,8 +868,9 @@ int launch_bin(fm_ctx* c) {
     St s = state_of(c->x(), c->cap);
     const int nxt = 1 - c->cur;
     int* cell_tmp = c->out_flag;  // cell index in old slot order
-    if (int rc = launch_fill_i32(c, c->cell_count, 0, (size_t)c->ncell + 1)) return rc;
-    if (int rc = launch_fill_i32(c, c->cell_cursor, 0, (size_t)c->ncell)) return rc;
+    k_zero2<<<nblocks((size_t)c->ncell + 1), FM_BLOCK, 0, c->stream>>>(c->cell_count, (size_t)c->ncell + 1,
+                                                                        c->cell_cursor, (size_t)c->ncell);
+    LAUNCH_CHECK();
     k_bin_count<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(s, c->ext[c->cur], c->np, c->half[0], c->half[1],
                                                              c->half[2], c->cs[0], c->cs[1], c->cs[2], c->nc[0],
                                                              c->nc[1], c->nc[2], cell_tmp, c->cell_count,
@@ -883,9 +883,8 @@ int launch_bin(fm_ctx* c) {
     int maxcell = c->p.max_cell_np > 0 ? c->p.max_cell_np : (1 << 26);
     const int nsort = nblocks((size_t)c->ncell * 64);
     k_bin_sort<<<nsort, FM_BLOCK, 0, c->stream>>>(c->cell_start, c->perm, c->ext[c->cur], s.x, c->ncell, maxcell,
-                                                  c->d_flags, c->part);
+                                                  c->d_flags);
     LAUNCH_CHECK();
-    if (int rc = launch_max_reduce(c, nsort, FLAG_MAX_CELL)) return rc;
     PermArgs a;
     a.src = c->st[c->cur];
     a.dst = c->st[nxt];

@@ -75,6 +75,12 @@ class MDContext:
         check(load().fm_get_geometry(self._h, iptr(nc), dptr(cs), C.byref(rn2), C.byref(rc2), C.byref(ps)))
         return dict(ncells=nc, cellsidelength=cs, rneighbr2=rn2.value, rcutoff2=rc2.value, potshift=ps.value)
 
+    def capacity(self):
+        """(neighbour-list entries per molecule, molecules per staged brick region)."""
+        n, m = C.c_int32(), C.c_int32()
+        check(load().fm_get_capacity(self._h, C.byref(n), C.byref(m)))
+        return n.value, m.value
+
     # ---- state ---------------------------------------------------------------------------------
     def upload_state(self, r, v, a=None):
         r, v = _f64(r), _f64(v)

@@ -157,6 +157,14 @@ int  fm_abi_version(void);
 int  fm_get_geometry(const fm_ctx* ctx, int32_t ncells[3], double cellsidelength[3],
                      double* rneighbr2, double* rcutoff2, double* potshift);
 
+/* Capacities of the device lists: neighbour-list entries per molecule, and molecules per staged
+   brick region (LDS).  No counterpart in the reference, whose lists are linked lists.  Both grow
+   when a build meets more; a fused loop's queued build that meets more repeats itself after
+   growing them.  The region capacity is the largest region of the last measured binning raised to
+   what the LDS share of a workgroup holds; FLOWMOL_TIGHT_REGIONS=1 in the environment at fm_init
+   keeps it at the measured value. */
+int  fm_get_capacity(const fm_ctx* ctx, int32_t* list_entries, int32_t* region_molecules);
+
 /* ---- host <-> device state (arrays_MD, src/modules.f90:363-397) ---------------------------
    r, v, a: (3,np) AoS; a may be NULL (zeros).  moltype/tag/rtether may be NULL. */
 int  fm_upload_state(fm_ctx* ctx, int32_t np, const double* r, const double* v, const double* a);

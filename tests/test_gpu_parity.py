@@ -77,6 +77,10 @@ def test_forces_c1(c1_state):
     mg = g.evaluate_macroscopic_properties()
     for k in ("potenergy", "kinenergy", "pressure", "virial", "temperature"):
         assert abs(mg[k] - mo[k]) <= 1e-12 * max(1.0, abs(mo[k])), (k, mg[k], mo[k])
+    # a step without tallies (short, partly filled list words)
+    o.compute_forces(3)
+    g.simulation_compute_forces(3)
+    assert rel_force_err(g.state()[2], o.state()[2], scale) < FORCE_TOL
     g.close()
 
 
@@ -112,6 +116,10 @@ def test_forces_lj_rc25():
     peg, virg = g.molecular()
     np.testing.assert_allclose(peg, peo, rtol=FORCE_TOL, atol=1e-13)
     np.testing.assert_allclose(virg, viro, rtol=FORCE_TOL, atol=1e-11)
+    # a step without tallies
+    o.compute_forces(3)
+    g.simulation_compute_forces(3)
+    assert rel_force_err(g.state()[2], o.state()[2], o.force_scale()) < FORCE_TOL
     g.close()
 
 
@@ -204,8 +212,9 @@ def test_brick_shapes_same_sets_and_forces(brick):
     oi, oj, os_ = o.neighbourlist()
     gi, gj, gs = g.neighbourlist()
     assert np.array_equal(canonical_pairs(oi, oj, os_), canonical_pairs(gi, gj, gs))
-    o.compute_forces(0)
-    _, _, ao = o.state()
-    _, _, ag = g.state()
-    assert rel_force_err(ag, ao, o.force_scale()) < FORCE_TOL
+    for it in (0, 3):  # record step, and a step without tallies
+        o.compute_forces(it)
+        g.simulation_compute_forces(it)
+        assert rel_force_err(g.state()[2], o.state()[2], o.force_scale()) < FORCE_TOL
     g.close()
+

@@ -424,3 +424,37 @@ def test_sort_mols_tethered_walls_lfv():
     Po, Pg = o.mop(), g.mop()
     assert np.abs(Pg - Po).max() <= 1e-10 * max(1.0, np.abs(Po).max())
     g.close()
+
+
+def test_queued_rebuild_grows_capacities(monkeypatch):
+    """The fused loop queues its rebuilds without waiting on the device.  A queued build that meets
+    a list longer than the list capacity, or a staged region larger than the region capacity,
+    stops the launches queued after it; the loop grows the capacity and builds again.  A converging
+    flow packs the molecules together so that both happen, and the fused trajectory stays the
+    step-by-step one (which grows its capacities at each build)."""
+    p, r, v = config.config_fcc((6, 6, 6), 0.8, 2.5, seed=41, tplot=1000)
+    r = r + np.random.default_rng(41).uniform(-1e-3, 1e-3, r.shape)
+    v = v - 0.3 * r
+    p.max_neighbours = 78   # the lattice's count; capacity 80, exceeded once the flow packs them
+    say("case: queued rebuilds under a converging flow, 864 LJ, 60 steps")
+    o = Oracle(p, r, v)
+    o.setup()
+    o.compute_forces(0)
+    g = MDContext(p, r, v)
+    g.setup(0)
+    monkeypatch.setenv("FLOWMOL_TIGHT_REGIONS", "1")
+    f = MDContext(p, r, v)
+    monkeypatch.delenv("FLOWMOL_TIGHT_REGIONS")
+    f.setup(0)
+    nb0, reg0 = f.capacity()
+    assert nb0 == 80
+    ro, rg = drop_in_steps(o, g, 1, 60)
+    _, nrb = f.run_vv(1, 60)
+    nb1, reg1 = f.capacity()
+    assert nb1 > nb0 and reg1 > reg0, (nb0, nb1, reg0, reg1)
+    assert ro == rg and nrb == len(rg)
+    for x, y in zip(f.state(), g.state()):
+        assert np.array_equal(x, y)
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-10
+    f.close()
+    g.close()

@@ -445,6 +445,17 @@ def test_sort_hilbert_curve_visits_every_block_through_faces():
             where = {int(H[i, j, k]): np.array([i, j, k]) for i in range(nb[0]) for j in range(nb[1]) for k in range(nb[2])}
             steps = [np.abs(where[m + 1] - where[m]).sum() for m in range(1, int(np.prod(nb)))]
             assert all(s_ == 1 for s_ in steps)
+    # orientation, by hand from hilbert3's octant formulas: on 2^3 blocks the points
+    # 0.5*(+-0.5 ...) shifted by 0.25001 and scaled by 2 visit (2,2,2), (2,2,1), (1,2,1), (1,2,2),
+    # (1,1,2), (1,1,1), (2,1,1), (2,1,2); on 4^3 the first octant 0.5*(0.5+zo, 0.5+xo, 0.5+yo)
+    # starts at (4,4,4), (3,4,4), (3,3,4) and the last one ends at (4,1,4)
+    p = config.make_params((8.4, 8.4, 8.4), 0.5, 2.5, sort_flag=2, sortblocksize=2, tplot=1000)
+    H = Oracle(p, pos, np.zeros_like(pos)).hcurve((2, 2, 2))
+    walk = [(1, 1, 1), (1, 1, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1), (0, 0, 0), (1, 0, 0), (1, 0, 1)]
+    assert [int(H[b]) for b in walk] == list(range(1, 9))
+    p, o = _sort_setup(12.0, pos, 2)
+    H = o.hcurve((4, 4, 4))
+    assert (H[3, 3, 3], H[2, 3, 3], H[2, 2, 3], H[3, 0, 3]) == (1, 2, 3, 64)
     # SORT_FLAG 1: blocks numbered x fastest
     p, o = _sort_setup(12.0, pos, 1)
     H = o.hcurve((4, 4, 4))
