@@ -150,7 +150,7 @@ static void free_ctx(fm_ctx* c) {
                     c->cell_count, c->cell_start, c->cell_cursor, c->perm, c->nbl, c->nbn, c->bond, c->bondn,
                     c->pe_lj, c->pe_poly, c->vir, c->rf, c->pxy, c->cvface, c->planes, c->out_flag, c->out_vals,
                     c->out_slot, c->out_count, c->red, c->d_macros, c->d_mie, c->d_vmax, c->d_rmax,
-                    c->d_flags, c->stage, c->cub_tmp, c->brick_order, c->hcurve, c->sort_key,
+                    c->d_flags, c->stage, c->brick_order, c->hcurve, c->sort_key,
                     c->sort_val, c->sort_key2, c->sort_val2, c->sort_cnt, c->sort_tmp};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -356,7 +356,7 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
         (rc = dalloc(&c->cell_start, (size_t)c->ncell + 1)) || (rc = dalloc(&c->cell_cursor, (size_t)c->ncell)) ||
         (rc = dalloc(&c->perm, cap)) || (rc = dalloc(&c->nbn, cap)) || (rc = dalloc(&c->pe_lj, cap)) ||
         (rc = dalloc(&c->pe_poly, cap)) || (rc = dalloc(&c->vir, cap)) || (rc = dalloc(&c->out_flag, cap)) ||
-        (rc = dalloc(&c->out_vals, cap)) || (rc = dalloc(&c->out_slot, cap)) || (rc = dalloc(&c->out_count, 4)) ||
+        (rc = dalloc(&c->out_vals, cap)) || (rc = dalloc(&c->out_slot, cap)) || (rc = dalloc(&c->out_count, (cap + FM_BLOCK - 1) / FM_BLOCK + 4)) ||
         (rc = dalloc(&c->red, (size_t)c->nred_blocks * 8)) || (rc = dalloc(&c->d_macros, 16)) ||
         (rc = dalloc(&c->d_mie, 7 * (size_t)T2 + FM_MAX_TYPES)) || (rc = dalloc(&c->d_vmax, 1)) ||
         (rc = dalloc(&c->d_rmax, 1)) || (rc = dalloc(&c->d_flags, FLAG_COUNT)) || (rc = dalloc(&c->stage, 9 * cap)) ||
@@ -487,18 +487,14 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
             return fm_set_error(FM_EHIP, "table upload failed");
         }
     }
-    c->cub_bytes = cub_temp_bytes((int)std::max<size_t>(cap, (size_t)c->ncell + 1));
-    if ((rc = dalloc((char**)&c->cub_tmp, c->cub_bytes))) {
-        free_ctx(c);
-        return rc;
-    }
     he = hipHostMalloc((void**)&c->h_flags, FLAG_COUNT * sizeof(int), hipHostMallocDefault);
     if (he != hipSuccess) {
         free_ctx(c);
         return fm_set_error(FM_EHIP, "hipHostMalloc: %s", hipGetErrorString(he));
     }
     c->timer = new KTimer();
-    if ((rc = launch_fill_i32(c, c->d_flags, 0, FLAG_COUNT)) || (rc = launch_fill_f64(c, c->d_rmax, 0.0, 1)) ||
+    if ((rc = launch_fill_i32(c, c->d_flags, 0, FLAG_COUNT)) ||
+        (rc = launch_fill_i32(c, c->cell_count, 0, (size_t)c->ncell + 1)) || (rc = launch_fill_f64(c, c->d_rmax, 0.0, 1)) ||
         (rc = launch_fill_f64(c, c->st[0], 0.0, 9 * cap)) || (rc = launch_fill_f64(c, c->pe_lj, 0.0, cap)) ||
         (rc = launch_fill_f64(c, c->pe_poly, 0.0, cap)) || (rc = launch_fill_f64(c, c->vir, 0.0, cap))) {
         free_ctx(c);
@@ -1031,12 +1027,14 @@ int rebuild_lists_internal(fm_ctx* c) { return rebuild_lists(c); }
 // FLAG_NEED and FLAG_HALT, so the conditional launches queued after it do nothing until
 // redo_lists() has grown the capacities at the next synchronisation.
 static int rebuild_lists_queued(fm_ctx* c) {
-    if (c->maxreg <= 0) return rebuild_lists(c);
-    c->cells_valid = c->nlist_valid = false;
-    TRY(launch_sendmols(c));
-    TRY(fm_sort_mols(c));
     static const int reset[] = {FLAG_REBUILD, FLAG_HALT, FLAG_NEED, FLAG_MAX_REGION, FLAG_MAX_NB};
-    TRY(launch_reset_flags(c, reset, 5));
+    if (c->maxreg <= 0) {  // no measured capacity yet: the synchronous build
+        TRY(launch_reset_flags(c, reset, 2));
+        return rebuild_lists(c);
+    }
+    c->cells_valid = c->nlist_valid = false;
+    TRY(launch_sendmols(c, reset, 5));
+    TRY(fm_sort_mols(c));
     TRY(launch_bin(c));
     c->moved_since_bin = false;
     c->cells_valid = true;
@@ -1142,9 +1140,9 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
             const int s = c->h_flags[FLAG_RB_STEP];
             if (s < iter || s >= wend || s == lists)
                 return fm_set_error(FM_ESTATE, "rebuild requested at step %d outside %d..%d", s, iter, wend - 1);
+            // (the rebuild's first launch clears FLAG_REBUILD and FLAG_HALT)
             ktimer_drop(c, KT_FORCES, s);
             ktimer_drop(c, KT_PASS1, s + 1);
-            TRY(launch_reset_flags(c, reset, 2));
             for (int k = iter; k < s; ++k) TRY(finish_step(c, k, macros, n_macros, nm));
             rb = s;
             iter = s;

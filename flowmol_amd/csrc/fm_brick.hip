@@ -820,10 +820,19 @@ __device__ inline bool halted(int* f) {
 }
 
 template <bool MIE, bool RECORD, bool EXTRA, bool VIR>
-__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_forces_brick(FbArgs a) {
+__global__ __launch_bounds__(FB_BLOCK) void k_forces_brick(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt)) return;
     forces_body<MIE, RECORD, EXTRA, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+}
+
+// LJ steps without tallies, bonds or tethers (the common step): registers held to what lets two
+// workgroups share a CU (the LDS of two regions fits its 160 KB)
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128, FB_BLOCK / 128))) void
+k_forces_brick_plain(FbArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    if (halted(a.halt)) return;
+    forces_body<false, false, false, false>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // packed launch over many domains: block b works on brick table[b].y of domain table[b].x
@@ -1199,7 +1208,9 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond) {
         else { if (a.rec) FB(true, true, false); else FB(true, false, false); }
     } else {
         if (extra) { if (a.rec) FB(false, true, true); else FB(false, false, true); }
-        else { if (a.rec) FB(false, true, false); else FB(false, false, false); }
+        else if (a.rec) FB(false, true, false);
+        else if (a.ev) FB(false, false, false);
+        else k_forces_brick_plain<<<grid, blk, lds, c->stream>>>(a);
     }
 #undef FB
 #undef FM

@@ -123,7 +123,7 @@ struct fm_ctx {
     int* out_flag = nullptr;              // cap
     int* out_vals = nullptr;              // cap
     int* out_slot = nullptr;              // cap
-    int* out_count = nullptr;             // 1
+    int* out_count = nullptr;             // per-workgroup leaver counts
 
     // reductions
     double* red = nullptr;                // partial sums
@@ -138,8 +138,6 @@ struct fm_ctx {
     double* stage = nullptr;              // 9*cap doubles: host-order staging
     bool pe_valid = false, vir_valid = false;
 
-    void* cub_tmp = nullptr;
-    size_t cub_bytes = 0;
 
     bool cells_valid = false, nlist_valid = false, rdisp_valid = false;
     bool forces_valid = false;
@@ -181,7 +179,7 @@ int launch_pass1(fm_ctx* c, bool cond = false);  // cond: do nothing while FLAG_
 int launch_pass2(fm_ctx* c);
 int launch_check_disp(fm_ctx* c);
 int launch_check_vmax(fm_ctx* c);
-int launch_sendmols(fm_ctx* c);
+int launch_sendmols(fm_ctx* c, const int* reset = nullptr, int nreset = 0);  // also zeroes the listed flags
 int launch_bin(fm_ctx* c);
 int launch_reset_flags(fm_ctx* c, const int* which, int n);  // zero the listed flag words
 int launch_nlist(fm_ctx* c);
@@ -220,9 +218,6 @@ int launch_region_max(fm_ctx* c);
 int launch_decode_nlist(fm_ctx* c, uint32_t* out);
 
 // hipcub wrappers (fm_cub.hip)
-int cub_exclusive_scan(fm_ctx* c, const int* in, int* out, int n);
-int cub_select_flagged_index(fm_ctx* c, const int* flags, int* out, int* count, int n);
-size_t cub_temp_bytes(int n_items);
 size_t cub_sort_pairs_bytes(int n_items, int end_bit);
 int cub_sort_pairs(fm_ctx* c, const int* kin, int* kout, const int* vin, int* vout, int n, int end_bit);
 int launch_sort_mols(fm_ctx* c, int* overflow_out);

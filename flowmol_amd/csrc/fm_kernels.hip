@@ -255,30 +255,80 @@ __global__ void k_vmax_finish(unsigned long long* vmax_bits, double* rmax, doubl
 
 // ---- sendmols on one periodic rank, src/messenger.MPI.f90:1383-1904 ---------------------------
 // 1) mark molecules (in molecule-number order) that lie outside the box on any axis
-__global__ __launch_bounds__(FM_BLOCK) void k_out_flags(St s, const int* slot_of, int np, double hx,
-                                                        double hy, double hz, int* flag, int* flags) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= np) return;
-    int sl = slot_of[e];
-    double x = s.x[sl], y = s.y[sl], z = s.z[sl];
-    int out = (x < -hx) | (x >= hx) | (y < -hy) | (y >= hy) | (z < -hz) | (z >= hz);
-    flag[e] = out;
+// zero the listed flag words
+struct FlagList {
+    int n;
+    int w[8];
+};
+
+// 1) molecules outside the box, in molecule-number order: workgroup b writes its leavers e
+//    (ascending) to seg[b*FM_BLOCK ..] and their count to bcnt[b].  Workgroup 0 also zeroes the
+//    flag words of `reset` (the resets of a queued rebuild, folded into its first launch).
+__global__ __launch_bounds__(FM_BLOCK) void k_out_select(St s, const int* slot_of, int np, double hx, double hy,
+                                                         double hz, int* seg, int* bcnt, int* flags, FlagList reset) {
+    __shared__ int wc[FM_BLOCK / 64];
+    if (blockIdx.x == 0 && (int)threadIdx.x < reset.n) flags[reset.w[threadIdx.x]] = 0;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int out = 0;
+    if (e < np) {
+        const int sl = slot_of[e];
+        const double x = s.x[sl], y = s.y[sl], z = s.z[sl];
+        out = (x < -hx) | (x >= hx) | (y < -hy) | (y >= hy) | (z < -hz) | (z >= hz);
+    }
+    const unsigned long long m = __ballot(out);
+    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wc[wid] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int w = 0; w < FM_BLOCK / 64; ++w) {
+        if (w < wid) off += wc[w];
+        tot += wc[w];
+    }
+    if (out) seg[(size_t)blockIdx.x * FM_BLOCK + off + rank] = e;
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = tot;
 }
 
 // 2) one workgroup replays the 3 x (3 axes x 2 directions) passes on the compacted set.
 //    vals[q] ascending molecule numbers of the set (fixed); sl[q] slot currently holding vals[q].
 //    Each pass: leavers in q order (= ascending molecule number) are wrapped by -dir*domain and
 //    their molecule numbers reversed, as checksendbuild + sendrecvface + reorderdata do.
+//    The set is first gathered from the segments of k_out_select in workgroup order.
 __global__ __launch_bounds__(1024) void k_sendmols_passes(St s, double* teth, const int* tag, int* ext,
-                                                          int* slot_of, const int* vals, int* sl,
-                                                          int* lst, int* tmp, const int* count,
+                                                          int* slot_of, const int* seg, const int* bcnt, int nblk,
+                                                          int* vals, int* sl, int* lst, int* tmp,
                                                           int cap, double Lx, double Ly, double Lz,
                                                           int tag_move, double wsx, double wsy,
                                                           double wsz) {
     __shared__ int wsum[16];
     __shared__ int chunk_base;
-    const int n = *count;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    {  // thread t gathers the segments of a contiguous range of workgroups
+        const int per = (nblk + (int)blockDim.x - 1) / (int)blockDim.x;
+        const int b0 = min(nblk, tid * per), b1 = min(nblk, b0 + per);
+        int mine = 0;
+        for (int b = b0; b < b1; ++b) mine += bcnt[b];
+        int x = mine;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
+        __syncthreads();
+        int pos = x - mine, tot = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            if (w < wid) pos += wsum[w];
+            tot += wsum[w];
+        }
+        for (int b = b0; b < b1; ++b) {
+            const int cb = bcnt[b];
+            for (int k = 0; k < cb; ++k) vals[pos++] = seg[(size_t)b * FM_BLOCK + k];
+        }
+        if (tid == 0) chunk_base = tot;
+        __syncthreads();
+    }
+    const int n = chunk_base;
+    __syncthreads();
     for (int q = tid; q < n; q += blockDim.x) sl[q] = slot_of[vals[q]];
     __syncthreads();
     const double L[3] = {Lx, Ly, Lz};
@@ -407,10 +457,62 @@ __global__ __launch_bounds__(FM_BLOCK) void k_bin_count(St s, const int* ext, in
     wave_aggregated_add(count, cell, cell >= 0);
 }
 
-__global__ __launch_bounds__(FM_BLOCK) void k_zero2(int* a, size_t na, int* b, size_t nb) {
-    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i < na) a[i] = 0;
-    if (i < nb) b[i] = 0;
+// start[0..n) = exclusive prefix of count[0..n) by one workgroup, in tiles of 14336 counts (60 KB
+// of LDS): a coalesced pass into LDS (one padding word per 14, so that the threads' runs of 14
+// consecutive counts start an odd number of words apart, in distinct banks), a scan by thread,
+// wavefront and workgroup, and a coalesced pass out.
+// The counts are zeroed as they are read (ready for the next binning) and so is cursor[0..ncur).
+#define SCAN_PER 14
+#define SCAN_TILE (1024 * SCAN_PER)
+__global__ __launch_bounds__(1024) void k_scan_cells(int* count, int* start, int n, int* cursor, int ncur) {
+    __shared__ int t[SCAN_TILE + SCAN_TILE / SCAN_PER];
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int carry = 0;
+    for (int base = 0; base < n; base += SCAN_TILE) {
+        int v[SCAN_PER], s = 0;
+        // all loads first (indices past the end re-read the last count), then the stores
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; ++k) v[k] = count[min(base + k * 1024 + tid, n - 1)];
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; ++k) {
+            const int j = k * 1024 + tid, i = base + j;
+            if (i < n) count[i] = 0;
+            t[j + j / SCAN_PER] = i < n ? v[k] : 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; ++k) {
+            v[k] = t[tid * (SCAN_PER + 1) + k];
+            s += v[k];
+        }
+        int x = s;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
+        __syncthreads();
+        int pre = carry + x - s, tot = 0;
+        for (int w = 0; w < 16; ++w) {
+            if (w < wid) pre += wsum[w];
+            tot += wsum[w];
+        }
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; ++k) {
+            t[tid * (SCAN_PER + 1) + k] = pre;
+            pre += v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; ++k) {
+            const int j = k * 1024 + tid, i = base + j;
+            if (i < n) start[i] = t[j + j / SCAN_PER];
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    for (int i = tid; i < ncur; i += blockDim.x) cursor[i] = 0;
 }
 
 __global__ __launch_bounds__(FM_BLOCK) void k_bin_scatter(const int* cell_of, const int* start, int* cursor,
@@ -704,10 +806,6 @@ int launch_check_disp(fm_ctx* c) {
 }
 
 // zero the listed flag words (one launch for the resets before a queued stage)
-struct FlagList {
-    int n;
-    int w[8];
-};
 __global__ void k_reset_flags(int* flags, FlagList l) {
     if ((int)threadIdx.x < l.n) flags[l.w[threadIdx.x]] = 0;
 }
@@ -843,20 +941,25 @@ int launch_check_vmax(fm_ctx* c) {
     return 0;
 }
 
-int launch_sendmols(fm_ctx* c) {
-    if (c->np == 0) return 0;
+int launch_sendmols(fm_ctx* c, const int* reset, int nreset) {
+    if (nreset > 8) return fm_set_error(FM_EINVAL, "reset of %d flag words", nreset);
+    FlagList l;
+    l.n = nreset;
+    for (int k = 0; k < nreset; ++k) l.w[k] = reset[k];
+    if (c->np == 0) return nreset ? launch_reset_flags(c, reset, nreset) : 0;
     ktimer_begin(c, KT_SENDMOLS);
     St s = state_of(c->x(), c->cap);
-    k_out_flags<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(s, c->slot_of, c->np, c->half[0], c->half[1],
-                                                             c->half[2], c->out_flag, c->d_flags);
+    const int nb = nblocks(c->np);
+    // segments in the flag buffer, counts in out_count; lst and tmp of the passes reuse the cell
+    // permutation buffer and the flag buffer (after the gather)
+    k_out_select<<<nb, FM_BLOCK, 0, c->stream>>>(s, c->slot_of, c->np, c->half[0], c->half[1], c->half[2],
+                                                   c->out_flag, c->out_count, c->d_flags, l);
     LAUNCH_CHECK();
-    if (int rc = cub_select_flagged_index(c, c->out_flag, c->out_vals, c->out_count, c->np)) return rc;
     const bool tm = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && c->has_tags;
-    // lst and tmp reuse the cell permutation buffer and the flag buffer
     k_sendmols_passes<<<1, 1024, 0, c->stream>>>(
         s, tm ? c->teth[c->cur] : nullptr, tm ? c->tag[c->cur] : nullptr, c->ext[c->cur], c->slot_of,
-        c->out_vals, c->out_slot, c->perm, c->out_flag, c->out_count, c->cap, c->p.domain[0], c->p.domain[1],
-        c->p.domain[2], tm ? 1 : 0, c->p.wallslidev[0], c->p.wallslidev[1], c->p.wallslidev[2]);
+        c->out_flag, c->out_count, nb, c->out_vals, c->out_slot, c->perm, c->out_flag, c->cap, c->p.domain[0],
+        c->p.domain[1], c->p.domain[2], tm ? 1 : 0, c->p.wallslidev[0], c->p.wallslidev[1], c->p.wallslidev[2]);
     ktimer_end(c, KT_SENDMOLS);
     LAUNCH_CHECK();
     return 0;
@@ -868,15 +971,13 @@ int launch_bin(fm_ctx* c) {
     St s = state_of(c->x(), c->cap);
     const int nxt = 1 - c->cur;
     int* cell_tmp = c->out_flag;  // cell index in old slot order
-    k_zero2<<<nblocks((size_t)c->ncell + 1), FM_BLOCK, 0, c->stream>>>(c->cell_count, (size_t)c->ncell + 1,
-                                                                        c->cell_cursor, (size_t)c->ncell);
-    LAUNCH_CHECK();
     k_bin_count<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(s, c->ext[c->cur], c->np, c->half[0], c->half[1],
                                                              c->half[2], c->cs[0], c->cs[1], c->cs[2], c->nc[0],
                                                              c->nc[1], c->nc[2], cell_tmp, c->cell_count,
                                                              c->d_flags);
     LAUNCH_CHECK();
-    if (int rc = cub_exclusive_scan(c, c->cell_count, c->cell_start, c->ncell + 1)) return rc;
+    k_scan_cells<<<1, 1024, 0, c->stream>>>(c->cell_count, c->cell_start, c->ncell + 1, c->cell_cursor, c->ncell);
+    LAUNCH_CHECK();
     k_bin_scatter<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(cell_tmp, c->cell_start, c->cell_cursor, c->perm,
                                                                c->np);
     LAUNCH_CHECK();
