@@ -26,7 +26,7 @@ extern "C" int fm_abi_version(void) { return FM_ABI_VERSION; }
 
 // ---- kernel timing ----------------------------------------------------------------------------
 const char* const kt_names[KT_NCLASS] = {"vv_pass1", "checkrebuild", "sendmols", "assign_to_cell",
-                                         "neighbourlist", "forces", "vv_pass2", "macros"};
+                                         "neighbourlist", "forces", "vv_pass2", "macros", "forces_pass1"};
 
 struct KTimer {
     bool on = false;
@@ -1082,7 +1082,10 @@ static int finish_step(fm_ctx* c, int iter, fm_macros* macros, int n_macros, int
 // records its number and the launches after it do nothing; the next window then starts with that
 // step's rebuild and force call.  A window that starts with a rebuild queues it too (the list
 // capacities are checked at the window's synchronisation, and a build beyond them is repeated
-// after growing them).  The arithmetic of every completed step is that of the step-by-step loop.
+// after growing them).  Inside a window the force launch of a plain step also applies VV pass 1
+// of the next step and its displacement check, writing the state into the other buffer; the host
+// keeps which buffer holds the state after each step's pass 1 for the rollback to a fired check.
+// The arithmetic of every completed step is that of the step-by-step loop.
 static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, int n_macros, int* nrb, int* nm) {
     const int end = iter0 + nsteps, W = 16;
     const bool disp = c->p.rebuild_criteria == FM_REBUILD_DISPLACEMENT;
@@ -1094,40 +1097,50 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
     TRY(launch_reset_flags(c, reset, 2));
     int iter = iter0;
     int rb = -1;  // step whose displacement check fired (its pass 1 has run)
+    int scur_p1[W];
     while (iter < end) {
         ktimer_make_room(c, 4 * W + 64);
-        int wend = iter, lists = -1;
-        if (rb == iter || host_rebuild(iter)) {
-            c->kt_step = iter;
-            if (rb != iter) {
+        const bool rebuild_first = rb == iter || host_rebuild(iter);
+        int wend = iter + 1;
+        while (wend < end && wend - iter < W && !host_rebuild(wend) && !sums_wanted(wend - 1)) ++wend;
+        int lists = -1, scur_lists = 0;
+        bool fused_prev = false;
+        for (int k = iter; k < wend; ++k) {
+            c->kt_step = k;
+            if (k == iter && rebuild_first) {
+                if (rb != iter) {
+                    c->moved_since_bin = true;
+                    TRY(launch_pass1(c));
+                }
+                scur_p1[0] = c->scur;
+                c->rdisp_valid = false;
+                TRY(rebuild_lists_queued(c));
+                ++*nrb;
+                lists = iter;
+                scur_lists = c->scur;
+            } else if (!fused_prev) {
                 c->moved_since_bin = true;
-                TRY(launch_pass1(c));
+                if (disp) TRY(launch_pass1_check_disp(c, true, k));
+                else TRY(launch_pass1(c, lists >= 0));
+                scur_p1[k - iter] = c->scur;
+            } else {
+                scur_p1[k - iter] = c->scur;  // pass 1 of k ran inside the force launch of k-1
             }
-            c->rdisp_valid = false;
-            TRY(rebuild_lists_queued(c));
-            ++*nrb;
-            TRY(launch_forces(c, iter, true, true));
-            lists = iter;
-            wend = iter + 1;
-        }
-        rb = -1;
-        while (wend < end && wend - iter < W && !host_rebuild(wend) && !(wend > iter && sums_wanted(wend - 1))) {
-            c->kt_step = wend;
-            c->moved_since_bin = true;
-            if (disp) TRY(launch_pass1_check_disp(c, true, wend));
-            else TRY(launch_pass1(c, lists >= 0));
-            TRY(launch_forces(c, wend, true, disp || lists >= 0));
-            ++wend;
+            const int next = (disp && k + 1 < wend && !sums_wanted(k)) ? k + 1 : -1;
+            TRY(launch_forces(c, k, true, disp || lists >= 0, next, &fused_prev));
+            if (fused_prev) c->moved_since_bin = true;
         }
         TRY(sync_flags(c));
         if (c->h_flags[FLAG_NEED]) {
             if (lists < 0) return fm_set_error(FM_ESTATE, "capacity request without a queued list build");
             ktimer_drop(c, KT_NLIST, lists);
             ktimer_drop(c, KT_FORCES, lists);
+            ktimer_drop(c, KT_FORCES_P1, lists);
             ktimer_drop(c, KT_PASS1, lists + 1);
-            // the launches after the build did nothing: positions are those of the binning, and
-            // the displacement sums start again after the rebuild
+            // the launches after the build did nothing: the state is that of the binning, and the
+            // displacement sums start again after the rebuild
             c->kt_step = lists;
+            c->scur = scur_lists;
             c->moved_since_bin = false;
             c->rdisp_valid = false;
             TRY(redo_lists(c));
@@ -1142,7 +1155,9 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
                 return fm_set_error(FM_ESTATE, "rebuild requested at step %d outside %d..%d", s, iter, wend - 1);
             // (the rebuild's first launch clears FLAG_REBUILD and FLAG_HALT)
             ktimer_drop(c, KT_FORCES, s);
+            ktimer_drop(c, KT_FORCES_P1, s);
             ktimer_drop(c, KT_PASS1, s + 1);
+            c->scur = scur_p1[s - iter];
             for (int k = iter; k < s; ++k) TRY(finish_step(c, k, macros, n_macros, nm));
             rb = s;
             iter = s;

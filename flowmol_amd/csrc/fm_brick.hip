@@ -472,6 +472,12 @@ struct FbArgs {
     double* cv;            // faces 1..3 of the control-volume tally, or null
     int cvnbo[3], cvnhb[3];
     double cvbs[3], cvhd[3];
+    // pass 1 of the next step fused into the force launch (k_forces_brick_plain<true>): the state
+    // after it goes to s2 (the other state buffer), with the displacement update of rd
+    St s2;
+    double* rd;
+    double hdt2, thresh;
+    int fresh, p1step;
 };
 
 // forward cell offsets of the reference's half-list scan (src/linklist.f90:686-688)
@@ -629,8 +635,10 @@ __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row,
     fz = 24.0 * sz;
 }
 
-// VIR: virial on a non-record step (teval); record steps always tally it
-template <bool MIE, bool RECORD, bool EXTRA, bool VIR = RECORD>
+// VIR: virial on a non-record step (teval); record steps always tally it.  P1: VV pass 2 of this
+// step, pass 1 of the next one and its displacement update follow the force (same arithmetic as
+// the separate kernels), written to the other state buffer.
+template <bool MIE, bool RECORD, bool EXTRA, bool VIR = RECORD, bool P1 = false>
 __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* smem) {
     Region r;
     const bool mop = RECORD && a.pxy != nullptr;
@@ -787,6 +795,38 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                 for (int d = 0; d < 3; ++d) T[i + d * cap] = T[i + d * cap] + T[i + (3 + d) * cap] * a.dt;
             }
         }
+        if constexpr (P1) {
+            // simulation_move_particles_vv(2) of this step, then (1) of the next step and the
+            // rdisp update of rebuild criterion 2, as k_pass2 and k_pass1_disp compute them
+            const double ux = a.s.vx[i] + a.hdt * fx, uy = a.s.vy[i] + a.hdt * fy, uz = a.s.vz[i] + a.hdt * fz;
+            a.s2.x[i] = a.s.x[i] + a.dt * ux + a.hdt2 * fx;
+            a.s2.y[i] = a.s.y[i] + a.dt * uy + a.hdt2 * fy;
+            a.s2.z[i] = a.s.z[i] + a.dt * uz + a.hdt2 * fz;
+            const double wx = ux + a.hdt * fx, wy = uy + a.hdt * fy, wz = uz + a.hdt * fz;
+            a.s2.vx[i] = wx;
+            a.s2.vy[i] = wy;
+            a.s2.vz[i] = wz;
+            a.s2.ax[i] = fx;
+            a.s2.ay[i] = fy;
+            a.s2.az[i] = fz;
+            const size_t cp = a.cap;
+            double d0 = a.fresh ? 0.0 : a.rd[i], d1 = a.fresh ? 0.0 : a.rd[i + cp],
+                   d2 = a.fresh ? 0.0 : a.rd[i + 2 * cp];
+            d0 = d0 + wx * a.dt;
+            d1 = d1 + wy * a.dt;
+            d2 = d2 + wz * a.dt;
+            a.rd[i] = d0;
+            a.rd[i + cp] = d1;
+            a.rd[i + 2 * cp] = d2;
+            const bool hit = fabs(d0) > a.thresh || fabs(d1) > a.thresh || fabs(d2) > a.thresh;
+            const unsigned long long m = __ballot(hit);
+            if (m != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1 &&
+                __atomic_load_n(&a.flags[FLAG_REBUILD], __ATOMIC_RELAXED) == 0) {
+                atomicOr(&a.flags[FLAG_REBUILD], 1);
+                atomicExch(&a.flags[FLAG_RB_STEP], a.p1step);
+            }
+            continue;
+        }
         a.s.ax[i] = fx;
         a.s.ay[i] = fy;
         a.s.az[i] = fz;
@@ -828,11 +868,13 @@ __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick(FbArgs a) {
 
 // LJ steps without tallies, bonds or tethers (the common step): registers held to what lets two
 // workgroups share a CU (the LDS of two regions fits its 160 KB)
+// (P1: with pass 1 of the next step)
+template <bool P1>
 __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128, FB_BLOCK / 128))) void
 k_forces_brick_plain(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt)) return;
-    forces_body<false, false, false, false>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+    forces_body<false, false, false, false, P1>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // packed launch over many domains: block b works on brick table[b].y of domain table[b].x
@@ -1182,7 +1224,8 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     return 0;
 }
 
-int launch_forces(fm_ctx* c, int iter, bool kick, bool cond) {
+int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fused) {
+    if (fused) *fused = false;
     if (c->np == 0) return 0;
     FbArgs a;
     size_t lds = 0;
@@ -1190,6 +1233,23 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond) {
     if (int rc = make_fargs(c, iter, kick, &a, &lds, &kind)) return rc;
     if (cond) a.halt = c->d_flags;
     const bool mixed = kind & 1, mie = kind & 2, extra = kind & 4;
+    if (next >= 0 && kind == 0 && kick && !a.rec && !a.ev && fused) {
+        // this step's force and kick, then pass 1 of step `next` into the other state buffer
+        a.s2 = state_of(c->st[1 - c->scur], c->cap);
+        a.rd = c->rdisp;
+        a.hdt2 = 0.5 * (c->p.delta_t * c->p.delta_t);
+        a.thresh = 0.5 * c->dmin;
+        a.fresh = c->rdisp_valid ? 0 : 1;
+        a.p1step = next;
+        ktimer_begin(c, KT_FORCES_P1);
+        k_forces_brick_plain<true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
+        ktimer_end(c, KT_FORCES_P1);
+        LAUNCH_CHECK();
+        c->scur = 1 - c->scur;
+        c->rdisp_valid = true;
+        *fused = true;
+        return 0;
+    }
     ktimer_begin(c, KT_FORCES);
     dim3 grid(brick_count(c)), blk(FB_BLOCK);
 #define FB(M, R, E) \
@@ -1210,7 +1270,7 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond) {
         if (extra) { if (a.rec) FB(false, true, true); else FB(false, false, true); }
         else if (a.rec) FB(false, true, false);
         else if (a.ev) FB(false, false, false);
-        else k_forces_brick_plain<<<grid, blk, lds, c->stream>>>(a);
+        else k_forces_brick_plain<false><<<grid, blk, lds, c->stream>>>(a);
     }
 #undef FB
 #undef FM

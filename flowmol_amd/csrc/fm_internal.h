@@ -147,7 +147,8 @@ struct fm_ctx {
 
     KTimer* timer = nullptr;
 
-    double* x() const { return st[cur]; }
+    int scur = 0;                         // state buffer in use (st[scur]); ext & co. follow cur
+    double* x() const { return st[scur]; }
 };
 
 // ---- error handling -----------------------------------------------------------------------------
@@ -169,7 +170,7 @@ int fm_set_error(int code, const char* fmt, ...);
 // ---- kernel timing (HIP events on the context stream) -------------------------------------------
 void ktimer_begin(fm_ctx* c, int cls);
 void ktimer_end(fm_ctx* c, int cls);
-enum { KT_PASS1 = 0, KT_CHECK, KT_SENDMOLS, KT_BIN, KT_NLIST, KT_FORCES, KT_PASS2, KT_MACROS, KT_NCLASS };
+enum { KT_PASS1 = 0, KT_CHECK, KT_SENDMOLS, KT_BIN, KT_NLIST, KT_FORCES, KT_PASS2, KT_MACROS, KT_FORCES_P1, KT_NCLASS };
 extern const char* const kt_names[KT_NCLASS];
 
 // ---- launchers (fm_kernels.hip, fm_brick.hip) ----------------------------------------------------
@@ -183,8 +184,11 @@ int launch_sendmols(fm_ctx* c, const int* reset = nullptr, int nreset = 0);  // 
 int launch_bin(fm_ctx* c);
 int launch_reset_flags(fm_ctx* c, const int* which, int n);  // zero the listed flag words
 int launch_nlist(fm_ctx* c);
-// cond: do nothing when a rebuild is pending (FLAG_REBUILD or FLAG_HALT), and raise FLAG_HALT
-int launch_forces(fm_ctx* c, int iter, bool kick = false, bool cond = false);
+// cond: do nothing when a rebuild is pending (FLAG_REBUILD or FLAG_HALT), and raise FLAG_HALT.
+// next >= 0 with fused != null: on a plain LJ step with the kick, also apply VV pass 1 of step
+// `next` and its displacement check in the same launch, into the other state buffer (c->scur
+// flips); *fused reports whether that happened.
+int launch_forces(fm_ctx* c, int iter, bool kick = false, bool cond = false, int next = -1, bool* fused = nullptr);
 int launch_pass1_check_disp(fm_ctx* c, bool cond = false, int step = 0);
 int launch_move_lfv(fm_ctx* c, bool check_disp);
 int launch_macros(fm_ctx* c);

@@ -987,8 +987,8 @@ int launch_bin(fm_ctx* c) {
                                                   c->d_flags);
     LAUNCH_CHECK();
     PermArgs a;
-    a.src = c->st[c->cur];
-    a.dst = c->st[nxt];
+    a.src = c->st[c->scur];
+    a.dst = c->st[1 - c->scur];
     a.ext_src = c->ext[c->cur];
     a.ext_dst = c->ext[nxt];
     a.cell_src = cell_tmp;
@@ -1009,6 +1009,7 @@ int launch_bin(fm_ctx* c) {
     ktimer_end(c, KT_BIN);
     LAUNCH_CHECK();
     c->cur = nxt;
+    c->scur = 1 - c->scur;
     return 0;
 }
 
