@@ -866,6 +866,16 @@ __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick(FbArgs a) {
     forces_body<MIE, RECORD, EXTRA, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
+// LJ record and virial steps: registers held to what lets two workgroups share a CU (a few
+// spill on record steps, which beats one workgroup per CU)
+template <bool RECORD, bool VIR>
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128))) void
+k_forces_brick_lj(FbArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    if (halted(a.halt)) return;
+    forces_body<false, RECORD, false, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+}
+
 // LJ steps without tallies, bonds or tethers (the common step): registers held to what lets two
 // workgroups share a CU (the LDS of two regions fits its 160 KB)
 // (P1: with pass 1 of the next step)
@@ -1268,8 +1278,8 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         else { if (a.rec) FB(true, true, false); else FB(true, false, false); }
     } else {
         if (extra) { if (a.rec) FB(false, true, true); else FB(false, false, true); }
-        else if (a.rec) FB(false, true, false);
-        else if (a.ev) FB(false, false, false);
+        else if (a.rec) k_forces_brick_lj<true, true><<<grid, blk, lds, c->stream>>>(a);
+        else if (a.ev) k_forces_brick_lj<false, true><<<grid, blk, lds, c->stream>>>(a);
         else k_forces_brick_plain<false><<<grid, blk, lds, c->stream>>>(a);
     }
 #undef FB
