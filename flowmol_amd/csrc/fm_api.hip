@@ -1101,8 +1101,18 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
     while (iter < end) {
         ktimer_make_room(c, 4 * W + 64);
         const bool rebuild_first = rb == iter || host_rebuild(iter);
+        if (rebuild_first) {
+            if (rb == iter && c->rb_last >= 0) c->rb_gap = iter - c->rb_last;
+            c->rb_last = iter;
+        }
+        // With the displacement criterion the window ends one step after the step at which the
+        // check is expected to fire (the last rebuild plus the last interval between a rebuild and
+        // a fired check): launches queued after a fired check only cost their launch, and a
+        // window that ends before the check fires costs one more synchronisation.
+        int wmax = W;
+        if (disp && c->rb_gap > 0 && c->rb_last >= 0) wmax = std::min(W, std::max(3, c->rb_last + c->rb_gap + 2 - iter));
         int wend = iter + 1;
-        while (wend < end && wend - iter < W && !host_rebuild(wend) && !sums_wanted(wend - 1)) ++wend;
+        while (wend < end && wend - iter < wmax && !host_rebuild(wend) && !sums_wanted(wend - 1)) ++wend;
         int lists = -1, scur_lists = 0;
         bool fused_prev = false;
         for (int k = iter; k < wend; ++k) {

@@ -148,6 +148,7 @@ struct fm_ctx {
     KTimer* timer = nullptr;
 
     int scur = 0;                         // state buffer in use (st[scur]); ext & co. follow cur
+    int rb_last = -1, rb_gap = 0;         // fused VV loop: last rebuild step, last rebuild-to-check interval
     double* x() const { return st[scur]; }
 };
 
