@@ -39,7 +39,9 @@ def parse():
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU sample (after its setup)")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP event timing")
-    ap.add_argument("--workload", choices=["c2", "serve"], default="c2",
+    ap.add_argument("--drift-steps", type=int, default=10000, help="drift: NVE steps per precision")
+    ap.add_argument("--c3-steps", type=int, default=100, help="c3: Couette steps per precision")
+    ap.add_argument("--workload", choices=["c2", "serve", "drift", "c3"], default="c2",
                     help="c2: 1M-molecule LJ step throughput (default); serve: replica stream latency")
     ap.add_argument("--replicas", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=6.0, help="serve: Poisson arrival rate (1/s)")
@@ -118,6 +120,118 @@ def serve_main(a):
     print(json.dumps(out), flush=True)
 
 
+def drift_main(a):
+    """NVE total-energy drift of the 1M-molecule LJ workload over --drift-steps steps, FP64 and
+    mixed precision from the same start (sums every TPLOT=10 steps)."""
+    import numpy as np
+
+    from flowmol_amd import config
+    from flowmol_amd._abi import FM_PREC_FP64, FM_PREC_MIXED
+    from flowmol_amd.md import MDContext
+
+    n_u, K = a.nunits, a.drift_steps
+    res = {}
+    for name, prec in (("fp64", FM_PREC_FP64), ("mixed", FM_PREC_MIXED)):
+        p, r, v = config.config_fcc((n_u, n_u, n_u), 0.8, 2.5, seed=2, tplot=10, precision=prec)
+        N = r.shape[0]
+        g = MDContext(p, r, v)
+        g.setup(0)
+        m0 = g.evaluate_macroscopic_properties()
+        g.synchronize()
+        t0 = time.perf_counter()
+        ms, nrb = g.run_vv(1, K, n_macros=K // 10)
+        g.synchronize()
+        dt = time.perf_counter() - t0
+        e = np.array([m0["totenergy"]] + [m["totenergy"] for m in ms])
+        t = np.arange(len(e)) * 10 * p.delta_t
+        slope = np.polyfit(t, e, 1)[0]
+        res[name] = {"atom_steps_per_s": N * K / dt, "rebuilds": nrb, "e0": e[0], "e_end": e[-1],
+                     "max_rel_dev": float(np.max(np.abs(e - e[0])) / abs(e[0])),
+                     "drift_per_time_unit": float(slope / abs(e[0]))}
+        g.close()
+    out = {
+        "metric": f"NVE total-energy drift over {K} steps (1M LJ, 1 GPU)",
+        "value": res["fp64"]["max_rel_dev"],
+        "unit": "max |E(t)-E(0)|/|E(0)|",
+        "higher_is_better": False,
+        "n_gpus": 1,
+        "steps": K,
+        "fp64": res["fp64"],
+        "mixed": res["mixed"],
+        "mixed_over_fp64_drift": res["mixed"]["max_rel_dev"] / max(res["fp64"]["max_rel_dev"], 1e-300),
+        "dtype": "f64 (and f32-pair/f64-state)",
+        "data": "synthetic",
+        "vs_baseline": None,
+        "config": {"workload": f"LJ NVE FCC {n_u}^3, density 0.8, rc 2.5, delta 0.3, dt 0.005, T 1.0, TPLOT 10"},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def c3_main(a):
+    """Couette flow between tethered FCC walls (3 lattice units, top wall at U = 1), MOP stress on
+    64 planes normal to y every step, leap-frog (the reference's tag_move path): FP64 and mixed
+    precision from the same start; the mixed run's force and MOP errors against FP64."""
+    import numpy as np
+
+    from flowmol_amd import config
+    from flowmol_amd._abi import FM_ENSEMBLE_NVE, FM_INTEGRATION_LFV, FM_PREC_FP64, FM_PREC_MIXED
+    from flowmol_amd.md import MDContext
+
+    n_u, K = a.nunits, a.c3_steps
+    res, mop, end = {}, {}, {}
+    for name, prec in (("fp64", FM_PREC_FP64), ("mixed", FM_PREC_MIXED)):
+        p, r, v, tag, rt = config.config_couette(nunits=(n_u, n_u, n_u), nplanes=64, tplot=1, precision=prec)
+        p.integration_algorithm = FM_INTEGRATION_LFV
+        N = r.shape[0]
+        g = MDContext(p, r, v)
+        g.set_tags(tag, rt)
+        g.setup(0)
+        g.reset_mop()
+        g.synchronize()
+        t0 = time.perf_counter()
+        _, nrb = g.run_lfv(0, K)
+        g.synchronize()
+        dt = time.perf_counter() - t0
+        mop[name] = g.mop()
+        end[name] = (p, g.state())
+        res[name] = {"atom_steps_per_s": N * K / dt, "rebuilds": nrb, "ms_per_step": dt / K * 1e3}
+        g.close()
+    # pair-force error on the sheared state the FP64 run reached (the lattice start has zero
+    # forces); tethers act in the leap-frog move, not in the force call, so NVE parameters serve
+    acc = {}
+    r1, v1, _ = end["fp64"][1]
+    for name in ("fp64", "mixed"):
+        q = type(end[name][0]).from_buffer_copy(end[name][0])
+        q.ensemble = FM_ENSEMBLE_NVE
+        g = MDContext(q, r1, v1)
+        g.setup(0)
+        acc[name] = g.state()[2]
+        g.close()
+    a64, a32 = acc["fp64"], acc["mixed"]
+    force_rms = float(np.sqrt(((a32 - a64) ** 2).sum() / (a64 ** 2).sum()))
+    pxy64, pxy32 = mop["fp64"][:, 0], mop["mixed"][:, 0]
+    mop_err = float(np.linalg.norm(pxy32 - pxy64) / np.linalg.norm(pxy64))
+    out = {
+        "metric": "atom-steps/s (1M Couette, tethered walls, MOP 64 planes every step, 1 GPU)",
+        "value": res["mixed"]["atom_steps_per_s"],
+        "unit": "atom-steps/s",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "steps": K,
+        "fp64": res["fp64"],
+        "mixed": res["mixed"],
+        "mixed_force_rms_rel_error": force_rms,
+        "mixed_mop_pxy_rel_error": mop_err,
+        "dtype": "f32-pair/f64-state (fp64 beside it)",
+        "data": "synthetic",
+        "vs_baseline": None,
+        "config": {"workload": f"Couette FCC {n_u}^3 (density 0.8, rc 2.5), walls 3 units tethered "
+                               f"(k4 5000, k6 5e6), top wall U = 1.0, leap-frog dt 0.005, TPLOT 1 "
+                               f"(record and rebuild every step), MOP 64 planes"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(p, r, v, steps):
     """The CPU restatement of the reference (oracle, one core) on the same workload."""
     from oracle.oracle import Oracle
@@ -144,6 +258,12 @@ def main():
         copy_calibration(a.copy_calibration_mb)
     if a.workload == "serve":
         serve_main(a)
+        return
+    if a.workload == "drift":
+        drift_main(a)
+        return
+    if a.workload == "c3":
+        c3_main(a)
         return
     import numpy as np
 
