@@ -331,61 +331,89 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
             const double gyl = fmax(yi - ylo - 1e-9, 0.0), gyh = fmax(ylo + a.cs[1] - yi - 1e-9, 0.0);
             const double gzl = fmax(zi - zlo - 1e-9, 0.0), gzh = fmax(zlo + a.cs[2] - zi - 1e-9, 0.0);
             // the 27 neighbour cells as 9 runs of three x-consecutive region cells, which are
-            // contiguous in the LDS layout
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy) {
-                    const int c0 = (rx - 1) + R0 * ((ry + dy) + R1 * (rz + dz));
-                    int jb = r.off[c0], je = r.off[c0 + 3];
-                    if (!POLY && a.xsorted) {
-                        const double gy = dy < 0 ? gyl : (dy > 0 ? gyh : 0.0);
-                        const double gz = dz < 0 ? gzl : (dz > 0 ? gzh : 0.0);
-                        const double w2 = a.rn2 - (gy * gy + gz * gz);
-                        if (!(w2 > 0.0)) continue;
-                        const double xw = (dy == 0 && dz == 0) ? a.xwin : sqrt(w2) * (1.0 + 1e-9) + 1e-12;
-                        // the run is sorted by x: keep only xi - xw <= x <= xi + xw
-                        const double xlo = xi - xw, xhi = xi + xw;
-                        int lo = jb, hi = je;
-                        while (lo < hi) {
-                            const int mid = (lo + hi) >> 1;
-                            if (r.pos[3 * mid] < xlo) lo = mid + 1;
-                            else hi = mid;
-                        }
-                        jb = lo;
-                        lo = jb;
-                        hi = je;
-                        while (lo < hi) {
-                            const int mid = (lo + hi) >> 1;
-                            if (r.pos[3 * mid] <= xhi) lo = mid + 1;
-                            else hi = mid;
-                        }
-                        je = lo;
+            // contiguous in the LDS layout; run k covers dz = k/3 - 1, dy = k%3 - 1
+            auto run_window = [&](int k, int& jb, int& je) {
+                const int dz = k / 3 - 1, dy = k % 3 - 1;
+                const int c0 = (rx - 1) + R0 * ((ry + dy) + R1 * (rz + dz));
+                jb = r.off[c0];
+                je = r.off[c0 + 3];
+                if (POLY || !a.xsorted) return;
+                const double gy = dy < 0 ? gyl : (dy > 0 ? gyh : 0.0);
+                const double gz = dz < 0 ? gzl : (dz > 0 ? gzh : 0.0);
+                const double w2 = a.rn2 - (gy * gy + gz * gz);
+                if (!(w2 > 0.0)) {
+                    je = jb;
+                    return;
+                }
+                const double xw = (dy == 0 && dz == 0) ? a.xwin : sqrt(w2) * (1.0 + 1e-9) + 1e-12;
+                // the run is sorted by x: keep only xi - xw <= x <= xi + xw
+                const double xlo = xi - xw, xhi = xi + xw;
+                int lo = jb, hi = je;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (r.pos[3 * mid] < xlo) lo = mid + 1;
+                    else hi = mid;
+                }
+                jb = lo;
+                lo = jb;
+                hi = je;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (r.pos[3 * mid] <= xhi) lo = mid + 1;
+                    else hi = mid;
+                }
+                je = lo;
+            };
+            if (!POLY) {
+                // All nine windows first, then the runs longest first: the lanes of a wavefront
+                // then walk runs of similar length together (the order of a list's entries is
+                // free; the set is the reference's).  Packed as size:20 | jb:20 | je:20.
+                uint64_t w[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    int jb, je;
+                    run_window(k, jb, je);
+                    w[k] = ((uint64_t)(je - jb) << 40) | ((uint64_t)jb << 20) | (uint64_t)je;
+                }
+#pragma unroll
+                for (int round = 0; round < 9; ++round)  // odd-even transposition sort, descending
+#pragma unroll
+                    for (int k = round & 1; k + 1 < 9; k += 2) {
+                        const uint64_t x = w[k], y = w[k + 1];
+                        w[k] = x > y ? x : y;
+                        w[k + 1] = x > y ? y : x;
                     }
-                    if (!POLY) {
-                        // 32 candidates at a time: a hit mask first (distance tests only), then the
-                        // hits in candidate order into the list words
-                        for (int j0 = jb; j0 < je; j0 += 32) {
-                            const int nn = min(32, je - j0);
-                            uint32_t m = 0;
-                            for (int u = 0; u < nn; ++u) {
-                                const int j = j0 + u;
-                                const double ddx = xi - r.pos[3 * j], ddy = yi - r.pos[3 * j + 1],
-                                             ddz = zi - r.pos[3 * j + 2];
-                                const double r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
-                                m |= (r2 < a.rn2) ? (1u << u) : 0u;
-                            }
-                            if (li >= j0 && li < j0 + nn) m &= ~(1u << (li - j0));
-                            // entries enter the word from the top: after four, the first is in
-                            // bits 0-15
-                            while (m) {
-                                const int j = j0 + __builtin_ctz(m);
-                                m &= m - 1;
-                                word = (word >> 16) | ((uint64_t)(uint32_t)j << 48);
-                                if ((cnt & 3) == 3 && cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word;
-                                ++cnt;
-                            }
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    const int jb = (int)((w[k] >> 20) & 0xfffffu), je = (int)(w[k] & 0xfffffu);
+                    // 32 candidates at a time: a hit mask first (distance tests only), then the
+                    // hits in candidate order into the list words
+                    for (int j0 = jb; j0 < je; j0 += 32) {
+                        const int nn = min(32, je - j0);
+                        uint32_t m = 0;
+                        for (int u = 0; u < nn; ++u) {
+                            const int j = j0 + u;
+                            const double ddx = xi - r.pos[3 * j], ddy = yi - r.pos[3 * j + 1],
+                                         ddz = zi - r.pos[3 * j + 2];
+                            const double r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                            m |= (r2 < a.rn2) ? (1u << u) : 0u;
                         }
-                        continue;
+                        if (li >= j0 && li < j0 + nn) m &= ~(1u << (li - j0));
+                        // entries enter the word from the top: after four, the first is in
+                        // bits 0-15
+                        while (m) {
+                            const int j = j0 + __builtin_ctz(m);
+                            m &= m - 1;
+                            word = (word >> 16) | ((uint64_t)(uint32_t)j << 48);
+                            if ((cnt & 3) == 3 && cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word;
+                            ++cnt;
+                        }
                     }
+                }
+            } else {
+                for (int k = 0; k < 9; ++k) {
+                    int jb, je;
+                    run_window(k, jb, je);
                     for (int j = jb; j < je; ++j) {
                         const double ddx = xi - r.pos[3 * j], ddy = yi - r.pos[3 * j + 1], ddz = zi - r.pos[3 * j + 2];
                         const double r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
@@ -405,6 +433,7 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                         }
                     }
                 }
+            }
             if ((cnt & 3) != 0 && cnt < a.maxnb) {
                 // the last, partial word: the non-polymer path fills words from the top
                 const uint64_t last = POLY ? word : word >> (16 * (4 - (cnt & 3)));
