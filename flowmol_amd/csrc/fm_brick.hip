@@ -657,7 +657,8 @@ __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row,
         lj_stage(pa, pos, kw + 8 < n ? w2 : 0);
         w2 = row[(size_t)min(k + 4, lastw) * cap];
         __builtin_amdgcn_sched_barrier(0);
-        lj_word(pb, n - kw - 4, xi, yi, zi, rc2, sx, sy, sz);
+        // the second word only when a lane of the wavefront has entries in it
+        if (__builtin_amdgcn_ballot_w64(kw + 4 < n) != 0ull) lj_word(pb, n - kw - 4, xi, yi, zi, rc2, sx, sy, sz);
     }
     fx = 24.0 * sx;
     fy = 24.0 * sy;
