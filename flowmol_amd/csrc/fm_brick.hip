@@ -16,6 +16,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace {
@@ -634,9 +636,37 @@ __device__ __forceinline__ void lj_word(const P4& p, int left, double xi, double
         fz = fmaf64(acc, rz, fz);
     }
 }
+// Mixed precision: the FP64 coordinate difference converted to FP32, then the same law in FP32
+// (rcp without refinement); an entry that does not count selects 1/r^2 = 0.
+__device__ __forceinline__ void lj_word_mixed(const P4& p, int left, double xi, double yi, double zi, float rc2,
+                                              float& fx, float& fy, float& fz) {
+#pragma clang fp contract(fast)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float rx = (float)(xi - p.x[q]), ry = (float)(yi - p.y[q]), rz = (float)(zi - p.z[q]);
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        float x = __builtin_amdgcn_rcpf(r2);
+        x = (r2 < rc2 && q < left) ? x : 0.0f;
+        const float x2 = x * x;
+        const float acc = (x2 * x2) * (2.0f * (x2 * x) - 1.0f);
+        fx = fx + acc * rx;
+        fy = fy + acc * ry;
+        fz = fz + acc * rz;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void lj_word_t(const P4& p, int left, double xi, double yi, double zi, double rc2, T& fx,
+                                          T& fy, T& fz) {
+    if constexpr (sizeof(T) == 8) lj_word(p, left, xi, yi, zi, rc2, fx, fy, fz);
+    else lj_word_mixed(p, left, xi, yi, zi, (float)rc2, fx, fy, fz);
+}
+
+// MIXED: the pair law and the per-molecule sum in FP32 (lj_word_mixed), the result in FP64
+template <bool MIXED = false>
 __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row, int n, size_t cap, double xi,
                                          double yi, double zi, double rc2, double& fx, double& fy, double& fz) {
-    double sx = 0.0, sy = 0.0, sz = 0.0;
+    using T = typename std::conditional<MIXED, float, double>::type;
+    T sx = 0, sy = 0, sz = 0;
     P4 pa, pb;
     // At the top of each pass pa holds the positions of word k = kw/4, w1 is word k+1 and w2
     // word k+2 (in flight).  Word loads past the end re-read the last word (the row always has
@@ -653,22 +683,22 @@ __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row,
         lj_stage(pb, pos, kw + 4 < n ? w1 : 0);
         w1 = row[(size_t)min(k + 3, lastw) * cap];
         __builtin_amdgcn_sched_barrier(0);  // keep the reads above the evaluation
-        lj_word(pa, n - kw, xi, yi, zi, rc2, sx, sy, sz);
+        lj_word_t(pa, n - kw, xi, yi, zi, rc2, sx, sy, sz);
         lj_stage(pa, pos, kw + 8 < n ? w2 : 0);
         w2 = row[(size_t)min(k + 4, lastw) * cap];
         __builtin_amdgcn_sched_barrier(0);
         // the second word only when a lane of the wavefront has entries in it
-        if (__builtin_amdgcn_ballot_w64(kw + 4 < n) != 0ull) lj_word(pb, n - kw - 4, xi, yi, zi, rc2, sx, sy, sz);
+        if (__builtin_amdgcn_ballot_w64(kw + 4 < n) != 0ull) lj_word_t(pb, n - kw - 4, xi, yi, zi, rc2, sx, sy, sz);
     }
-    fx = 24.0 * sx;
-    fy = 24.0 * sy;
-    fz = 24.0 * sz;
+    fx = 24.0 * (double)sx;
+    fy = 24.0 * (double)sy;
+    fz = 24.0 * (double)sz;
 }
 
 // VIR: virial on a non-record step (teval); record steps always tally it.  P1: VV pass 2 of this
 // step, pass 1 of the next one and its displacement update follow the force (same arithmetic as
 // the separate kernels), written to the other state buffer.
-template <bool MIE, bool RECORD, bool EXTRA, bool VIR = RECORD, bool P1 = false>
+template <bool MIE, bool RECORD, bool EXTRA, bool VIR = RECORD, bool P1 = false, bool MIXED = false>
 __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* smem) {
     Region r;
     const bool mop = RECORD && a.pxy != nullptr;
@@ -702,7 +732,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         }
         const int exti = (mop || (RECORD && a.cv)) ? a.ext[i] : 0;
         const uint64_t* row = a.nbl + i;
-        if constexpr (!MIE && !RECORD && !VIR) lj_plain(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz);
+        if constexpr (!MIE && !RECORD && !VIR) lj_plain<MIXED>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz);
         // Four 16-bit entries per word.  The next word is loaded while this one is worked on,
         // and the four entries' positions are read from LDS together (the unused entries of a
         // last, partial word are 0, a valid region index).
@@ -908,13 +938,13 @@ k_forces_brick_lj(FbArgs a) {
 
 // LJ steps without tallies, bonds or tethers (the common step): registers held to what lets two
 // workgroups share a CU (the LDS of two regions fits its 160 KB)
-// (P1: with pass 1 of the next step)
-template <bool P1>
+// (P1: with pass 1 of the next step; MIXED: the FP32 pair law)
+template <bool P1, bool MIXED>
 __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128, FB_BLOCK / 128))) void
 k_forces_brick_plain(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt)) return;
-    forces_body<false, false, false, false, P1>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+    forces_body<false, false, false, false, P1, MIXED>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // packed launch over many domains: block b works on brick table[b].y of domain table[b].x
@@ -1273,7 +1303,7 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
     if (int rc = make_fargs(c, iter, kick, &a, &lds, &kind)) return rc;
     if (cond) a.halt = c->d_flags;
     const bool mixed = kind & 1, mie = kind & 2, extra = kind & 4;
-    if (next >= 0 && kind == 0 && kick && !a.rec && !a.ev && fused) {
+    if (next >= 0 && (kind == 0 || kind == 1) && kick && !a.rec && !a.ev && fused) {
         // this step's force and kick, then pass 1 of step `next` into the other state buffer
         a.s2 = state_of(c->st[1 - c->scur], c->cap);
         a.rd = c->rdisp;
@@ -1282,7 +1312,8 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         a.fresh = c->rdisp_valid ? 0 : 1;
         a.p1step = next;
         ktimer_begin(c, KT_FORCES_P1);
-        k_forces_brick_plain<true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
+        if (mixed) k_forces_brick_plain<true, true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
+        else k_forces_brick_plain<true, false><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
         ktimer_end(c, KT_FORCES_P1);
         LAUNCH_CHECK();
         c->scur = 1 - c->scur;
@@ -1302,7 +1333,9 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
                : k_forces_brick_mixed<false, E, false><<<grid, blk, lds, c->stream>>>(a)))
     if (mixed) {
         if (extra) { if (a.rec) FM(true, true); else FM(false, true); }
-        else { if (a.rec) FM(true, false); else FM(false, false); }
+        else if (a.rec) FM(true, false);
+        else if (a.ev) FM(false, false);
+        else k_forces_brick_plain<false, true><<<grid, blk, lds, c->stream>>>(a);
     } else if (mie) {
         if (extra) { if (a.rec) FB(true, true, true); else FB(true, false, true); }
         else { if (a.rec) FB(true, true, false); else FB(true, false, false); }
@@ -1310,7 +1343,7 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         if (extra) { if (a.rec) FB(false, true, true); else FB(false, false, true); }
         else if (a.rec) k_forces_brick_lj<true, true><<<grid, blk, lds, c->stream>>>(a);
         else if (a.ev) k_forces_brick_lj<false, true><<<grid, blk, lds, c->stream>>>(a);
-        else k_forces_brick_plain<false><<<grid, blk, lds, c->stream>>>(a);
+        else k_forces_brick_plain<false, false><<<grid, blk, lds, c->stream>>>(a);
     }
 #undef FB
 #undef FM

@@ -353,8 +353,32 @@ def test_mixed_precision_error_vs_fp64():
     pe64, _ = g64.molecular()
     pe32, _ = g32.molecular()
     assert abs(pe32.sum() - pe64.sum()) < 1e-5 * abs(pe64.sum())
+    # a step without tallies takes the branch-free FP32 loop
+    g64.simulation_compute_forces(3)
+    g32.simulation_compute_forces(3)
+    a64, a32 = g64.state()[2], g32.state()[2]
+    assert np.sqrt(((a32 - a64) ** 2).sum() / (a64 ** 2).sum()) < 1e-5
     g64.close()
     g32.close()
+
+
+def test_mixed_precision_fused_loop_equals_routine_sequence():
+    """The fused loop's mixed-precision force launches (with the next step's pass 1) repeat the
+    step-by-step sequence bit for bit."""
+    pm, r, v = config.config_fcc((6, 6, 6), 0.8, 2.5, seed=27, tplot=10, precision=FM_PREC_MIXED)
+    r = r + np.random.default_rng(27).uniform(-1e-3, 1e-3, r.shape)
+    say("case: mixed precision fused loop vs routine sequence, 864 LJ, 60 steps")
+    g = MDContext(pm, r, v)
+    f = MDContext(pm, r, v)
+    g.setup(0)
+    f.setup(0)
+    rg = [it for it in range(1, 61) if g.md_advance_vv(it)]
+    _, nrb = f.run_vv(1, 60)
+    assert nrb == len(rg) and len(rg) > 2
+    for x, y in zip(f.state(), g.state()):
+        assert np.array_equal(x, y)
+    f.close()
+    g.close()
 
 
 # 2-cell blocks on 8 lattice units: 4 cells a side, so the 2 blocks a side split the box evenly.
