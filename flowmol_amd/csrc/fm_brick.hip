@@ -636,6 +636,32 @@ __device__ __forceinline__ void lj_word(const P4& p, int left, double xi, double
         fz = fmaf64(acc, rz, fz);
     }
 }
+// The same loop on a record step with only PE and virial tallied: 1/r^2 from two Newton steps (the
+// IEEE quotient), the PE term in the reference's expansion 4(x^6 - x^3) - potshift (an entry that
+// does not count adds exactly zero), and the virial as x^4 (2x^3 - 1) r^2, scaled by 24 at the end.
+__device__ __forceinline__ void lj_word_rec(const P4& p, int left, double xi, double yi, double zi, double rc2,
+                                            double potshift, double& fx, double& fy, double& fz, double& pe,
+                                            double& vt) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double rx = xi - p.x[q], ry = yi - p.y[q], rz = zi - p.z[q];
+        const double r2 = (rx * rx + ry * ry) + rz * rz;
+        const bool in = r2 < rc2 && q < left;
+        double x = recip(r2);
+        x = in ? x : __hiloint2double(0, __double2loint(x));
+        const double x2 = x * x;
+        const double x3 = x * x2;
+        const double x4 = x2 * x2;
+        const double t = x4 * fmaf64(2.0, x3, -1.0);
+        fx = fmaf64(t, rx, fx);
+        fy = fmaf64(t, ry, fy);
+        fz = fmaf64(t, rz, fz);
+        vt = fmaf64(t, r2, vt);
+        const double e = 4.0 * (x3 * x3 - x3) - potshift;
+        pe = pe + (in ? e : 0.0);
+    }
+}
+
 // Mixed precision: the FP64 coordinate difference converted to FP32, then the same law in FP32
 // (rcp without refinement); an entry that does not count selects 1/r^2 = 0.
 __device__ __forceinline__ void lj_word_mixed(const P4& p, int left, double xi, double yi, double zi, float rc2,
@@ -661,12 +687,15 @@ __device__ __forceinline__ void lj_word_t(const P4& p, int left, double xi, doub
     else lj_word_mixed(p, left, xi, yi, zi, (float)rc2, fx, fy, fz);
 }
 
-// MIXED: the pair law and the per-molecule sum in FP32 (lj_word_mixed), the result in FP64
-template <bool MIXED = false>
+// MIXED: the pair law and the per-molecule sum in FP32 (lj_word_mixed), the result in FP64.
+// REC (FP64 only): also the molecule's PE and virial (lj_word_rec).
+template <bool MIXED = false, bool REC = false>
 __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row, int n, size_t cap, double xi,
-                                         double yi, double zi, double rc2, double& fx, double& fy, double& fz) {
+                                         double yi, double zi, double rc2, double& fx, double& fy, double& fz,
+                                         double potshift = 0.0, double* pe_out = nullptr, double* vir_out = nullptr) {
     using T = typename std::conditional<MIXED, float, double>::type;
     T sx = 0, sy = 0, sz = 0;
+    double pe = 0.0, vt = 0.0;
     P4 pa, pb;
     // At the top of each pass pa holds the positions of word k = kw/4, w1 is word k+1 and w2
     // word k+2 (in flight).  Word loads past the end re-read the last word (the row always has
@@ -683,16 +712,24 @@ __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row,
         lj_stage(pb, pos, kw + 4 < n ? w1 : 0);
         w1 = row[(size_t)min(k + 3, lastw) * cap];
         __builtin_amdgcn_sched_barrier(0);  // keep the reads above the evaluation
-        lj_word_t(pa, n - kw, xi, yi, zi, rc2, sx, sy, sz);
+        if constexpr (REC) lj_word_rec(pa, n - kw, xi, yi, zi, rc2, potshift, sx, sy, sz, pe, vt);
+        else lj_word_t(pa, n - kw, xi, yi, zi, rc2, sx, sy, sz);
         lj_stage(pa, pos, kw + 8 < n ? w2 : 0);
         w2 = row[(size_t)min(k + 4, lastw) * cap];
         __builtin_amdgcn_sched_barrier(0);
         // the second word only when a lane of the wavefront has entries in it
-        if (__builtin_amdgcn_ballot_w64(kw + 4 < n) != 0ull) lj_word_t(pb, n - kw - 4, xi, yi, zi, rc2, sx, sy, sz);
+        if (__builtin_amdgcn_ballot_w64(kw + 4 < n) != 0ull) {
+            if constexpr (REC) lj_word_rec(pb, n - kw - 4, xi, yi, zi, rc2, potshift, sx, sy, sz, pe, vt);
+            else lj_word_t(pb, n - kw - 4, xi, yi, zi, rc2, sx, sy, sz);
+        }
     }
     fx = 24.0 * (double)sx;
     fy = 24.0 * (double)sy;
     fz = 24.0 * (double)sz;
+    if constexpr (REC) {
+        *pe_out = pe;
+        *vir_out = 24.0 * vt;
+    }
 }
 
 // VIR: virial on a non-record step (teval); record steps always tally it.  P1: VV pass 2 of this
@@ -714,6 +751,8 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
     }
     const size_t cap = a.cap;
     const int T2 = FM_MAX_TYPES * FM_MAX_TYPES;
+    // a record step that tallies PE and virial only (no MOP, CV or rfmol)
+    const bool simple_rec = RECORD && !mop && !cvrec && a.rf == nullptr;
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
         int rci, li, i;
         imol(r, a.start, q, rci, li, i);
@@ -732,12 +771,21 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         }
         const int exti = (mop || (RECORD && a.cv)) ? a.ext[i] : 0;
         const uint64_t* row = a.nbl + i;
-        if constexpr (!MIE && !RECORD && !VIR) lj_plain<MIXED>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz);
+        bool looped = false;  // the pairs went through the branch-free loop
+        if constexpr (!MIE && !RECORD && !VIR) {
+            lj_plain<MIXED>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz);
+            looped = true;
+        } else if constexpr (!MIE && RECORD && VIR && !EXTRA && !MIXED) {
+            if (simple_rec) {  // PE and virial only
+                lj_plain<false, true>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz, a.potshift, &pe, &vir);
+                looped = true;
+            }
+        }
         // Four 16-bit entries per word.  The next word is loaded while this one is worked on,
         // and the four entries' positions are read from LDS together (the unused entries of a
         // last, partial word are 0, a valid region index).
         uint64_t wnext = n > 0 ? row[0] : 0;
-        for (int kw = 0; (MIE || RECORD || VIR) && kw < n; kw += 4) {
+        for (int kw = 0; !looped && kw < n; kw += 4) {
             const uint64_t word = wnext;
             if (kw + 4 < n) wnext = row[(size_t)((kw >> 2) + 1) * cap];
             int jq[4];
@@ -885,6 +933,8 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                 atomicOr(&a.flags[FLAG_REBUILD], 1);
                 atomicExch(&a.flags[FLAG_RB_STEP], a.p1step);
             }
+            if (VIR) a.vir[i] = vir;
+            if (RECORD) a.pe_lj[i] = pe;
             continue;
         }
         a.s.ax[i] = fx;
@@ -938,13 +988,14 @@ k_forces_brick_lj(FbArgs a) {
 
 // LJ steps without tallies, bonds or tethers (the common step): registers held to what lets two
 // workgroups share a CU (the LDS of two regions fits its 160 KB)
-// (P1: with pass 1 of the next step; MIXED: the FP32 pair law)
-template <bool P1, bool MIXED>
+// (P1: with pass 1 of the next step; MIXED: the FP32 pair law; REC: a record step tallying PE and
+// virial only)
+template <bool P1, bool MIXED, bool REC = false>
 __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128, FB_BLOCK / 128))) void
 k_forces_brick_plain(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt)) return;
-    forces_body<false, false, false, false, P1, MIXED>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+    forces_body<false, REC, false, REC, P1, MIXED>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // packed launch over many domains: block b works on brick table[b].y of domain table[b].x
@@ -1303,7 +1354,9 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
     if (int rc = make_fargs(c, iter, kick, &a, &lds, &kind)) return rc;
     if (cond) a.halt = c->d_flags;
     const bool mixed = kind & 1, mie = kind & 2, extra = kind & 4;
-    if (next >= 0 && (kind == 0 || kind == 1) && kick && !a.rec && !a.ev && fused) {
+    // a record step tallying PE and virial only takes the branch-free loop too (FP64)
+    const bool simple_rec = kind == 0 && a.rec && a.pxy == nullptr && a.cv == nullptr && a.rf == nullptr;
+    if (next >= 0 && (((kind == 0 || kind == 1) && !a.rec && !a.ev) || simple_rec) && kick && fused) {
         // this step's force and kick, then pass 1 of step `next` into the other state buffer
         a.s2 = state_of(c->st[1 - c->scur], c->cap);
         a.rd = c->rdisp;
@@ -1312,7 +1365,8 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         a.fresh = c->rdisp_valid ? 0 : 1;
         a.p1step = next;
         ktimer_begin(c, KT_FORCES_P1);
-        if (mixed) k_forces_brick_plain<true, true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
+        if (simple_rec) k_forces_brick_plain<true, false, true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
+        else if (mixed) k_forces_brick_plain<true, true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
         else k_forces_brick_plain<true, false><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
         ktimer_end(c, KT_FORCES_P1);
         LAUNCH_CHECK();
@@ -1341,6 +1395,7 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         else { if (a.rec) FB(true, true, false); else FB(true, false, false); }
     } else {
         if (extra) { if (a.rec) FB(false, true, true); else FB(false, false, true); }
+        else if (simple_rec) k_forces_brick_plain<false, false, true><<<grid, blk, lds, c->stream>>>(a);
         else if (a.rec) k_forces_brick_lj<true, true><<<grid, blk, lds, c->stream>>>(a);
         else if (a.ev) k_forces_brick_lj<false, true><<<grid, blk, lds, c->stream>>>(a);
         else k_forces_brick_plain<false, false><<<grid, blk, lds, c->stream>>>(a);
