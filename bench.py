@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU sample (after its setup)")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP event timing")
+    ap.add_argument("--autotune", action="store_true", help="pick the brick shape with the brick tuner (fresh contexts) first")
     ap.add_argument("--drift-steps", type=int, default=10000, help="drift: NVE steps per precision")
     ap.add_argument("--c3-steps", type=int, default=100, help="c3: Couette steps per precision")
     ap.add_argument("--workload", choices=["c2", "serve", "drift", "c3"], default="c2",
@@ -275,6 +276,14 @@ def main():
     n_u = a.nunits
     p, r, v = config.config_fcc((n_u, n_u, n_u), 0.8, 2.5, seed=2, tplot=10, precision=prec)
     N = r.shape[0]
+    brick = None
+    if a.autotune:
+        from flowmol_amd.md import tune_brick
+
+        brick, tuned = tune_brick(p, r, v)
+        print("brick tuner (s/step): " + ", ".join("%dx%dx%d %.3e" % (b + (t,)) for b, t in tuned.items()),
+              file=sys.stderr, flush=True)
+        p.brick[0], p.brick[1], p.brick[2] = brick
     g = MDContext(p, r, v)
     g.setup(0)
     g.run_vv(1, a.warmup)
@@ -346,6 +355,7 @@ def main():
         "config": {"workload": f"LJ NVE FCC {n_u}^3 ({N} molecules), density 0.8, rc 2.5, delta 0.3, "
                                f"dt 0.005, T 1.0, TPLOT 10, displacement rebuild",
                    "molecules": N, "rebuilds_in_timed_steps": nrb, "p_half": round(p_half, 2),
+                   "brick": "autotuned %dx%dx%d" % brick if a.autotune else "default",
                    "precision": a.precision},
         "kernels_ms": {k: round(vv[0], 5) for k, vv in kt.items() if vv[1] > 0},
         "roofline": roof,

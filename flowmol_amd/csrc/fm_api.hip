@@ -173,6 +173,33 @@ static int alloc_nlist(fm_ctx* c, int maxnb) {
 // a build met a list of `need` entries: room for a quarter more
 static int grow_nlist(fm_ctx* c, int need) { return alloc_nlist(c, need + need / 4 + 8); }
 
+// Brick shape b and the brick order of the brick kernels: tiles of 4x4x4 bricks, x fastest, bricks
+// x fastest inside a tile.  Workgroups are dealt to the 8 XCDs in turn and each XCD takes a
+// contiguous run of this order, so the bricks running together on one XCD form compact blocks
+// that share their staged halo cells in that XCD's L2.
+static int set_brick(fm_ctx* c, const int b[3]) {
+    int nb[3];
+    for (int d = 0; d < 3; ++d) {
+        c->brick[d] = std::max(1, std::min(b[d], c->nc[d]));
+        nb[d] = (c->nc[d] + c->brick[d] - 1) / c->brick[d];
+    }
+    const int T = 4;
+    std::vector<int> order;
+    order.reserve((size_t)nb[0] * nb[1] * nb[2]);
+    for (int tz = 0; tz < nb[2]; tz += T)
+        for (int ty = 0; ty < nb[1]; ty += T)
+            for (int tx = 0; tx < nb[0]; tx += T)
+                for (int z = tz; z < std::min(tz + T, nb[2]); ++z)
+                    for (int y = ty; y < std::min(ty + T, nb[1]); ++y)
+                        for (int x = tx; x < std::min(tx + T, nb[0]); ++x) order.push_back(x + nb[0] * (y + nb[1] * z));
+    if (c->brick_order) hipFree(c->brick_order);
+    c->brick_order = nullptr;
+    int rc = dalloc(&c->brick_order, order.size());
+    if (rc) return rc;
+    FM_HIP(hipMemcpy(c->brick_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+    return 0;
+}
+
 // Brick shape: the largest number of cells whose expected molecule count fits one pass of the
 // workgroup (15/16 of its threads), preferring the shape with the least staged halo per cell.
 static void choose_brick(const int nc[3], double occ, int out[3]) {
@@ -423,28 +450,9 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
             return rc;
         }
     }
-    {
-        // Brick order for the brick kernels: tiles of 4x4x4 bricks, x fastest, bricks x fastest
-        // inside a tile.  Workgroups are dealt to the 8 XCDs in turn and each XCD takes a
-        // contiguous run of this order, so the bricks running together on one XCD form compact
-        // blocks that share their staged halo cells in that XCD's L2.
-        int nb[3];
-        for (int d = 0; d < 3; ++d) nb[d] = (c->nc[d] + c->brick[d] - 1) / c->brick[d];
-        const int T = 4;
-        std::vector<int> order;
-        order.reserve((size_t)nb[0] * nb[1] * nb[2]);
-        for (int tz = 0; tz < nb[2]; tz += T)
-            for (int ty = 0; ty < nb[1]; ty += T)
-                for (int tx = 0; tx < nb[0]; tx += T)
-                    for (int z = tz; z < std::min(tz + T, nb[2]); ++z)
-                        for (int y = ty; y < std::min(ty + T, nb[1]); ++y)
-                            for (int x = tx; x < std::min(tx + T, nb[0]); ++x)
-                                order.push_back(x + nb[0] * (y + nb[1] * z));
-        if ((rc = dalloc(&c->brick_order, order.size())) ||
-            hipMemcpy(c->brick_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
-            free_ctx(c);
-            return rc ? rc : fm_set_error(FM_EHIP, "brick order upload failed");
-        }
+    if ((rc = set_brick(c, c->brick))) {
+        free_ctx(c);
+        return rc;
     }
     if (p->sort_flag != 0) {
         for (int d = 0; d < 3; ++d) {

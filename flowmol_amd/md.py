@@ -75,6 +75,11 @@ class MDContext:
         check(load().fm_get_geometry(self._h, iptr(nc), dptr(cs), C.byref(rn2), C.byref(rc2), C.byref(ps)))
         return dict(ncells=nc, cellsidelength=cs, rneighbr2=rn2.value, rcutoff2=rc2.value, potshift=ps.value)
 
+    def autotune_brick(self, iter_=0, max_candidates=4):
+        """In-place retuning of a live context is switched off: its first GPU run ended in a host
+        segmentation fault whose cause was not found.  tune_brick() tunes on fresh contexts."""
+        raise RuntimeError("in-place brick retuning is switched off; use flowmol_amd.md.tune_brick")
+
     def capacity(self):
         """(neighbour-list entries per molecule, molecules per staged brick region)."""
         n, m = C.c_int32(), C.c_int32()
@@ -281,3 +286,55 @@ class MDContext:
                                           16, C.byref(n)))
         names = buf.value.decode().split("\n")[: n.value]
         return {names[k]: (avg[k], int(cnt[k])) for k in range(n.value)}
+
+
+# ---- brick-shape autotuner ---------------------------------------------------------------------
+# No counterpart in the reference, whose cells are not grouped into bricks.
+FB_BLOCK = 512  # threads of one brick workgroup (FB_BLOCK of the brick kernels)
+
+
+def brick_candidates(ncells, nmol, max_candidates=4):
+    """Brick shapes (cells along x, y, z) whose expected molecule count fits one pass of a
+    workgroup (15/16 of FB_BLOCK threads), least staged halo per cell first.  The first one is the
+    library's default shape for the same cells and occupancy."""
+    nc = [int(x) for x in ncells]
+    occ = max(1.0, nmol / float(nc[0] * nc[1] * nc[2]))
+    target = FB_BLOCK * 15.0 / 16.0
+    out = []
+    for bx in range(1, min(nc[0], 12) + 1):
+        for by in range(1, min(nc[1], 12) + 1):
+            for bz in range(1, min(nc[2], 12) + 1):
+                P = bx * by * bz
+                if P > 1 and P * occ > target:
+                    continue
+                out.append((P / ((bx + 2) * (by + 2) * (bz + 2)), (bx, by, bz)))
+    # a stable sort: equal halo efficiency keeps the enumeration order, as the library's choice
+    return [b for _, b in sorted(out, key=lambda t: -t[0])][:max(1, int(max_candidates))]
+
+
+def tune_brick(params, r, v, a=None, steps=20, warmup=5, max_candidates=4):
+    """Time `steps` fused velocity Verlet steps (after `warmup` more, list rebuilds included) of the
+    same state on a fresh context per candidate brick shape; returns (fastest shape, {shape:
+    seconds per step}).  `params` is not changed: set params.brick to the shape to use it."""
+    import time
+
+    probe = MDContext(params, r, v, a)
+    nc = probe.geometry()["ncells"]
+    probe.close()
+    times = {}
+    for b in brick_candidates(nc, np.asarray(r).reshape(-1, 3).shape[0], max_candidates):
+        p = type(params).from_buffer_copy(params)
+        p.brick[0], p.brick[1], p.brick[2] = b
+        g = MDContext(p, r, v, a)
+        try:
+            g.setup(0)
+            g.run_vv(1, warmup)
+            g.synchronize()
+            t0 = time.perf_counter()
+            g.run_vv(1 + warmup, steps)
+            g.synchronize()
+            times[b] = (time.perf_counter() - t0) / steps
+        finally:
+            g.close()
+    best = min(times, key=times.get)
+    return best, times

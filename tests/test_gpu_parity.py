@@ -218,3 +218,65 @@ def test_brick_shapes_same_sets_and_forces(brick):
         assert rel_force_err(g.state()[2], o.state()[2], o.force_scale()) < FORCE_TOL
     g.close()
 
+
+@pytest.mark.skip(reason="in-place brick retuning of a live context (MDContext.autotune_brick) ended in a "
+                  "host segmentation fault on its first GPU run and no cause was found by reading the code; "
+                  "it is switched off and shapes are tuned on fresh contexts (tune_brick)")
+def test_autotuned_brick_keeps_sets_and_forces():
+    """The brick-shape autotuner only changes how cells are grouped: the neighbour set, the forces
+    and the trajectory stay the reference's."""
+    p, r, v, a = _lj_liquid((8, 8, 8), seed=13, steps=100, tplot=10)
+    say("case: autotuned brick, 2048 LJ")
+    o, g = _pair(p, r, v, a)
+    o.setup()
+    g.setup(0)
+    shape = g.autotune_brick(0)
+    assert all(s >= 1 for s in shape)
+    oi, oj, os_ = o.neighbourlist()
+    gi, gj, gs = g.neighbourlist()
+    assert np.array_equal(canonical_pairs(oi, oj, os_), canonical_pairs(gi, gj, gs))
+    o.compute_forces(0)
+    assert rel_force_err(g.state()[2], o.state()[2], o.force_scale()) < FORCE_TOL
+    for it in range(1, 31):
+        o.move_particles_vv(1)
+        o.updateborders(0)
+        if o.checkrebuild(it):
+            o.rebuild()
+        o.compute_forces(it)
+        o.move_particles_vv(2)
+        g.md_advance_vv(it)
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-10
+    g.close()
+
+
+def test_tuned_brick_keeps_sets_and_forces():
+    """The brick-shape tuner times candidate shapes on fresh contexts; the shape it returns gives
+    the reference's neighbour set, forces and trajectory."""
+    from flowmol_amd.md import brick_candidates, tune_brick
+
+    p, r, v, a = _lj_liquid((8, 8, 8), seed=13, steps=100, tplot=10)
+    say("case: tuned brick, 2048 LJ")
+    shape, times = tune_brick(p, r, v, a, steps=10, warmup=2, max_candidates=3)
+    assert len(times) == 3 and shape in times and all(t > 0 for t in times.values())
+    assert tuple(p.brick) == (0, 0, 0)  # the caller's parameters are left alone
+    p.brick[0], p.brick[1], p.brick[2] = shape
+    say(f"case: tuned brick {shape}")
+    o, g = _pair(p, r, v, a)
+    o.setup()
+    g.setup(0)
+    assert shape in brick_candidates(g.geometry()["ncells"], r.shape[0], 3)
+    oi, oj, os_ = o.neighbourlist()
+    gi, gj, gs = g.neighbourlist()
+    assert np.array_equal(canonical_pairs(oi, oj, os_), canonical_pairs(gi, gj, gs))
+    o.compute_forces(0)
+    assert rel_force_err(g.state()[2], o.state()[2], o.force_scale()) < FORCE_TOL
+    for it in range(1, 31):
+        o.move_particles_vv(1)
+        o.updateborders(0)
+        if o.checkrebuild(it):
+            o.rebuild()
+        o.compute_forces(it)
+        o.move_particles_vv(2)
+        g.md_advance_vv(it)
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-10
+    g.close()
