@@ -144,11 +144,11 @@ __device__ inline void wave0_scan(int* off, int n) {
     }
 }
 
-// Region metadata + interior-cell list + staged positions.  Ends with __syncthreads().  A region
-// of more than capm molecules is not staged: the workgroup raises NEED_REGION (and the size in
-// FLAG_MAX_REGION) and FLAG_HALT, and the call returns false on every thread.
-__device__ bool stage_region(Region& r, const BG& g, int brick, const int* start, const St& s, bool stage_pos,
-                             const int* mono_src, int cap, int* lchain, int* lsub, int capm, int* flags) {
+// The region tables of a brick: global cell, image shift and molecule offset of each region cell,
+// and the interior-cell list.  A region of more than capm molecules is not staged: the workgroup
+// raises NEED_REGION (and the size in FLAG_MAX_REGION) and FLAG_HALT, and the call returns false
+// on every thread.
+__device__ bool stage_tables(Region& r, const BG& g, int brick, const int* start, int capm, int* flags) {
     const int bx = brick % g.nb[0], by = (brick / g.nb[0]) % g.nb[1], bz = brick / (g.nb[0] * g.nb[1]);
     const int org[3] = {bx * g.B[0], by * g.B[1], bz * g.B[2]};
     for (int t = threadIdx.x; t < g.nreg; t += blockDim.x) {
@@ -201,6 +201,13 @@ __device__ bool stage_region(Region& r, const BG& g, int brick, const int* start
         }
         return false;
     }
+    return true;
+}
+
+// Region tables + staged positions (and slots / chain ids when asked).  Ends with __syncthreads().
+__device__ bool stage_region(Region& r, const BG& g, int brick, const int* start, const St& s, bool stage_pos,
+                             const int* mono_src, int cap, int* lchain, int* lsub, int capm, int* flags) {
+    if (!stage_tables(r, g, brick, start, capm, flags)) return false;
     if (stage_pos || r.lslot || lchain) {
         for (int q = threadIdx.x; q < r.total; q += blockDim.x) {
             int lo = 0, hi = g.nreg - 1;  // last rc with off[rc] <= q
@@ -453,6 +460,213 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
     }
     // a list longer than the capacity: the workgroup's longest into FLAG_MAX_NB, NEED_LIST and
     // FLAG_HALT (the region tables are no longer read, so their first words hold the per-wave values)
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) r.off[threadIdx.x >> 6] = wmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = max(m, r.off[w]);
+        if (m > a.maxnb) {
+            atomicMax(&a.flags[FLAG_MAX_NB], m);
+            atomicOr(&a.flags[FLAG_NEED], NEED_LIST);
+            a.flags[FLAG_HALT] = 1;
+        }
+    }
+}
+
+// ---- neighbour list with a quantised prefilter: cells freshly binned (sorted by x), no bonds ----
+// The same set as k_nlist_brick, from fewer and cheaper operations per candidate.  Positions are
+// staged as 15-bit integers q = round((x - o)/h) on a grid of step h from o, two steps below the
+// region's low corner, so a distance test is two packed 16-bit subtractions and two 16-bit dot
+// products (8 bytes of LDS per candidate instead of 24).  Each component of a quantised
+// difference is within eps = 1 + 1e-6 grid steps of the exact difference over h, so with
+// s = |rij|/h and t = rn/h (Cauchy-Schwarz on the three components):
+//   r2q <  lo = t^2 - 2 sqrt(3) eps t, less margins              =>  s < t: a pair,
+//   r2q >= hi = t^2 + 2 sqrt(3) eps t + 3 eps^2, plus margins    =>  s > t: no pair,
+// and a candidate between the two is decided by the reference's FP64 test
+// (rij2 < rneighbr2, src/linklist.f90:717-720,750-754) on its positions in HBM.  The margins (1e-9 t^2)
+// cover the rounding of the FP64 test itself, so the set is the FP64 set exactly.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+struct NlqArgs {
+    St s;
+    BG g;
+    const int* start;
+    uint64_t* nbl;
+    int* nbn;
+    int* flags;
+    int cap, maxnb, maxreg;
+    double rn2, xwin, h, inv_h;
+    int lo, hi;
+    double cs[3], hd[3];
+    const int* order;
+};
+
+__device__ inline int quantise(double x, double o, double inv_h) {
+    const int q = (int)floor((x - o) * inv_h + 0.5);
+    return min(max(q, 0), 32767);
+}
+
+__global__ __launch_bounds__(FB_BLOCK) void k_nlist_quant(NlqArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    const BG& g = a.g;
+    uint2* qv = (uint2*)smem;               // maxreg: x | y << 16, z
+    int* ls = (int*)(qv + a.maxreg);        // maxreg: slot << 5 | image shift code
+    Region r;
+    r.pos = nullptr;
+    r.lslot = nullptr;
+    int* ip = ls + a.maxreg;
+    r.off = ip;
+    ip += g.nreg + 1;
+    r.gc = ip;
+    ip += g.nreg;
+    r.sh = ip;
+    ip += g.nreg;
+    r.ic_off = ip;
+    ip += g.B[0] * g.B[1] * g.B[2] + 1;
+    r.ic_rc = ip;
+    const int brick = brick_of_block(a.order, blockIdx.x, gridDim.x);
+    if (!stage_tables(r, g, brick, a.start, a.maxreg, a.flags)) return;
+    const int bx = brick % g.nb[0], by = (brick / g.nb[0]) % g.nb[1], bz = brick / (g.nb[0] * g.nb[1]);
+    const double ox = -a.hd[0] + (bx * g.B[0] - 1) * a.cs[0] - 2.0 * a.h;
+    const double oy = -a.hd[1] + (by * g.B[1] - 1) * a.cs[1] - 2.0 * a.h;
+    const double oz = -a.hd[2] + (bz * g.B[2] - 1) * a.cs[2] - 2.0 * a.h;
+    for (int q = threadIdx.x; q < r.total; q += blockDim.x) {
+        const int rc = region_cell_of(r, g.nreg, q);
+        const int slot = a.start[r.gc[rc]] + (q - r.off[rc]);
+        int sx, sy, sz;
+        decode_shift(r.sh[rc], sx, sy, sz);
+        const int xq = quantise(img(a.s.x[slot], sx, g.L[0]), ox, a.inv_h);
+        const int yq = quantise(img(a.s.y[slot], sy, g.L[1]), oy, a.inv_h);
+        const int zq = quantise(img(a.s.z[slot], sz, g.L[2]), oz, a.inv_h);
+        qv[q] = make_uint2((uint32_t)xq | ((uint32_t)yq << 16), (uint32_t)zq);
+        ls[q] = (slot << 5) | r.sh[rc];
+    }
+    __syncthreads();
+    int wmax = 0;
+    const size_t cap = a.cap;
+    const int R0 = g.R[0], R1 = g.R[1];
+    const int nrounds = (r.ni + blockDim.x - 1) / blockDim.x;
+    for (int round = 0; round < nrounds; ++round) {
+        const int qi = round * blockDim.x + threadIdx.x;
+        const bool active = qi < r.ni;
+        int cnt = 0;
+        if (active) {
+            int rc, li, slot;
+            imol(r, a.start, qi, rc, li, slot);
+            const int rx = rc % R0, ry = (rc / R0) % R1, rz = rc / (R0 * R1);
+            // interior cells carry no image shift: the staged position is the stored one
+            const double xi = a.s.x[slot], yi = a.s.y[slot], zi = a.s.z[slot];
+            const uint2 wi = qv[li];
+            const s16x2 ixy = __builtin_bit_cast(s16x2, wi.x);
+            const int izq = (int)wi.y;
+            // faces of the molecule's cell in y and z, as in k_nlist_brick
+            const int vy = by * g.B[1] + ry - 1, vz = bz * g.B[2] + rz - 1;
+            const double ylo = -a.hd[1] + vy * a.cs[1], zlo = -a.hd[2] + vz * a.cs[2];
+            const double gyl = fmax(yi - ylo - 1e-9, 0.0), gyh = fmax(ylo + a.cs[1] - yi - 1e-9, 0.0);
+            const double gzl = fmax(zi - zlo - 1e-9, 0.0), gzh = fmax(zlo + a.cs[2] - zi - 1e-9, 0.0);
+            // nine runs of three x-consecutive region cells; each x window widened by a grid step
+            // on both sides of its quantised bounds, runs longest first
+            uint64_t w[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const int dz = k / 3 - 1, dy = k % 3 - 1;
+                const int c0 = (rx - 1) + R0 * ((ry + dy) + R1 * (rz + dz));
+                int jb = r.off[c0], je = r.off[c0 + 3];
+                const double gy = dy < 0 ? gyl : (dy > 0 ? gyh : 0.0);
+                const double gz = dz < 0 ? gzl : (dz > 0 ? gzh : 0.0);
+                const double w2 = a.rn2 - (gy * gy + gz * gz);
+                if (!(w2 > 0.0)) {
+                    je = jb;
+                } else {
+                    const double xw = (dy == 0 && dz == 0) ? a.xwin : sqrt(w2) * (1.0 + 1e-9) + 1e-12;
+                    const int xl = (int)floor((xi - xw - ox) * a.inv_h) - 1;
+                    const int xh = (int)ceil((xi + xw - ox) * a.inv_h) + 1;
+                    int lo = jb, hi = je;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if ((int)(qv[mid].x & 0xffffu) < xl) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    jb = lo;
+                    hi = je;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if ((int)(qv[mid].x & 0xffffu) <= xh) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    je = lo;
+                }
+                w[k] = ((uint64_t)(je - jb) << 40) | ((uint64_t)jb << 20) | (uint64_t)je;
+            }
+#pragma unroll
+            for (int sr = 0; sr < 9; ++sr)  // odd-even transposition sort, descending
+#pragma unroll
+                for (int k = sr & 1; k + 1 < 9; k += 2) {
+                    const uint64_t x = w[k], y = w[k + 1];
+                    w[k] = x > y ? x : y;
+                    w[k + 1] = x > y ? y : x;
+                }
+            uint64_t word = 0;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const int jb = (int)((w[k] >> 20) & 0xfffffu), je = (int)(w[k] & 0xfffffu);
+                for (int j0 = jb; j0 < je; j0 += 32) {
+                    const int nn = min(32, je - j0);
+                    // each test shifts its two bits in from the bottom (the first candidate ends
+                    // highest); one bit reversal afterwards puts candidate u at bit u
+                    uint32_t m = 0, b = 0;
+                    auto test = [&](int j) {
+                        const uint2 wj = qv[j];
+                        const s16x2 d1 = ixy - __builtin_bit_cast(s16x2, wj.x);
+                        const int dz = izq - (int)wj.y;
+                        const int r2q = __builtin_amdgcn_sdot2(d1, d1, __mul24(dz, dz), false);
+                        m = m + m + (uint32_t)(r2q < a.lo);
+                        b = b + b + (uint32_t)(r2q < a.hi);
+                    };
+                    int u = 0;
+                    for (; u + 4 <= nn; u += 4) {
+                        test(j0 + u);
+                        test(j0 + u + 1);
+                        test(j0 + u + 2);
+                        test(j0 + u + 3);
+                    }
+                    for (; u < nn; ++u) test(j0 + u);
+                    m = __builtin_bitreverse32(m) >> (32 - nn);
+                    b = __builtin_bitreverse32(b) >> (32 - nn);
+                    if (li >= j0 && li < j0 + nn) {
+                        m &= ~(1u << (li - j0));
+                        b &= ~(1u << (li - j0));
+                    }
+                    // the undecided candidates: the FP64 test on the positions in HBM
+                    for (uint32_t u8 = b & ~m; u8; u8 &= u8 - 1) {
+                        const int u = __builtin_ctz(u8);
+                        const int pk = ls[j0 + u];
+                        const int sl = pk >> 5;
+                        int sx, sy, sz;
+                        decode_shift(pk & 31, sx, sy, sz);
+                        const double ddx = xi - img(a.s.x[sl], sx, g.L[0]), ddy = yi - img(a.s.y[sl], sy, g.L[1]),
+                                     ddz = zi - img(a.s.z[sl], sz, g.L[2]);
+                        const double r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                        if (r2 < a.rn2) m |= 1u << u;
+                    }
+                    // entries enter the word from the top: after four, the first is in bits 0-15
+                    while (m) {
+                        const int j = j0 + __builtin_ctz(m);
+                        m &= m - 1;
+                        word = (word >> 16) | ((uint64_t)(uint32_t)j << 48);
+                        if ((cnt & 3) == 3 && cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word;
+                        ++cnt;
+                    }
+                }
+            }
+            if ((cnt & 3) != 0 && cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word >> (16 * (4 - (cnt & 3)));
+            a.nbn[slot] = cnt < a.maxnb ? cnt : a.maxnb;
+        }
+        int m = active ? cnt : 0;
+        for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+        wmax = max(wmax, m);
+    }
     __syncthreads();
     if ((threadIdx.x & 63) == 0) r.off[threadIdx.x >> 6] = wmax;
     __syncthreads();
@@ -1258,6 +1472,42 @@ int launch_nlist(fm_ctx* c) {
     for (int d = 0; d < 3; ++d) {
         a.cs[d] = c->cs[d];
         a.hd[d] = c->half[d];
+    }
+    if (!a.mono && a.xsorted && c->cap < (1 << 26)) {
+        // quantised prefilter (slots packed with the shift code in 31 bits)
+        NlqArgs q;
+        q.s = a.s;
+        q.g = g;
+        q.start = a.start;
+        q.nbl = a.nbl;
+        q.nbn = a.nbn;
+        q.flags = a.flags;
+        q.cap = a.cap;
+        q.maxnb = a.maxnb;
+        q.maxreg = a.maxreg;
+        q.rn2 = a.rn2;
+        q.xwin = a.xwin;
+        q.order = a.order;
+        double ext = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            q.cs[d] = a.cs[d];
+            q.hd[d] = a.hd[d];
+            ext = fmax(ext, (g.B[d] + 2) * c->cs[d]);
+        }
+        // grid step: the region and two steps on each side within 0..32767; the quantised
+        // squared distances of neighbouring-cell candidates then stay below 2^31
+        q.h = ext / 32700.0;
+        q.inv_h = 1.0 / q.h;
+        const double t = sqrt(c->rn2) / q.h, eps = 1.0 + 1e-6, r3 = sqrt(3.0);
+        q.lo = (int)floor(t * t - 2.0 * r3 * eps * t - 1e-9 * t * t) - 2;
+        q.hi = (int)ceil(t * t * (1.0 + 3e-9) + 2.0 * r3 * eps * t * (1.0 + 1e-9) + 3.0 * eps * eps) + 2;
+        const int nbmax = g.B[0] * g.B[1] * g.B[2];
+        const size_t lds = (size_t)c->maxreg * (sizeof(uint2) + sizeof(int)) + (size_t)(3 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int);
+        ktimer_begin(c, KT_NLIST);
+        k_nlist_quant<<<brick_count(c), FB_BLOCK, lds, c->stream>>>(q);
+        LAUNCH_CHECK();
+        ktimer_end(c, KT_NLIST);
+        return 0;
     }
     const size_t lds = brick_lds_bytes(c, c->maxreg, 0);
     ktimer_begin(c, KT_NLIST);

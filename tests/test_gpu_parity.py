@@ -58,6 +58,39 @@ def test_cells_and_neighbourlist_c1(c1_state):
     g.close()
 
 
+@pytest.mark.parametrize("jitter", [0.0, 1e-15, 1e-12])
+def test_neighbourlist_pairs_on_the_threshold(jitter):
+    """A simple cubic lattice whose face diagonals are rneighbr apart (2 a^2 = rneighbr2): twelve
+    pairs per molecule sit on the list threshold, where the FP64 rounding of rij2 (and of the
+    halo images) decides membership.  The list build's quantised prefilter leaves every such pair
+    to the FP64 test, so the set must still be the reference's bit for bit."""
+    rc, delta = 2.5, 0.3
+    rn2 = (rc + delta) ** 2
+    a_lat = np.sqrt(rn2 / 2.0)
+    n = 12
+    L = n * a_lat
+    g1 = (np.arange(n) + 0.5) * a_lat - 0.5 * L
+    r = np.stack(np.meshgrid(g1, g1, g1, indexing="ij"), axis=-1).reshape(-1, 3).copy()
+    r += np.random.default_rng(21).uniform(-jitter, jitter, r.shape) * L
+    v = np.zeros_like(r)
+    p = config.make_params((L, L, L), r.shape[0] / L ** 3, rc, delta=delta, tplot=10)
+    say(f"case: {r.shape[0]} molecules on the list threshold, jitter {jitter}")
+    o, g = _pair(p, r, v)
+    o.assign_to_cell()
+    o.updateborders(1)
+    o.assign_to_neighbourlist()
+    g.assign_to_cell()
+    g.messenger_updateborders(1)
+    g.assign_to_neighbourlist()
+    oi, oj, os_ = o.neighbourlist()
+    gi, gj, gs = g.neighbourlist()
+    # three face pairs per molecule are always in, the six face diagonals per molecule split
+    # by the rounding (all twelve in would give 9N entries and more with the boundary images)
+    assert 3 * r.shape[0] < len(canonical_pairs(oi, oj, os_)) < 9 * r.shape[0]
+    assert np.array_equal(canonical_pairs(oi, oj, os_), canonical_pairs(gi, gj, gs))
+    g.close()
+
+
 def test_forces_c1(c1_state):
     p, r, v, a = c1_state
     say("case: forces C1 2048 WCA")
