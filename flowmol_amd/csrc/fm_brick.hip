@@ -81,6 +81,8 @@ struct Region {
     int* sh;       // nreg: shift code of the region cell
     int* ic_off;   // nic+1: prefix over interior (i-) cells
     int* ic_rc;    // nic: region cell of each i-cell
+    int* cst;      // nreg: first slot of each region cell's global cell
+    uint16_t* rq;  // staged molecule -> region cell
     double* pos;   // staged positions, x y z interleaved (one address, three reads)
     int* lslot;    // optional: slot of each staged molecule
     int nic;
@@ -103,11 +105,15 @@ __device__ inline void carve(Region& r, char* smem, const BG& g, int maxreg, boo
         r.lslot = ip;
         ip += maxreg;
     }
+    r.rq = (uint16_t*)ip;
+    ip += (maxreg + 1) / 2;
     r.off = ip;
     ip += g.nreg + 1;
     r.gc = ip;
     ip += g.nreg;
     r.sh = ip;
+    ip += g.nreg;
+    r.cst = ip;
     ip += g.nreg;
     const int nbmax = g.B[0] * g.B[1] * g.B[2];
     r.ic_off = ip;
@@ -120,7 +126,7 @@ __device__ inline void carve(Region& r, char* smem, const BG& g, int maxreg, boo
 __host__ __device__ inline size_t region_lds_bytes(const BG& g, int maxreg, bool with_slot) {
     const int nbmax = g.B[0] * g.B[1] * g.B[2];
     const size_t b = (size_t)maxreg * (3 * sizeof(double) + (with_slot ? sizeof(int) : 0)) +
-                     (size_t)(3 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int) + 16;
+                     (size_t)((maxreg + 1) / 2 + 4 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int) + 16;
     return (b + 15) & ~(size_t)15;
 }
 
@@ -169,25 +175,38 @@ __device__ bool stage_tables(Region& r, const BG& g, int brick, const int* start
         const int c = gg[0] + g.nc[0] * (gg[1] + g.nc[1] * gg[2]);
         r.gc[t] = c;
         r.sh[t] = shift_code(ss[0], ss[1], ss[2]);
-        r.off[t] = start[c + 1] - start[c];
+        const int s0 = start[c];
+        r.cst[t] = s0;
+        r.off[t] = start[c + 1] - s0;
     }
     __syncthreads();
-    wave0_scan(r.off, g.nreg);
-    if (threadIdx.x == 0) {  // interior cells of this brick (partial bricks at the far faces)
-        int nic = 0;
+    if (threadIdx.x < 64) {
+        wave0_scan(r.off, g.nreg);
+        // interior cells of this brick (partial bricks at the far faces), x fastest, and the
+        // prefix of their molecule counts, by the same wavefront (its LDS accesses stay in order)
+        const int lane = threadIdx.x;
         const int ex = min(g.B[0], g.nc[0] - org[0]), ey = min(g.B[1], g.nc[1] - org[1]),
                   ez = min(g.B[2], g.nc[2] - org[2]);
-        int acc = 0;
-        for (int z = 1; z <= ez; ++z)
-            for (int y = 1; y <= ey; ++y)
-                for (int x = 1; x <= ex; ++x) {
-                    const int rc = x + g.R[0] * (y + g.R[1] * z);
-                    r.ic_rc[nic] = rc;
-                    r.ic_off[nic] = acc;
-                    acc += r.off[rc + 1] - r.off[rc];
-                    ++nic;
-                }
-        r.ic_off[nic] = acc;
+        const int nic = ex * ey * ez;
+        int carry = 0;
+        for (int base = 0; base < nic; base += 64) {
+            const int k = base + lane;
+            int cnt = 0;
+            if (k < nic) {
+                const int x = k % ex + 1, y = (k / ex) % ey + 1, z = k / (ex * ey) + 1;
+                const int rc = x + g.R[0] * (y + g.R[1] * z);
+                r.ic_rc[k] = rc;
+                cnt = r.off[rc + 1] - r.off[rc];
+            }
+            int xs = cnt;
+            for (int d = 1; d < 64; d <<= 1) {
+                const int y2 = __shfl_up(xs, d, 64);
+                if (lane >= d) xs += y2;
+            }
+            if (k < nic) r.ic_off[k] = carry + xs - cnt;
+            carry += __shfl(xs, 63, 64);
+        }
+        if (lane == 0) r.ic_off[nic] = carry;
     }
     __syncthreads();
     r.total = r.off[g.nreg];
@@ -201,6 +220,13 @@ __device__ bool stage_tables(Region& r, const BG& g, int brick, const int* start
         }
         return false;
     }
+    // region cell of every staged molecule, cell by cell
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int c = threadIdx.x >> 6; c < g.nreg; c += nw) {
+        const int b = r.off[c], n = r.off[c + 1] - b;
+        for (int l = lane; l < n; l += 64) r.rq[b + l] = (uint16_t)c;
+    }
+    __syncthreads();
     return true;
 }
 
@@ -209,26 +235,40 @@ __device__ bool stage_region(Region& r, const BG& g, int brick, const int* start
                              const int* mono_src, int cap, int* lchain, int* lsub, int capm, int* flags) {
     if (!stage_tables(r, g, brick, start, capm, flags)) return false;
     if (stage_pos || r.lslot || lchain) {
-        for (int q = threadIdx.x; q < r.total; q += blockDim.x) {
-            int lo = 0, hi = g.nreg - 1;  // last rc with off[rc] <= q
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (r.off[mid] <= q) lo = mid;
-                else hi = mid - 1;
+        // up to SU molecules per thread and pass, all their loads issued before the first store
+        // (out-of-range indices are clamped to the last molecule and not stored)
+        constexpr int SU = 6;
+        for (int q0 = threadIdx.x; q0 < r.total; q0 += SU * blockDim.x) {
+            int sl[SU], sc[SU];
+            double px[SU], py[SU], pz[SU];
+#pragma unroll
+            for (int k = 0; k < SU; ++k) {
+                const int qc = min(q0 + k * (int)blockDim.x, r.total - 1);
+                const int rc = r.rq[qc];
+                sl[k] = r.cst[rc] + (qc - r.off[rc]);
+                sc[k] = r.sh[rc];
+                if (stage_pos) {
+                    px[k] = s.x[sl[k]];
+                    py[k] = s.y[sl[k]];
+                    pz[k] = s.z[sl[k]];
+                }
             }
-            const int rc = lo;
-            const int slot = start[r.gc[rc]] + (q - r.off[rc]);
-            if (stage_pos) {
-                int sx, sy, sz;
-                decode_shift(r.sh[rc], sx, sy, sz);
-                r.pos[3 * q] = img(s.x[slot], sx, g.L[0]);
-                r.pos[3 * q + 1] = img(s.y[slot], sy, g.L[1]);
-                r.pos[3 * q + 2] = img(s.z[slot], sz, g.L[2]);
-            }
-            if (r.lslot) r.lslot[q] = slot;
-            if (lchain) {
-                lchain[q] = mono_src[slot];
-                lsub[q] = mono_src[slot + (size_t)cap];
+#pragma unroll
+            for (int k = 0; k < SU; ++k) {
+                const int q = q0 + k * (int)blockDim.x;
+                if (q >= r.total) break;
+                if (stage_pos) {
+                    int sx, sy, sz;
+                    decode_shift(sc[k], sx, sy, sz);
+                    r.pos[3 * q] = img(px[k], sx, g.L[0]);
+                    r.pos[3 * q + 1] = img(py[k], sy, g.L[1]);
+                    r.pos[3 * q + 2] = img(pz[k], sz, g.L[2]);
+                }
+                if (r.lslot) r.lslot[q] = sl[k];
+                if (lchain) {
+                    lchain[q] = mono_src[sl[k]];
+                    lsub[q] = mono_src[sl[k] + (size_t)cap];
+                }
             }
         }
     }
@@ -247,18 +287,10 @@ __device__ inline void imol(const Region& r, const int* start, int q, int& rc, i
     rc = r.ic_rc[lo];
     const int k = q - r.ic_off[lo];
     li = r.off[rc] + k;
-    slot = start[r.gc[rc]] + k;
+    slot = r.cst[rc] + k;
 }
 
-__device__ inline int region_cell_of(const Region& r, int nreg, int q) {
-    int lo = 0, hi = nreg - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (r.off[mid] <= q) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
+__device__ inline int region_cell_of(const Region& r, int, int q) { return r.rq[q]; }
 
 // Brick of workgroup b of a one-domain launch (gridDim.x == number of bricks).  Workgroups are
 // dealt to the 8 XCDs in turn; XCD x takes the contiguous run x of the tiled brick order.
@@ -516,11 +548,15 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_quant(NlqArgs a) {
     r.pos = nullptr;
     r.lslot = nullptr;
     int* ip = ls + a.maxreg;
+    r.rq = (uint16_t*)ip;
+    ip += (a.maxreg + 1) / 2;
     r.off = ip;
     ip += g.nreg + 1;
     r.gc = ip;
     ip += g.nreg;
     r.sh = ip;
+    ip += g.nreg;
+    r.cst = ip;
     ip += g.nreg;
     r.ic_off = ip;
     ip += g.B[0] * g.B[1] * g.B[2] + 1;
@@ -531,16 +567,32 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_quant(NlqArgs a) {
     const double ox = -a.hd[0] + (bx * g.B[0] - 1) * a.cs[0] - 2.0 * a.h;
     const double oy = -a.hd[1] + (by * g.B[1] - 1) * a.cs[1] - 2.0 * a.h;
     const double oz = -a.hd[2] + (bz * g.B[2] - 1) * a.cs[2] - 2.0 * a.h;
-    for (int q = threadIdx.x; q < r.total; q += blockDim.x) {
-        const int rc = region_cell_of(r, g.nreg, q);
-        const int slot = a.start[r.gc[rc]] + (q - r.off[rc]);
-        int sx, sy, sz;
-        decode_shift(r.sh[rc], sx, sy, sz);
-        const int xq = quantise(img(a.s.x[slot], sx, g.L[0]), ox, a.inv_h);
-        const int yq = quantise(img(a.s.y[slot], sy, g.L[1]), oy, a.inv_h);
-        const int zq = quantise(img(a.s.z[slot], sz, g.L[2]), oz, a.inv_h);
-        qv[q] = make_uint2((uint32_t)xq | ((uint32_t)yq << 16), (uint32_t)zq);
-        ls[q] = (slot << 5) | r.sh[rc];
+    constexpr int SU = 6;  // molecules per thread and pass, loads first (as stage_region)
+    for (int q0 = threadIdx.x; q0 < r.total; q0 += SU * blockDim.x) {
+        int sl[SU], sc[SU];
+        double px[SU], py[SU], pz[SU];
+#pragma unroll
+        for (int k = 0; k < SU; ++k) {
+            const int qc = min(q0 + k * (int)blockDim.x, r.total - 1);
+            const int rc = r.rq[qc];
+            sl[k] = r.cst[rc] + (qc - r.off[rc]);
+            sc[k] = r.sh[rc];
+            px[k] = a.s.x[sl[k]];
+            py[k] = a.s.y[sl[k]];
+            pz[k] = a.s.z[sl[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < SU; ++k) {
+            const int q = q0 + k * (int)blockDim.x;
+            if (q >= r.total) break;
+            int sx, sy, sz;
+            decode_shift(sc[k], sx, sy, sz);
+            const int xq = quantise(img(px[k], sx, g.L[0]), ox, a.inv_h);
+            const int yq = quantise(img(py[k], sy, g.L[1]), oy, a.inv_h);
+            const int zq = quantise(img(pz[k], sz, g.L[2]), oz, a.inv_h);
+            qv[q] = make_uint2((uint32_t)xq | ((uint32_t)yq << 16), (uint32_t)zq);
+            ls[q] = (sl[k] << 5) | sc[k];
+        }
     }
     __syncthreads();
     int wmax = 0;
@@ -1502,7 +1554,8 @@ int launch_nlist(fm_ctx* c) {
         q.lo = (int)floor(t * t - 2.0 * r3 * eps * t - 1e-9 * t * t) - 2;
         q.hi = (int)ceil(t * t * (1.0 + 3e-9) + 2.0 * r3 * eps * t * (1.0 + 1e-9) + 3.0 * eps * eps) + 2;
         const int nbmax = g.B[0] * g.B[1] * g.B[2];
-        const size_t lds = (size_t)c->maxreg * (sizeof(uint2) + sizeof(int)) + (size_t)(3 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int);
+        const size_t lds = (size_t)c->maxreg * (sizeof(uint2) + sizeof(int)) +
+                           (size_t)((c->maxreg + 1) / 2 + 4 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int);
         ktimer_begin(c, KT_NLIST);
         k_nlist_quant<<<brick_count(c), FB_BLOCK, lds, c->stream>>>(q);
         LAUNCH_CHECK();
