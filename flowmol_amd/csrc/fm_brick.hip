@@ -1383,8 +1383,10 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
     }
 }
 
+// registers held to what lets two workgroups share a CU, as for k_forces_brick_lj
 template <bool RECORD, bool EXTRA, bool VIR>
-__global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_mixed(FbArgs a) {
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128))) void
+k_forces_brick_mixed(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt)) return;
     forces_body_mixed<RECORD, EXTRA, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
