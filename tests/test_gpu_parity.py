@@ -91,6 +91,33 @@ def test_neighbourlist_pairs_on_the_threshold(jitter):
     g.close()
 
 
+def test_neighbourlist_after_moves_without_binning():
+    """assign_to_neighbourlist on positions moved since the last assign_to_cell: the cells keep
+    their old members (molecules may sit a little outside them, and no longer in x order), so
+    the build takes the FP64 kernel without x windows; the set is still the reference's for the
+    same stale cells, and the forces over it agree."""
+    p, r, v, a = _lj_liquid((6, 6, 6), seed=11, steps=100, tplot=10)
+    say("case: list after three pass-1 moves without binning, 864 LJ")
+    o, g = _pair(p, r, v, a)
+    o.setup()
+    o.compute_forces(0)
+    g.setup(0)
+    for _ in range(3):
+        o.move_particles_vv(1)
+        o.updateborders(0)
+        g.simulation_move_particles_vv(1)
+        g.messenger_updateborders(0)
+    o.assign_to_neighbourlist()
+    g.assign_to_neighbourlist()
+    oi, oj, os_ = o.neighbourlist()
+    gi, gj, gs = g.neighbourlist()
+    assert np.array_equal(canonical_pairs(oi, oj, os_), canonical_pairs(gi, gj, gs))
+    o.compute_forces(3)
+    g.simulation_compute_forces(3)
+    assert rel_force_err(g.state()[2], o.state()[2], o.force_scale()) < FORCE_TOL
+    g.close()
+
+
 def test_forces_c1(c1_state):
     p, r, v, a = c1_state
     say("case: forces C1 2048 WCA")
