@@ -1043,7 +1043,9 @@ static int rebuild_lists_queued(fm_ctx* c) {
     c->cells_valid = c->nlist_valid = false;
     TRY(launch_sendmols(c, reset, 5));
     TRY(fm_sort_mols(c));
-    TRY(launch_bin(c));
+    // every launch queued after this rebuild (the force launch, or nothing until the lists are
+    // rebuilt) recomputes all accelerations before reading them
+    TRY(launch_bin(c, false));
     c->moved_since_bin = false;
     c->cells_valid = true;
     TRY(launch_nlist(c));

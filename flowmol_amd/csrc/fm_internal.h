@@ -182,7 +182,9 @@ int launch_pass2(fm_ctx* c);
 int launch_check_disp(fm_ctx* c);
 int launch_check_vmax(fm_ctx* c);
 int launch_sendmols(fm_ctx* c, const int* reset = nullptr, int nreset = 0);  // also zeroes the listed flags
-int launch_bin(fm_ctx* c);
+// binning (counting sort into cells, x order per cell, permutation of the state); keep_acc false
+// leaves the accelerations out of the permutation (the next force launch overwrites them all)
+int launch_bin(fm_ctx* c, bool keep_acc = true);
 int launch_reset_flags(fm_ctx* c, const int* which, int n);  // zero the listed flag words
 int launch_nlist(fm_ctx* c);
 // cond: do nothing when a rebuild is pending (FLAG_REBUILD or FLAG_HALT), and raise FLAG_HALT.

@@ -580,6 +580,7 @@ struct PermArgs {
     int* slot_of;
     const int* perm;
     int np, cap;
+    int nf;  // state fields moved: 9, or 6 when the accelerations are about to be recomputed
 };
 
 __global__ __launch_bounds__(FM_BLOCK) void k_permute(PermArgs a) {
@@ -588,7 +589,8 @@ __global__ __launch_bounds__(FM_BLOCK) void k_permute(PermArgs a) {
     int o = a.perm[k];
     const size_t cap = a.cap;
 #pragma unroll
-    for (int f = 0; f < 9; ++f) a.dst[k + f * cap] = a.src[o + f * cap];
+    for (int f = 0; f < 9; ++f)
+        if (f < a.nf) a.dst[k + f * cap] = a.src[o + f * cap];
     int e = a.ext_src[o];
     a.ext_dst[k] = e;
     a.slot_of[e] = k;
@@ -965,7 +967,7 @@ int launch_sendmols(fm_ctx* c, const int* reset, int nreset) {
     return 0;
 }
 
-int launch_bin(fm_ctx* c) {
+int launch_bin(fm_ctx* c, bool keep_acc) {
     if (c->np == 0) return 0;
     ktimer_begin(c, KT_BIN);
     St s = state_of(c->x(), c->cap);
@@ -1005,6 +1007,7 @@ int launch_bin(fm_ctx* c) {
     a.perm = c->perm;
     a.np = c->np;
     a.cap = c->cap;
+    a.nf = keep_acc ? 9 : 6;
     k_permute<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(a);
     ktimer_end(c, KT_BIN);
     LAUNCH_CHECK();
