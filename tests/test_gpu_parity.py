@@ -58,16 +58,16 @@ def test_cells_and_neighbourlist_c1(c1_state):
     g.close()
 
 
-@pytest.mark.parametrize("jitter", [0.0, 1e-15, 1e-12])
-def test_neighbourlist_pairs_on_the_threshold(jitter):
+@pytest.mark.parametrize("n,jitter", [(12, 0.0), (12, 1e-15), (12, 1e-12), (5, 0.0), (5, 1e-12)])
+def test_neighbourlist_pairs_on_the_threshold(n, jitter):
     """A simple cubic lattice whose face diagonals are rneighbr apart (2 a^2 = rneighbr2): twelve
     pairs per molecule sit on the list threshold, where the FP64 rounding of rij2 (and of the
     halo images) decides membership.  The list build's quantised prefilter leaves every such pair
-    to the FP64 test, so the set must still be the reference's bit for bit."""
+    to the FP64 test, so the set must still be the reference's bit for bit.  n = 5 gives three
+    cells per side, where a brick's region holds each cell twice, under two image shifts."""
     rc, delta = 2.5, 0.3
     rn2 = (rc + delta) ** 2
     a_lat = np.sqrt(rn2 / 2.0)
-    n = 12
     L = n * a_lat
     g1 = (np.arange(n) + 0.5) * a_lat - 0.5 * L
     r = np.stack(np.meshgrid(g1, g1, g1, indexing="ij"), axis=-1).reshape(-1, 3).copy()
