@@ -253,10 +253,11 @@ __device__ bool stage_region(Region& r, const BG& g, int brick, const int* start
                     pz[k] = s.z[sl[k]];
                 }
             }
+            // stores without branches, so that the loads stay ahead of them: an index past the
+            // region writes the last molecule again, with the same values
 #pragma unroll
             for (int k = 0; k < SU; ++k) {
-                const int q = q0 + k * (int)blockDim.x;
-                if (q >= r.total) break;
+                const int q = min(q0 + k * (int)blockDim.x, r.total - 1);
                 if (stage_pos) {
                     int sx, sy, sz;
                     decode_shift(sc[k], sx, sy, sz);
@@ -976,12 +977,15 @@ __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row,
     for (int kw = 0; kw < n; kw += 8) {
         const int k = kw >> 2;
         lj_stage(pb, pos, kw + 4 < n ? w1 : 0);
+        // both words two ahead are loaded here, at the top of the pass: the wait the compiler
+        // places at the loop head is then for loads a whole pass old
+        const uint64_t wk2 = w2;
         w1 = row[(size_t)min(k + 3, lastw) * cap];
+        w2 = row[(size_t)min(k + 4, lastw) * cap];
         __builtin_amdgcn_sched_barrier(0);  // keep the reads above the evaluation
         if constexpr (REC) lj_word_rec(pa, n - kw, xi, yi, zi, rc2, potshift, sx, sy, sz, pe, vt);
         else lj_word_t(pa, n - kw, xi, yi, zi, rc2, sx, sy, sz);
-        lj_stage(pa, pos, kw + 8 < n ? w2 : 0);
-        w2 = row[(size_t)min(k + 4, lastw) * cap];
+        lj_stage(pa, pos, kw + 8 < n ? wk2 : 0);
         __builtin_amdgcn_sched_barrier(0);
         // the second word only when a lane of the wavefront has entries in it
         if (__builtin_amdgcn_ballot_w64(kw + 4 < n) != 0ull) {
