@@ -540,7 +540,10 @@ __device__ inline int quantise(double x, double o, double inv_h) {
     return min(max(q, 0), 32767);
 }
 
-__global__ __launch_bounds__(FB_BLOCK) void k_nlist_quant(NlqArgs a) {
+// registers held to what lets three workgroups share a CU (its LDS region is a third of the
+// force kernels')
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(3 * FB_BLOCK / 256))) void
+k_nlist_quant(NlqArgs a) {
     extern __shared__ __align__(16) char smem[];
     const BG& g = a.g;
     uint2* qv = (uint2*)smem;               // maxreg: x | y << 16, z
