@@ -42,7 +42,8 @@ def parse():
     ap.add_argument("--autotune", action="store_true", help="pick the brick shape with the brick tuner (fresh contexts) first")
     ap.add_argument("--drift-steps", type=int, default=10000, help="drift: NVE steps per precision")
     ap.add_argument("--c3-steps", type=int, default=100, help="c3: Couette steps per precision")
-    ap.add_argument("--workload", choices=["c2", "serve", "drift", "c3"], default="c2",
+    ap.add_argument("--c4-steps", type=int, default=200, help="c4: polymer steps (after as many warm-up steps / 10)")
+    ap.add_argument("--workload", choices=["c2", "serve", "drift", "c3", "c4"], default="c2",
                     help="c2: 1M-molecule LJ step throughput (default); serve: replica stream latency")
     ap.add_argument("--replicas", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=6.0, help="serve: Poisson arrival rate (1/s)")
@@ -168,6 +169,39 @@ def drift_main(a):
     print(json.dumps(out), flush=True)
 
 
+def c4_main(a):
+    """FENE chains of 8 beads on the FCC lattice (64x32x64 units: 524,288 beads, density 0.85,
+    dt 0.002) with WCA between all beads and Mie 14-6 between the beads of the second species
+    (half of the chains); velocity Verlet, displacement rebuild, TPLOT 10, FP64."""
+    from flowmol_amd import config
+    from flowmol_amd.md import MDContext
+
+    p, r, v, moltype, mono = config.config_polymer(nunits=(64, 32, 64), density=0.85, nmono=8, seed=4)
+    N = r.shape[0]
+    g = MDContext(p, r, v)
+    g.set_moltype(moltype)
+    g.set_monomers(*mono)
+    g.setup(0)
+    K = a.c4_steps
+    W = max(1, K // 10)
+    g.run_vv(1, W)
+    g.synchronize()
+    t0 = time.perf_counter()
+    _, nrb = g.run_vv(1 + W, K)
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    g.close()
+    print(json.dumps({
+        "metric": "atom-steps/s (524k-bead FENE chains, WCA + Mie 14-6, 1 GPU)",
+        "value": N * K / dt, "unit": "atom-steps/s", "higher_is_better": True, "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": dt / K * 1e3, "rebuilds_in_timed_steps": nrb,
+        "dtype": "f64", "data": "synthetic", "vs_baseline": None,
+        "config": {"workload": "FENE k 30 R0 1.5, 65,536 chains of 8 beads on FCC 64x32x64, density 0.85, "
+                               "WCA all pairs, Mie 14-6 (rc 2.5) within species 1 (half the chains), dt 0.002, "
+                               "TPLOT 10, displacement rebuild", "molecules": N},
+    }), flush=True)
+
+
 def c3_main(a):
     """Couette flow between tethered FCC walls (3 lattice units, top wall at U = 1), MOP stress on
     64 planes normal to y every step, leap-frog (the reference's tag_move path): FP64 and mixed
@@ -265,6 +299,9 @@ def main():
         return
     if a.workload == "c3":
         c3_main(a)
+        return
+    if a.workload == "c4":
+        c4_main(a)
         return
     import numpy as np
 

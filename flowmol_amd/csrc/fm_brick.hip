@@ -60,6 +60,22 @@ __device__ inline double recip1(double x) {
     return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
 }
 
+// x^e for the Mie exponents (lambda/2 and lambda/2 + 1): a whole e in 0..31 by binary powers,
+// branch-free (within a few ulp of pow, far inside the pair law's 1e-12), any other e by pow
+__device__ inline double mie_pow(double x, double e) {
+    const int n = (int)e;
+    if ((double)n == e && n >= 0 && n < 32) {
+        double r = 1.0, b = x;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            r = ((n >> k) & 1) ? r * b : r;
+            b = b * b;
+        }
+        return r;
+    }
+    return pow(x, e);
+}
+
 __device__ inline double p3(double x) { return x * (x * x); }
 __device__ inline double p4(double x) { double x2 = x * x; return x2 * x2; }
 __device__ inline double p6(double x) { double x3 = x * (x * x); return x3 * x3; }
@@ -1092,8 +1108,8 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
             } else {
                 const double eps = a.mie[t], lr = a.mie[T2 + t], la = a.mie[2 * T2 + t], C = a.mie[3 * T2 + t],
                              al = a.mie[4 * T2 + t];
-                acc = C * eps * (lr * pow(inv, 0.5 * lr + 1) - al * la * pow(inv, 0.5 * la + 1));
-                if (RECORD) e = C * eps * (pow(inv, 0.5 * lr) - al * pow(inv, 0.5 * la)) - a.mie[5 * T2 + t];
+                acc = C * eps * (lr * mie_pow(inv, 0.5 * lr + 1) - al * la * mie_pow(inv, 0.5 * la + 1));
+                if (RECORD) e = C * eps * (mie_pow(inv, 0.5 * lr) - al * mie_pow(inv, 0.5 * la)) - a.mie[5 * T2 + t];
             }
             if (MIE) {
                 fx = fx + acc * rx / mi;
