@@ -101,6 +101,7 @@ struct Region {
     uint16_t* rq;  // staged molecule -> region cell
     double* pos;   // staged positions, x y z interleaved (one address, three reads)
     int* lslot;    // optional: slot of each staged molecule
+    int* ltype;    // optional: molecule type (Mie tables) of each staged molecule
     int nic;
     int total;     // staged molecules
     int ni;        // i-molecules
@@ -112,13 +113,19 @@ struct BG {  // brick geometry
 };
 
 // Carve the LDS buffer (8-byte aligned pieces first).
-__device__ inline void carve(Region& r, char* smem, const BG& g, int maxreg, bool with_slot) {
+__device__ inline void carve(Region& r, char* smem, const BG& g, int maxreg, bool with_slot,
+                           bool with_type = false) {
     double* d = (double*)smem;
     r.pos = d;
     int* ip = (int*)(d + 3 * maxreg);
     r.lslot = nullptr;
     if (with_slot) {
         r.lslot = ip;
+        ip += maxreg;
+    }
+    r.ltype = nullptr;
+    if (with_type) {
+        r.ltype = ip;
         ip += maxreg;
     }
     r.rq = (uint16_t*)ip;
@@ -139,9 +146,9 @@ __device__ inline void carve(Region& r, char* smem, const BG& g, int maxreg, boo
 
 // Bytes of the region tables, rounded up to 16 so that an array of doubles placed after them
 // (the MOP plane tallies, updated with 64-bit LDS atomics) is naturally aligned.
-__host__ __device__ inline size_t region_lds_bytes(const BG& g, int maxreg, bool with_slot) {
+__host__ __device__ inline size_t region_lds_bytes(const BG& g, int maxreg, bool with_slot, bool with_type = false) {
     const int nbmax = g.B[0] * g.B[1] * g.B[2];
-    const size_t b = (size_t)maxreg * (3 * sizeof(double) + (with_slot ? sizeof(int) : 0)) +
+    const size_t b = (size_t)maxreg * (3 * sizeof(double) + (with_slot ? sizeof(int) : 0) + (with_type ? sizeof(int) : 0)) +
                      (size_t)((maxreg + 1) / 2 + 4 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int) + 16;
     return (b + 15) & ~(size_t)15;
 }
@@ -248,7 +255,8 @@ __device__ bool stage_tables(Region& r, const BG& g, int brick, const int* start
 
 // Region tables + staged positions (and slots / chain ids when asked).  Ends with __syncthreads().
 __device__ bool stage_region(Region& r, const BG& g, int brick, const int* start, const St& s, bool stage_pos,
-                             const int* mono_src, int cap, int* lchain, int* lsub, int capm, int* flags) {
+                             const int* mono_src, int cap, int* lchain, int* lsub, int capm, int* flags,
+                             const int* type_src = nullptr) {
     if (!stage_tables(r, g, brick, start, capm, flags)) return false;
     if (stage_pos || r.lslot || lchain) {
         // up to SU molecules per thread and pass, all their loads issued before the first store
@@ -282,6 +290,7 @@ __device__ bool stage_region(Region& r, const BG& g, int brick, const int* start
                     r.pos[3 * q + 2] = img(pz[k], sz, g.L[2]);
                 }
                 if (r.lslot) r.lslot[q] = sl[k];
+                if (r.ltype) r.ltype[q] = type_src[sl[k]];
                 if (lchain) {
                     lchain[q] = mono_src[sl[k]];
                     lsub[q] = mono_src[sl[k] + (size_t)cap];
@@ -1030,10 +1039,12 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
     const bool mop = RECORD && a.pxy != nullptr;
     // the stress tallies need each staged molecule's slot (molecule numbers for the owner rule)
     const bool cvrec = RECORD && a.cv != nullptr;
-    carve(r, smem, a.g, a.maxreg, mop || MIE || cvrec);
+    carve(r, smem, a.g, a.maxreg, mop || MIE || cvrec, MIE);
     double* spxy = nullptr;
-    if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
-    if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags)) return;
+    if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true, MIE));
+    if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags,
+                      MIE ? a.moltype : nullptr))
+        return;
     if (mop) {
         for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
         __syncthreads();
@@ -1053,10 +1064,12 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         if (do_rf)
             for (int k = 0; k < 9; ++k) rf[k] = 0.0;
         int ti = 0;
-        double mi = 1.0;
+        // 1/mass of molecule i, once: the per-pair a += f rij / m becomes a multiplication (exact
+        // for unit masses, within an ulp otherwise)
+        double inv_mi = 1.0;
         if (MIE) {
             ti = a.moltype[i];
-            mi = a.mie[7 * T2 + ti];
+            inv_mi = 1.0 / a.mie[7 * T2 + ti];
         }
         const int exti = (mop || (RECORD && a.cv)) ? a.ext[i] : 0;
         const uint64_t* row = a.nbl + i;
@@ -1095,7 +1108,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
             double rc2 = a.rc2;
             int t = 0;
             if (MIE) {
-                t = ti * FM_MAX_TYPES + a.moltype[r.lslot[j]];
+                t = ti * FM_MAX_TYPES + r.ltype[j];
                 const double c = a.mie[6 * T2 + t];
                 if (c > 0.0) rc2 = c;
             }
@@ -1112,9 +1125,9 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                 if (RECORD) e = C * eps * (mie_pow(inv, 0.5 * lr) - al * mie_pow(inv, 0.5 * la)) - a.mie[5 * T2 + t];
             }
             if (MIE) {
-                fx = fx + acc * rx / mi;
-                fy = fy + acc * ry / mi;
-                fz = fz + acc * rz / mi;
+                fx = fx + acc * rx * inv_mi;
+                fy = fy + acc * ry * inv_mi;
+                fz = fz + acc * rz * inv_mi;
             } else {  // LJ mass is 1
                 fx = fmaf64(acc, rx, fx);
                 fy = fmaf64(acc, ry, fy);
@@ -1159,9 +1172,9 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                     continue;
                 }
                 const double acc = -a.k_c / (1.0 - (r2 / a.R02));
-                fx = fx + acc * rx / mi;
-                fy = fy + acc * ry / mi;
-                fz = fz + acc * rz / mi;
+                fx = fx + acc * rx * inv_mi;
+                fy = fy + acc * ry * inv_mi;
+                fz = fz + acc * rz * inv_mi;
                 if (VIR) vir = vir + acc * r2;
                 if (RECORD) {
                     if (a.cv) {
@@ -1512,8 +1525,9 @@ size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind) {
         bool poly = c->p.potential_flag == 1;
         return region_lds_bytes(g, maxreg, poly) + (poly ? 2 * (size_t)maxreg * sizeof(int) : 0);
     }
-    if (kind == 1) return region_lds_bytes(g, maxreg, c->p.mie_potential == 1 || c->p.vflux_outflag == 4);
-    return region_lds_bytes(g, maxreg, true) + 3 * (size_t)c->p.nplanes * sizeof(double);
+    const bool mie = c->p.mie_potential == 1;
+    if (kind == 1) return region_lds_bytes(g, maxreg, mie || c->p.vflux_outflag == 4, mie);
+    return region_lds_bytes(g, maxreg, true, mie) + 3 * (size_t)c->p.nplanes * sizeof(double);
 }
 
 // measure the largest region (after binning); result in d_flags[FLAG_MAX_REGION]
@@ -1664,7 +1678,7 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     // LDS: the MOP layout whenever MOP is enabled, so that one launch can mix record steps
     const bool mop_enabled = c->p.vflux_outflag >= 1 && c->p.vflux_outflag <= 3 && c->pxy;
     const size_t lds = brick_lds_bytes(c, c->maxreg, mop_enabled ? 2 : 1);
-    if (mop && (region_lds_bytes(g, c->maxreg, true) % sizeof(double)) != 0)
+    if (mop && (region_lds_bytes(g, c->maxreg, true, mie) % sizeof(double)) != 0)
         return fm_set_error(FM_EINVAL, "MOP tallies would be misaligned in LDS");
     memcpy(out, &a, sizeof a);
     if (lds_out) *lds_out = lds;
