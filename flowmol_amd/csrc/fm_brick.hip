@@ -20,6 +20,9 @@
 
 #pragma clang fp contract(off)
 
+// doubles in the Mie tables: 7 fields per type pair, then the mass of each type
+#define FM_MIE_WORDS (7 * FM_MAX_TYPES * FM_MAX_TYPES + FM_MAX_TYPES)
+
 namespace {
 
 struct St {
@@ -1042,6 +1045,16 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
     carve(r, smem, a.g, a.maxreg, mop || MIE || cvrec, MIE);
     double* spxy = nullptr;
     if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true, MIE));
+    // the Mie tables (7 fields per type pair, then the mass per type) in LDS after the region and
+    // the MOP planes
+    const int T2 = FM_MAX_TYPES * FM_MAX_TYPES;
+    const double* mtab = a.mie;
+    if constexpr (MIE) {
+        double* t = (double*)(smem + region_lds_bytes(a.g, a.maxreg, mop || MIE || cvrec, MIE) +
+                              (mop ? 3 * (size_t)a.nplanes * sizeof(double) : 0));
+        for (int k = threadIdx.x; k < FM_MIE_WORDS; k += blockDim.x) t[k] = a.mie[k];
+        mtab = t;  // visible after the staging's barriers below
+    }
     if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags,
                       MIE ? a.moltype : nullptr))
         return;
@@ -1050,7 +1063,6 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         __syncthreads();
     }
     const size_t cap = a.cap;
-    const int T2 = FM_MAX_TYPES * FM_MAX_TYPES;
     // a record step that tallies PE and virial only (no MOP, CV or rfmol)
     const bool simple_rec = RECORD && !mop && !cvrec && a.rf == nullptr;
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
@@ -1069,7 +1081,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         double inv_mi = 1.0;
         if (MIE) {
             ti = a.moltype[i];
-            inv_mi = 1.0 / a.mie[7 * T2 + ti];
+            inv_mi = 1.0 / mtab[7 * T2 + ti];
         }
         const int exti = (mop || (RECORD && a.cv)) ? a.ext[i] : 0;
         const uint64_t* row = a.nbl + i;
@@ -1109,7 +1121,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
             int t = 0;
             if (MIE) {
                 t = ti * FM_MAX_TYPES + r.ltype[j];
-                const double c = a.mie[6 * T2 + t];
+                const double c = mtab[6 * T2 + t];
                 if (c > 0.0) rc2 = c;
             }
             if (!(r2 < rc2)) continue;
@@ -1119,10 +1131,10 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                 acc = 48.0 * (p7(inv) - 0.5 * p4(inv));
                 if (RECORD) e = 4.0 * (p6(inv) - p3(inv)) - a.potshift;
             } else {
-                const double eps = a.mie[t], lr = a.mie[T2 + t], la = a.mie[2 * T2 + t], C = a.mie[3 * T2 + t],
-                             al = a.mie[4 * T2 + t];
+                const double eps = mtab[t], lr = mtab[T2 + t], la = mtab[2 * T2 + t], C = mtab[3 * T2 + t],
+                             al = mtab[4 * T2 + t];
                 acc = C * eps * (lr * mie_pow(inv, 0.5 * lr + 1) - al * la * mie_pow(inv, 0.5 * la + 1));
-                if (RECORD) e = C * eps * (mie_pow(inv, 0.5 * lr) - al * mie_pow(inv, 0.5 * la)) - a.mie[5 * T2 + t];
+                if (RECORD) e = C * eps * (mie_pow(inv, 0.5 * lr) - al * mie_pow(inv, 0.5 * la)) - mtab[5 * T2 + t];
             }
             if (MIE) {
                 fx = fx + acc * rx * inv_mi;
@@ -1526,8 +1538,9 @@ size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind) {
         return region_lds_bytes(g, maxreg, poly) + (poly ? 2 * (size_t)maxreg * sizeof(int) : 0);
     }
     const bool mie = c->p.mie_potential == 1;
-    if (kind == 1) return region_lds_bytes(g, maxreg, mie || c->p.vflux_outflag == 4, mie);
-    return region_lds_bytes(g, maxreg, true, mie) + 3 * (size_t)c->p.nplanes * sizeof(double);
+    const size_t tab = mie ? FM_MIE_WORDS * sizeof(double) : 0;  // Mie tables
+    if (kind == 1) return region_lds_bytes(g, maxreg, mie || c->p.vflux_outflag == 4, mie) + tab;
+    return region_lds_bytes(g, maxreg, true, mie) + 3 * (size_t)c->p.nplanes * sizeof(double) + tab;
 }
 
 // measure the largest region (after binning); result in d_flags[FLAG_MAX_REGION]
