@@ -103,7 +103,7 @@ struct Region {
     int* cst;      // nreg: first slot of each region cell's global cell
     uint16_t* rq;  // staged molecule -> region cell
     double* pos;   // staged positions, x y z interleaved (one address, three reads)
-    int* lslot;    // optional: slot of each staged molecule
+    int* lslot;    // optional: slot of each staged molecule (the force kernels: its molecule number)
     int* ltype;    // optional: molecule type (Mie tables) of each staged molecule
     int nic;
     int total;     // staged molecules
@@ -259,7 +259,7 @@ __device__ bool stage_tables(Region& r, const BG& g, int brick, const int* start
 // Region tables + staged positions (and slots / chain ids when asked).  Ends with __syncthreads().
 __device__ bool stage_region(Region& r, const BG& g, int brick, const int* start, const St& s, bool stage_pos,
                              const int* mono_src, int cap, int* lchain, int* lsub, int capm, int* flags,
-                             const int* type_src = nullptr) {
+                             const int* type_src = nullptr, const int* slot_map = nullptr) {
     if (!stage_tables(r, g, brick, start, capm, flags)) return false;
     if (stage_pos || r.lslot || lchain) {
         // up to SU molecules per thread and pass, all their loads issued before the first store
@@ -292,7 +292,7 @@ __device__ bool stage_region(Region& r, const BG& g, int brick, const int* start
                     r.pos[3 * q + 1] = img(py[k], sy, g.L[1]);
                     r.pos[3 * q + 2] = img(pz[k], sz, g.L[2]);
                 }
-                if (r.lslot) r.lslot[q] = sl[k];
+                if (r.lslot) r.lslot[q] = slot_map ? slot_map[sl[k]] : sl[k];
                 if (r.ltype) r.ltype[q] = type_src[sl[k]];
                 if (lchain) {
                     lchain[q] = mono_src[sl[k]];
@@ -1055,8 +1055,10 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         for (int k = threadIdx.x; k < FM_MIE_WORDS; k += blockDim.x) t[k] = a.mie[k];
         mtab = t;  // visible after the staging's barriers below
     }
+    // the staged slot array carries molecule numbers here (the stress tallies' owner rule is its
+    // only reader)
     if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags,
-                      MIE ? a.moltype : nullptr))
+                      MIE ? a.moltype : nullptr, a.ext))
         return;
     if (mop) {
         for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
@@ -1155,7 +1157,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                 }
                 if (mop || a.cv) {
                     const int rcj = region_cell_of(r, a.g.nreg, j);
-                    if (mop_owner(r, a.g, rci, rcj, exti, a.ext[r.lslot[j]])) {
+                    if (mop_owner(r, a.g, rci, rcj, exti, r.lslot[j])) {
                         const double ri[3] = {xi, yi, zi}, rj[3] = {r.pos[3 * j], r.pos[3 * j + 1], r.pos[3 * j + 2]}, rij[3] = {rx, ry, rz};
                         if (mop) mop_entry(a, ri, rj, rij, acc, spxy);
                         // interior pairs carry 2 acc rij, entries against a periodic image acc rij
@@ -1333,7 +1335,10 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
     carve(r, smem, a.g, a.maxreg, mop || cvrec);
     double* spxy = nullptr;
     if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
-    if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags)) return;
+    // staged molecule numbers in the slot array (the owner rule is its only reader)
+    if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags,
+                      nullptr, a.ext))
+        return;
     if (mop) {
         for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
         __syncthreads();
@@ -1387,7 +1392,7 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
                 pe = pe + (4.0 * (p6(id) - p3(id)) - a.potshift);
                 if (mop || a.cv) {
                     const int rcj = region_cell_of(r, a.g.nreg, j);
-                    if (mop_owner(r, a.g, rci, rcj, exti, a.ext[r.lslot[j]])) {
+                    if (mop_owner(r, a.g, rci, rcj, exti, r.lslot[j])) {
                         const double ri[3] = {xi, yi, zi}, rj[3] = {r.pos[3 * j], r.pos[3 * j + 1], r.pos[3 * j + 2]},
                                      rij[3] = {rxd, ryd, rzd};
                         if (mop) mop_entry(a, ri, rj, rij, (double)acc, spxy);
