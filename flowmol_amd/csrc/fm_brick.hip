@@ -407,7 +407,9 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                 const int c0 = (rx - 1) + R0 * ((ry + dy) + R1 * (rz + dz));
                 jb = r.off[c0];
                 je = r.off[c0 + 3];
-                if (POLY || !a.xsorted) return;
+                // (with bonds too: a bonded partner is closer than R0 <= r_n, so inside the window,
+                // and the runs keep their order, so do the list and bond entries)
+                if (!a.xsorted) return;
                 const double gy = dy < 0 ? gyl : (dy > 0 ? gyh : 0.0);
                 const double gz = dz < 0 ? gzl : (dz > 0 ? gzh : 0.0);
                 const double w2 = a.rn2 - (gy * gy + gz * gz);
