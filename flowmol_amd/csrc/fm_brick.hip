@@ -800,6 +800,7 @@ struct FbArgs {
     const double* mie;     // [7][FM_MAX_TYPES^2] + mass[FM_MAX_TYPES]
     const int* order;      // brick order of one-domain launches (brick_of_block)
     int* halt;             // flag words of a conditional launch (null: unconditional)
+    int step;              // step whose forces this launch computes (the halt rule compares it)
     double* cv;            // faces 1..3 of the control-volume tally, or null
     int cvnbo[3], cvnhb[3];
     double cvbs[3], cvhd[3];
@@ -1277,12 +1278,19 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
     }
 }
 
-// A conditional launch does nothing when a rebuild is pending, and passes the halt on to the
-// launches queued after it (flag words are written by earlier launches only, so every workgroup
-// reads the same values; the FLAG_HALT store by workgroup 0 is read by later launches).
-__device__ inline bool halted(int* f) {
+// A conditional launch does nothing when a rebuild is pending for its own step or an earlier one,
+// and passes the halt on to the launches queued after it.  A force launch that carries the next
+// step's pass 1 may itself raise FLAG_REBUILD for that next step while its later workgroups are
+// still being dispatched: those must not halt.  FLAG_RB_STEP holds INT_MAX from every reset of
+// FLAG_REBUILD until a check stores its step, so a workgroup of the launch of step k reads either
+// INT_MAX or k+1 for a request raised by its own launch (both above k), and a step <= k only for a
+// request raised by an earlier launch (ordered before it on the stream, so every workgroup reads
+// the same values).
+__device__ inline bool halted(int* f, int step) {
     if (!f) return false;
-    const bool h = f[FLAG_HALT] != 0 || f[FLAG_REBUILD] != 0;
+    const bool h = f[FLAG_HALT] != 0 ||
+                   (__atomic_load_n(&f[FLAG_REBUILD], __ATOMIC_RELAXED) != 0 &&
+                    __atomic_load_n(&f[FLAG_RB_STEP], __ATOMIC_RELAXED) <= step);
     if (h && blockIdx.x == 0 && threadIdx.x == 0) f[FLAG_HALT] = 1;
     return h;
 }
@@ -1290,7 +1298,7 @@ __device__ inline bool halted(int* f) {
 template <bool MIE, bool RECORD, bool EXTRA, bool VIR>
 __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
-    if (halted(a.halt)) return;
+    if (halted(a.halt, a.step)) return;
     forces_body<MIE, RECORD, EXTRA, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
@@ -1300,7 +1308,7 @@ template <bool RECORD, bool VIR>
 __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128))) void
 k_forces_brick_lj(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
-    if (halted(a.halt)) return;
+    if (halted(a.halt, a.step)) return;
     forces_body<false, RECORD, false, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
@@ -1312,7 +1320,7 @@ template <bool P1, bool MIXED, bool REC = false>
 __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128, FB_BLOCK / 128))) void
 k_forces_brick_plain(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
-    if (halted(a.halt)) return;
+    if (halted(a.halt, a.step)) return;
     forces_body<false, REC, false, REC, P1, MIXED>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
@@ -1443,7 +1451,7 @@ template <bool RECORD, bool EXTRA, bool VIR>
 __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128))) void
 k_forces_brick_mixed(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
-    if (halted(a.halt)) return;
+    if (halted(a.halt, a.step)) return;
     forces_body_mixed<RECORD, EXTRA, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
@@ -1661,6 +1669,7 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     a.cap = c->cap;
     a.maxreg = c->maxreg;
     const int tplot = c->p.tplot, teval = c->p.teval;
+    a.step = iter;
     a.rec = (iter % tplot) == 0;
     a.ev = teval > 0 && (iter % teval) == 0;
     a.rf = (a.rec && c->p.pressure_outflag == 1) ? c->rf : nullptr;

@@ -261,13 +261,20 @@ struct FlagList {
     int w[8];
 };
 
+// zero one flag word; a reset of FLAG_REBUILD also returns FLAG_RB_STEP to "no step" (INT_MAX),
+// which the halt rule of conditional force launches relies on
+__device__ inline void reset_flag(int* flags, int w) {
+    flags[w] = 0;
+    if (w == FLAG_REBUILD) flags[FLAG_RB_STEP] = 0x7fffffff;
+}
+
 // 1) molecules outside the box, in molecule-number order: workgroup b writes its leavers e
 //    (ascending) to seg[b*FM_BLOCK ..] and their count to bcnt[b].  Workgroup 0 also zeroes the
 //    flag words of `reset` (the resets of a queued rebuild, folded into its first launch).
 __global__ __launch_bounds__(FM_BLOCK) void k_out_select(St s, const int* slot_of, int np, double hx, double hy,
                                                          double hz, int* seg, int* bcnt, int* flags, FlagList reset) {
     __shared__ int wc[FM_BLOCK / 64];
-    if (blockIdx.x == 0 && (int)threadIdx.x < reset.n) flags[reset.w[threadIdx.x]] = 0;
+    if (blockIdx.x == 0 && (int)threadIdx.x < reset.n) reset_flag(flags, reset.w[threadIdx.x]);
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int out = 0;
@@ -809,7 +816,7 @@ int launch_check_disp(fm_ctx* c) {
 
 // zero the listed flag words (one launch for the resets before a queued stage)
 __global__ void k_reset_flags(int* flags, FlagList l) {
-    if ((int)threadIdx.x < l.n) flags[l.w[threadIdx.x]] = 0;
+    if ((int)threadIdx.x < l.n) reset_flag(flags, l.w[threadIdx.x]);
 }
 
 int launch_reset_flags(fm_ctx* c, const int* which, int n) {

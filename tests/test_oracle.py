@@ -137,6 +137,73 @@ def test_mie_14_6_force_is_minus_energy_gradient(r):
     assert run(r).state()[2][0, 0] == pytest.approx(dedr, rel=1e-7)
 
 
+def _mie_pair(d, eps, lr=14.0, la=6.0, rc=2.5):
+    p = box_params()
+    config.set_mie_types(p, {(0, 0): dict(lr=lr, la=la, eps=eps, rc=rc)})
+    pos = np.array([[0.1, 0.2, 0.3], [0.1 + d, 0.2, 0.3]])
+    o = Oracle(p, pos, np.zeros((2, 3)))
+    o.set_moltype(np.zeros(2, np.int32))
+    o.setup()
+    o.compute_forces(0)
+    return p, o
+
+
+@pytest.mark.parametrize("eps", [1.0, 1.7])
+def test_mie_14_6_energy_vanishes_at_cutoff_and_minimum_is_minus_eps(eps):
+    """Closed forms of the 14-6 law, written out by hand: C = (14/8)(14/6)^(6/8); the minimum of
+    C eps (r^-14 - r^-6) lies at r_min = (14/6)^(1/8) with depth -eps; after the shift by the
+    pair's own energy at rc the energy is zero at rc.  A dropped eps, a wrong C, a swapped
+    exponent or a shift taken at another radius fails one of them."""
+    C = 1.75 * (7.0 / 3.0) ** 0.75
+    assert config.mie_C(14.0, 6.0) == pytest.approx(C, rel=1e-15)
+    U_rc = C * eps * (2.5 ** -14 - 2.5 ** -6)
+    p, o = _mie_pair(2.5 - 1e-12, eps)
+    assert p.potshift_lookup[0] == pytest.approx(U_rc, rel=1e-14)
+    assert abs(o.molecular()[0][0]) < 1e-11
+    r_min = (14.0 / 6.0) ** 0.125
+    _, o = _mie_pair(r_min, eps)
+    pe, _ = o.molecular()
+    assert pe[0] == pytest.approx(-eps - U_rc, rel=1e-13)
+    assert pe[1] == pe[0]
+    assert np.abs(o.state()[2]).max() < 1e-12 * eps
+
+
+# ---- pressure: pressure = density/(3N) (sum m v^2 + virial/2), src/simulation_record.f90:1286 ----
+def test_pressure_without_pairs_is_density_times_temperature():
+    """Eight molecules 10 apart (no pair within rc + delta): virial 0 and P = rho T exactly."""
+    L = 20.0
+    pos = np.array([[x, y, z] for x in (-5.0, 5.0) for y in (-5.0, 5.0) for z in (-5.0, 5.0)])
+    v = np.random.default_rng(3).standard_normal((8, 3))
+    p = config.make_params((L, L, L), 8 / L ** 3, 2.5)
+    o = Oracle(p, pos, v)
+    o.setup()
+    o.compute_forces(0)
+    m = o.macros()
+    assert m["virial"] == 0.0
+    T = float((v * v).sum()) / 24.0
+    assert m["temperature"] == pytest.approx(T, rel=1e-14)
+    assert m["pressure"] == pytest.approx((8 / L ** 3) * T, rel=1e-14)
+
+
+@pytest.mark.parametrize("r", [1.05, 1.2, 1.6])
+def test_pressure_two_body_closed_form(r):
+    """Two molecules at rest at separation r in a box of volume V (density 2/V): the pressure is
+    the pair's virial r.f(r)/(3V) with f(r) = 48 (r^-13 - r^-7/2), counted once although both
+    molecules tally it (the virial/2 of the reference)."""
+    L = 20.0
+    V = L ** 3
+    p = config.make_params((L, L, L), 2 / V, 2.5)
+    pos = np.array([[0.1, 0.2, 0.3], [0.1 + r, 0.2, 0.3]])
+    o = Oracle(p, pos, np.zeros((2, 3)))
+    o.setup()
+    o.compute_forces(0)
+    m = o.macros()
+    rf = r * 48.0 * (r ** -13 - 0.5 * r ** -7)
+    assert m["virial"] == pytest.approx(2 * rf, rel=1e-13)
+    assert m["pressure"] == pytest.approx(rf / (3 * V), rel=1e-13)
+    assert m["temperature"] == 0.0
+
+
 # ---- neighbour list ------------------------------------------------------------------------------
 def _disordered(nunits, seed, density=0.8, rc=2.5, steps=150, **kw):
     p, r, v = config.config_fcc(nunits, density, rc, seed=seed, **kw)
