@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--nunits", type=int, default=64, help="FCC units per side (64 -> 1,048,576 molecules)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
-    ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU sample (after its setup)")
+    ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the one-core CPU sample (after its setup)")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP event timing")
     ap.add_argument("--autotune", action="store_true", help="pick the brick shape with the brick tuner (fresh contexts) first")
     ap.add_argument("--drift-steps", type=int, default=10000, help="drift: NVE steps per precision")
@@ -267,20 +267,57 @@ def c3_main(a):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(p, r, v, steps):
-    """The CPU restatement of the reference (oracle, one core) on the same workload."""
+def _oracle_run(args):
+    """One oracle process: setup, then `steps` md_advance_vv steps with the record-step sums every
+    TPLOT steps; returns (molecules, seconds of the time loop)."""
+    nunits, steps, seed = args
+    from flowmol_amd import config
     from oracle.oracle import Oracle
 
+    p, r, v = config.config_fcc(nunits, 0.8, 2.5, seed=seed, tplot=10)
     o = Oracle(p, r, v)
     o.setup()
     o.compute_forces(0)
     t0 = time.perf_counter()
-    o.run_vv(1, steps)
-    dt = time.perf_counter() - t0
-    n = r.shape[0]
-    return {"value": n * steps / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-            "sample": f"same {n}-molecule workload, {steps} md_advance_vv steps after setup, "
-                      f"{dt:.1f} s, single-threaded C restatement (gcc -O3) on this host"}
+    o.run_vv(1, steps, n_macros=steps // 10 + 1)
+    return r.shape[0], time.perf_counter() - t0
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(nunits, steps):
+    """The CPU restatement of the reference (oracle) on this host's cores, two legs:
+    * one core on the same 1M-molecule workload (`steps` steps after its setup);
+    * every core: C processes at once, each on a 1/C share of the molecules (an FCC box of
+      64 x 64 x 64/C units, the share one rank of a C-rank domain decomposition holds), aggregate
+      atom-steps/s = sum of molecules x steps / the slowest process's time.  No halo exchange is
+      modelled, so this leg bounds an MPI run of the reference from above."""
+    import multiprocessing as mp
+
+    n1, t1 = _oracle_run(((nunits, nunits, nunits), steps, 2))
+    one = n1 * steps / t1
+    C = host_cores()
+    while C > 1 and nunits % C:
+        C -= 1
+    share = (nunits, nunits, nunits // C)
+    steps_c = max(steps, 10 * (int(1.0e7 / (4 * nunits * nunits * (nunits // C)) + 9) // 10))
+    with mp.get_context("spawn").Pool(C) as pool:
+        res = pool.map(_oracle_run, [(share, steps_c, 100 + k) for k in range(C)])
+    n_all = sum(n for n, _ in res)
+    t_all = max(t for _, t in res)
+    allc = n_all * steps_c / t_all
+    return {"value": allc, "unit": "atom-steps/s", "cores": C, "kind": "port",
+            "sample": f"{C} concurrent single-threaded processes of the C restatement (gcc -O3), each on "
+                      f"a {4 * share[0] * share[1] * share[2]}-molecule share (FCC {share[0]}x{share[1]}x{share[2]}) "
+                      f"for {steps_c} md_advance_vv steps with sums every 10, {t_all:.1f} s; no halo exchange",
+            "single_core": {"value": one, "cores": 1,
+                            "sample": f"the {n1}-molecule workload, {steps} steps after setup, {t1:.1f} s"}}
 
 
 def main():
@@ -326,10 +363,13 @@ def main():
     g.run_vv(1, a.warmup)
     g.synchronize()
     it0 = 1 + a.warmup
+    # the energy/virial log: the record-step sums every TPLOT = 10 steps, reduced on the device
+    nrec = a.steps // 10 + 1
     t0 = time.perf_counter()
-    _, nrb = g.run_vv(it0, a.steps)
+    macros, nrb = g.run_vv(it0, a.steps, n_macros=nrec)
     g.synchronize()
     t1 = time.perf_counter()
+    logged = [m for m in macros if m["msum"] > 0]
     dt = t1 - t0
     value = N * a.steps / dt
     # kernel durations: the next K steps again, with HIP events around every launch on the
@@ -338,7 +378,7 @@ def main():
     kt = {}
     if not a.no_timing:
         g.kernel_timing(True)
-        g.run_vv(it0 + a.steps, a.steps)
+        g.run_vv(it0 + a.steps, a.steps, n_macros=nrec)
         g.synchronize()
         kt = g.get_kernel_timing()
 
@@ -390,8 +430,13 @@ def main():
         "dtype": "f64" if prec == FM_PREC_FP64 else "f32-pair/f64-state",
         "data": "synthetic",
         "config": {"workload": f"LJ NVE FCC {n_u}^3 ({N} molecules), density 0.8, rc 2.5, delta 0.3, "
-                               f"dt 0.005, T 1.0, TPLOT 10, displacement rebuild",
+                               f"dt 0.005, T 1.0, TPLOT 10, displacement rebuild, energy/virial sums every "
+                               f"10 steps",
                    "molecules": N, "rebuilds_in_timed_steps": nrb, "p_half": round(p_half, 2),
+                   "energy_log": {"records": len(logged),
+                                  "totenergy_first": logged[0]["totenergy"] if logged else None,
+                                  "totenergy_last": logged[-1]["totenergy"] if logged else None,
+                                  "pressure_last": logged[-1]["pressure"] if logged else None},
                    "brick": "autotuned %dx%dx%d" % brick if a.autotune else "default",
                    "precision": a.precision},
         "kernels_ms": {k: round(vv[0], 5) for k, vv in kt.items() if vv[1] > 0},
@@ -400,8 +445,7 @@ def main():
     }
     g.close()
     if a.cpu_baseline == "on":
-        p2, r2, v2 = config.config_fcc((n_u, n_u, n_u), 0.8, 2.5, seed=2, tplot=10)
-        out["cpu_baseline"] = cpu_baseline(p2, r2, v2, a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(n_u, a.cpu_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -10,6 +10,7 @@ FM_MAX_FUNCY = 4
 FM_MAX_PLANES = 1024
 
 FM_OK = 0
+FM_EABORT = 8
 FM_ENSEMBLE_NVE = 0
 FM_ENSEMBLE_TAG_MOVE = 6
 FM_REBUILD_VMAX = 0

@@ -27,9 +27,11 @@ SIGNATURES = {
     "fm_upload_state": (C.c_int, [_vp, _i32, _dp, _dp, _dp]),
     "fm_upload_moltype": (C.c_int, [_vp, _ip]),
     "fm_upload_tags": (C.c_int, [_vp, _ip, _dp]),
-    "fm_upload_monomers": (C.c_int, [_vp, _ip, _ip, _ip, _ip]),
+    "fm_upload_monomers": (C.c_int, [_vp, _ip, _ip, _ip, _ip, _ip]),
     "fm_download_state": (C.c_int, [_vp, _dp, _dp, _dp]),
     "fm_download_tags": (C.c_int, [_vp, _dp]),
+    "fm_download_labels": (C.c_int, [_vp, _ip, _ip, _ip, _ip, _ip, _ip, _ip]),
+    "fm_download_potenergy_parts": (C.c_int, [_vp, _dp, _dp]),
     "fm_download_molecular": (C.c_int, [_vp, _dp, _dp]),
     "fm_download_rfmol": (C.c_int, [_vp, _dp]),
     "fm_get_np": (C.c_int, [_vp, _ip]),

@@ -149,12 +149,13 @@ static void free_ctx(fm_ctx* c) {
                     c->tag[0], c->tag[1], c->teth[0], c->teth[1], c->mono[0], c->mono[1], c->cell_of,
                     c->cell_count, c->cell_start, c->cell_cursor, c->perm, c->nbl, c->nbn, c->bond, c->bondn,
                     c->pe_lj, c->pe_poly, c->vir, c->rf, c->pxy, c->cvface, c->planes, c->out_flag, c->out_vals,
-                    c->out_slot, c->out_count, c->red, c->d_macros, c->d_mie, c->d_vmax, c->d_rmax,
+                    c->out_slot, c->out_count, c->red, c->d_macros, c->d_ring, c->d_mie, c->d_vmax, c->d_rmax,
                     c->d_flags, c->stage, c->brick_order, c->hcurve, c->sort_key,
                     c->sort_val, c->sort_key2, c->sort_val2, c->sort_cnt, c->sort_tmp};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->h_flags) hipHostFree(c->h_flags);
+    if (c->h_ring) hipHostFree(c->h_ring);
     if (c->timer) {
         for (auto e : c->timer->pool) hipEventDestroy(e);
         delete c->timer;
@@ -385,6 +386,7 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
         (rc = dalloc(&c->pe_poly, cap)) || (rc = dalloc(&c->vir, cap)) || (rc = dalloc(&c->out_flag, cap)) ||
         (rc = dalloc(&c->out_vals, cap)) || (rc = dalloc(&c->out_slot, cap)) || (rc = dalloc(&c->out_count, (cap + FM_BLOCK - 1) / FM_BLOCK + 4)) ||
         (rc = dalloc(&c->red, (size_t)c->nred_blocks * 8)) || (rc = dalloc(&c->d_macros, 16)) ||
+        (rc = dalloc(&c->d_ring, 8 * (size_t)FM_RING_SLOTS)) ||
         (rc = dalloc(&c->d_mie, 7 * (size_t)T2 + FM_MAX_TYPES)) || (rc = dalloc(&c->d_vmax, 1)) ||
         (rc = dalloc(&c->d_rmax, 1)) || (rc = dalloc(&c->d_flags, FLAG_COUNT)) || (rc = dalloc(&c->stage, 9 * cap)) ||
         (rc = alloc_nlist(c, maxnb))) {
@@ -393,7 +395,7 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
     }
     if (p->potential_flag == 1) {
         if ((rc = dalloc(&c->bond, FM_MAX_FUNCY * cap)) || (rc = dalloc(&c->bondn, cap)) ||
-            (rc = dalloc(&c->mono[0], 7 * cap)) || (rc = dalloc(&c->mono[1], 7 * cap))) {
+            (rc = dalloc(&c->mono[0], 8 * cap)) || (rc = dalloc(&c->mono[1], 8 * cap))) {
             free_ctx(c);
             return rc;
         }
@@ -496,6 +498,7 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
         }
     }
     he = hipHostMalloc((void**)&c->h_flags, FLAG_COUNT * sizeof(int), hipHostMallocDefault);
+    if (he == hipSuccess) he = hipHostMalloc((void**)&c->h_ring, 8 * FM_RING_SLOTS * sizeof(double), hipHostMallocDefault);
     if (he != hipSuccess) {
         free_ctx(c);
         return fm_set_error(FM_EHIP, "hipHostMalloc: %s", hipGetErrorString(he));
@@ -665,20 +668,22 @@ extern "C" int fm_upload_tags(fm_ctx* c, const int32_t* tag, const double* rteth
     return 0;
 }
 
+// glob_no may be null: then molecule n carries glob_no n+1, as setup numbers them on one process
 extern "C" int fm_upload_monomers(fm_ctx* c, const int32_t* chainID, const int32_t* subchainID, const int32_t* funcy,
-                                  const int32_t* bin_bflag) {
+                                  const int32_t* bin_bflag, const int32_t* glob_no) {
     CHECK_CTX(c);
     if (!c->mono[0]) return fm_set_error(FM_EINVAL, "monomers need potential_flag = 1");
     const size_t np = c->np;
-    std::vector<int32_t> h(7 * np);
+    std::vector<int32_t> h(8 * np);
     for (size_t n = 0; n < np; ++n) {
         if (funcy[n] < 0 || funcy[n] > FM_MAX_FUNCY) return fm_set_error(FM_EINVAL, "funcy out of range at %zu", n);
-        h[7 * n] = chainID[n];
-        h[7 * n + 1] = subchainID[n];
-        h[7 * n + 2] = funcy[n];
-        for (int g = 0; g < 4; ++g) h[7 * n + 3 + g] = bin_bflag[4 * n + g];
+        h[8 * n] = chainID[n];
+        h[8 * n + 1] = subchainID[n];
+        h[8 * n + 2] = funcy[n];
+        for (int g = 0; g < 4; ++g) h[8 * n + 3 + g] = bin_bflag[4 * n + g];
+        h[8 * n + 7] = glob_no ? glob_no[n] : (int32_t)(n + 1);
     }
-    TRY(upload_i32(c, c->mono[c->cur], h.data(), 7));
+    TRY(upload_i32(c, c->mono[c->cur], h.data(), 8));
     c->has_mono = true;
     return 0;
 }
@@ -700,6 +705,63 @@ extern "C" int fm_download_tags(fm_ctx* c, double* rtether) {
     TRY(launch_slot_to_ext_f64(c, c->stage, c->teth[c->cur], 3));
     FM_HIP(hipMemcpyAsync(rtether, c->stage, 3 * (size_t)c->np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     FM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Per-molecule labels in the current molecule numbering: sendmols and sort_mols renumber the
+// molecules, and tag, moltype and monomer move with them (src/messenger.MPI.f90:1460-1904,
+// src/linklist.f90:2820-2860), so a host mirror reads them back after a rebuild.
+extern "C" int fm_download_labels(fm_ctx* c, int32_t* tag, int32_t* moltype, int32_t* chainID, int32_t* subchainID,
+                                  int32_t* funcy, int32_t* bin_bflag, int32_t* glob_no) {
+    CHECK_CTX(c);
+    const size_t np = c->np;
+    int* st = (int*)c->stage;
+    if (tag) {
+        if (!c->has_tags) return fm_set_error(FM_EINVAL, "no tags uploaded");
+        TRY(launch_slot_to_ext_i32(c, st, c->tag[c->cur], 1));
+        FM_HIP(hipMemcpyAsync(tag, st, np * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        FM_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (moltype) {
+        if (!c->has_moltype) return fm_set_error(FM_EINVAL, "no moltype uploaded");
+        TRY(launch_slot_to_ext_i32(c, st, c->moltype[c->cur], 1));
+        FM_HIP(hipMemcpyAsync(moltype, st, np * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        FM_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (chainID || subchainID || funcy || bin_bflag || glob_no) {
+        if (!c->has_mono) return fm_set_error(FM_EINVAL, "no monomer data uploaded");
+        std::vector<int32_t> h(8 * np);
+        TRY(launch_slot_to_ext_i32(c, st, c->mono[c->cur], 8));
+        FM_HIP(hipMemcpyAsync(h.data(), st, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        FM_HIP(hipStreamSynchronize(c->stream));
+        for (size_t n = 0; n < np; ++n) {
+            if (chainID) chainID[n] = h[8 * n];
+            if (subchainID) subchainID[n] = h[8 * n + 1];
+            if (funcy) funcy[n] = h[8 * n + 2];
+            if (bin_bflag)
+                for (int g = 0; g < 4; ++g) bin_bflag[4 * n + g] = h[8 * n + 3 + g];
+            if (glob_no) glob_no[n] = h[8 * n + 7];
+        }
+    }
+    return 0;
+}
+
+// potenergymol_LJ and potenergymol_POLY of the last record step (src/modules.f90:371-375)
+extern "C" int fm_download_potenergy_parts(fm_ctx* c, double* pe_lj, double* pe_poly) {
+    CHECK_CTX(c);
+    const size_t np = c->np;
+    const bool poly = c->p.potential_flag == 1 && c->has_mono;
+    for (int k = 0; k < 2; ++k) {
+        double* out = k == 0 ? pe_lj : pe_poly;
+        if (!out) continue;
+        if (!c->pe_valid || (k == 1 && !poly)) {
+            for (size_t n = 0; n < np; ++n) out[n] = 0.0;
+            continue;
+        }
+        TRY(launch_slot_to_ext_f64(c, c->stage, k == 0 ? c->pe_lj : c->pe_poly, 1));
+        FM_HIP(hipMemcpyAsync(out, c->stage, np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        FM_HIP(hipStreamSynchronize(c->stream));
+    }
     return 0;
 }
 
@@ -944,6 +1006,17 @@ static void derive_macros(const fm_ctx* c, const double* s, fm_macros* o) {
     o->pressure = (c->p.density / (double)(c->np * 3)) * (o->mv2sum + o->virial / 2.0);
 }
 
+// The reference aborts when a molecule's potential energy exceeds 1e4 at a record step
+// (src/simulation_record.f90:1271-1282); the sums are still returned.
+static int check_max_pe(const fm_macros* m) {
+    if (m->max_potenergymol > 10000.0)
+        return fm_set_error(FM_EABORT,
+                            "Simulation aborted because max PE has reached an unreasonably high value "
+                            "(max(potenergymol) = %.6g)",
+                            m->max_potenergymol);
+    return 0;
+}
+
 extern "C" int fm_evaluate_macroscopic_properties(fm_ctx* c, fm_macros* out) {
     CHECK_CTX(c);
     if (!out) return fm_set_error(FM_EINVAL, "null output");
@@ -952,7 +1025,7 @@ extern "C" int fm_evaluate_macroscopic_properties(fm_ctx* c, fm_macros* out) {
     FM_HIP(hipMemcpyAsync(s, c->d_macros, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     FM_HIP(hipStreamSynchronize(c->stream));
     derive_macros(c, s, out);
-    return 0;
+    return check_max_pe(out);
 }
 
 extern "C" int fm_get_mop(fm_ctx* c, double* pxy, int32_t nplanes) {
@@ -1070,14 +1143,20 @@ static int redo_lists(fm_ctx* c) {
     return c->h_flags[FLAG_NEED] ? fm_set_error(FM_EOVERFLOW, "list build still beyond its capacities") : 0;
 }
 
-// bookkeeping of a completed step of the fused loops
-static int finish_step(fm_ctx* c, int iter, fm_macros* macros, int n_macros, int* nm) {
+// bookkeeping of a completed step of the fused loops; ring: the step's record sums, taken on the
+// device when the step was queued (null: evaluate them now, the state being that of this step)
+static int finish_step(fm_ctx* c, int iter, fm_macros* macros, int n_macros, int* nm, const double* ring = nullptr) {
     const bool rec = (iter % c->p.tplot) == 0;
     const bool ev = c->p.teval > 0 && (iter % c->p.teval) == 0;
     c->pe_valid = rec;
     c->vir_valid = rec || ev;
     c->simtime = c->simtime + c->p.delta_t;
     if (rec && macros && *nm < n_macros) {
+        if (ring) {
+            derive_macros(c, ring, &macros[*nm]);
+            ++*nm;
+            return check_max_pe(&macros[*nm - 1]);
+        }
         TRY(fm_evaluate_macroscopic_properties(c, &macros[*nm]));
         ++*nm;
     }
@@ -1086,8 +1165,10 @@ static int finish_step(fm_ctx* c, int iter, fm_macros* macros, int n_macros, int
 
 // md_advance_vv without a host synchronisation per step (rebuild criteria 1 and 2, no bonds or
 // tethers).  Steps are queued in windows that end before the next step whose rebuild the host
-// knows in advance (forced at mod(iter+1,tplot)==0, or the fixed interval), after a record step
-// whose sums are wanted, or after 16 steps; the host synchronises once per window.  With the
+// knows in advance (forced at mod(iter+1,tplot)==0, or the fixed interval), or after
+// FM_RING_SLOTS steps; the host synchronises once per window.  The sums of a record step are
+// reduced on the device right after its force launch, from the buffer that holds the step's
+// final velocities, into a ring slot that the host reads at the window's synchronisation.  With the
 // displacement criterion the queued launches are conditional: the first step whose check fires
 // records its number and the launches after it do nothing; the next window then starts with that
 // step's rebuild and force call.  A window that starts with a rebuild queues it too (the list
@@ -1097,12 +1178,11 @@ static int finish_step(fm_ctx* c, int iter, fm_macros* macros, int n_macros, int
 // keeps which buffer holds the state after each step's pass 1 for the rollback to a fired check.
 // The arithmetic of every completed step is that of the step-by-step loop.
 static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, int n_macros, int* nrb, int* nm) {
-    const int end = iter0 + nsteps, W = 16;
+    const int end = iter0 + nsteps, W = FM_RING_SLOTS;
     const bool disp = c->p.rebuild_criteria == FM_REBUILD_DISPLACEMENT;
     auto host_rebuild = [&](int it) {
         return ((it + 1) % c->p.tplot) == 0 || (!disp && it % c->p.fixed_rebuild == 0);
     };
-    auto sums_wanted = [&](int it) { return macros && *nm < n_macros && (it % c->p.tplot) == 0; };
     static const int reset[] = {FLAG_REBUILD, FLAG_HALT};
     TRY(launch_reset_flags(c, reset, 2));
     int iter = iter0;
@@ -1122,9 +1202,11 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
         int wmax = W;
         if (disp && c->rb_gap > 0 && c->rb_last >= 0) wmax = std::min(W, std::max(3, c->rb_last + c->rb_gap + 2 - iter));
         int wend = iter + 1;
-        while (wend < end && wend - iter < wmax && !host_rebuild(wend) && !sums_wanted(wend - 1)) ++wend;
+        while (wend < end && wend - iter < wmax && !host_rebuild(wend)) ++wend;
         int lists = -1, scur_lists = 0;
         bool fused_prev = false;
+        bool ringed[W];
+        int want = macros ? n_macros - *nm : 0;  // record sums still wanted
         for (int k = iter; k < wend; ++k) {
             c->kt_step = k;
             if (k == iter && rebuild_first) {
@@ -1146,10 +1228,18 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
             } else {
                 scur_p1[k - iter] = c->scur;  // pass 1 of k ran inside the force launch of k-1
             }
-            const int next = (disp && k + 1 < wend && !sums_wanted(k)) ? k + 1 : -1;
+            const int next = (disp && k + 1 < wend) ? k + 1 : -1;
             TRY(launch_forces(c, k, true, disp || lists >= 0, next, &fused_prev));
             if (fused_prev) c->moved_since_bin = true;
+            ringed[k - iter] = want > 0 && (k % c->p.tplot) == 0;
+            if (ringed[k - iter]) {
+                // a fused launch left this step's final velocities in the buffer it read
+                TRY(launch_macros(c, fused_prev ? 1 - c->scur : c->scur, c->d_ring + 8 * (k - iter)));
+                --want;
+            }
         }
+        FM_HIP(hipMemcpyAsync(c->h_ring, c->d_ring, 8 * (size_t)(wend - iter) * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
         TRY(sync_flags(c));
         if (c->h_flags[FLAG_NEED]) {
             if (lists < 0) return fm_set_error(FM_ESTATE, "capacity request without a queued list build");
@@ -1178,11 +1268,13 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
             ktimer_drop(c, KT_FORCES_P1, s);
             ktimer_drop(c, KT_PASS1, s + 1);
             c->scur = scur_p1[s - iter];
-            for (int k = iter; k < s; ++k) TRY(finish_step(c, k, macros, n_macros, nm));
+            for (int k = iter; k < s; ++k)
+                TRY(finish_step(c, k, macros, n_macros, nm, ringed[k - iter] ? c->h_ring + 8 * (k - iter) : nullptr));
             rb = s;
             iter = s;
         } else {
-            for (int k = iter; k < wend; ++k) TRY(finish_step(c, k, macros, n_macros, nm));
+            for (int k = iter; k < wend; ++k)
+                TRY(finish_step(c, k, macros, n_macros, nm, ringed[k - iter] ? c->h_ring + 8 * (k - iter) : nullptr));
             iter = wend;
         }
     }

@@ -1253,7 +1253,14 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
                 atomicExch(&a.flags[FLAG_RB_STEP], a.p1step);
             }
             if (VIR) a.vir[i] = vir;
-            if (RECORD) a.pe_lj[i] = pe;
+            if (RECORD) {
+                a.pe_lj[i] = pe;
+                // v at the end of this step stays in this buffer for the record-step sums (no
+                // other thread of this launch reads this molecule's velocity)
+                a.s.vx[i] = ux;
+                a.s.vy[i] = uy;
+                a.s.vz[i] = uz;
+            }
             continue;
         }
         a.s.ax[i] = fx;

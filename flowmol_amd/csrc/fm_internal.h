@@ -24,6 +24,9 @@
 // larger workgroup shares one staged region among more waves (less LDS per wave, less halo per
 // molecule).  The brick shape is chosen so that a brick holds about this many molecules.
 #define FB_BLOCK 512
+// record steps of one queued window of the fused loop whose sums stay on the device until the
+// window's synchronisation (the window is at most this many steps)
+#define FM_RING_SLOTS 16
 
 // device flag words (d_flags)
 enum {
@@ -65,7 +68,7 @@ struct fm_ctx {
     int* moltype[2] = {nullptr, nullptr};
     int* tag[2] = {nullptr, nullptr};
     double* teth[2] = {nullptr, nullptr}; // 6*cap: rtether xyz, slidev xyz
-    int* mono[2] = {nullptr, nullptr};    // 7*cap: chainID subchainID funcy bflag[4]
+    int* mono[2] = {nullptr, nullptr};    // 8*cap: chainID subchainID funcy bflag[4] glob_no
     bool has_moltype = false, has_tags = false, has_mono = false;
 
     // cells
@@ -129,6 +132,8 @@ struct fm_ctx {
     double* red = nullptr;                // partial sums
     int nred_blocks = 0;
     double* d_macros = nullptr;           // 16 doubles
+    double* d_ring = nullptr;             // FM_RING_SLOTS x 8 record-step sums of a queued window
+    double* h_ring = nullptr;             // pinned copy of d_ring, read at the window's synchronisation
     double* d_mie = nullptr;              // eps, lr, la, C, alpha, potshift, rc2 tables + mass[4]
     unsigned long long* d_vmax = nullptr; // bits of max |v|^2
     double* d_rmax = nullptr;
@@ -194,7 +199,9 @@ int launch_nlist(fm_ctx* c);
 int launch_forces(fm_ctx* c, int iter, bool kick = false, bool cond = false, int next = -1, bool* fused = nullptr);
 int launch_pass1_check_disp(fm_ctx* c, bool cond = false, int step = 0);
 int launch_move_lfv(fm_ctx* c, bool check_disp);
-int launch_macros(fm_ctx* c);
+// record-step sums over state buffer st[buf] (buf < 0: the current one) and the per-molecule
+// tallies into out[8] (null: d_macros)
+int launch_macros(fm_ctx* c, int buf = -1, double* out = nullptr);
 int launch_scatter_state_to_host_order(fm_ctx* c, double* d_out9, double* d_tally);
 int launch_from_host(fm_ctx* c, const double* d_in9);
 int launch_ext_to_slot_i32(fm_ctx* c, int* dst, const int* src, int nf);

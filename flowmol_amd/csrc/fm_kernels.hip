@@ -608,7 +608,7 @@ __global__ __launch_bounds__(FM_BLOCK) void k_permute(PermArgs a) {
         for (int f = 0; f < 6; ++f) a.teth_dst[k + f * cap] = a.teth_src[o + f * cap];
     }
     if (a.mono_dst)
-        for (int f = 0; f < 7; ++f) a.mono_dst[k + f * cap] = a.mono_src[o + f * cap];
+        for (int f = 0; f < 8; ++f) a.mono_dst[k + f * cap] = a.mono_src[o + f * cap];
 }
 
 // ---- macroscopic sums, src/simulation_record.f90:1205-1314 ------------------------------------
@@ -656,12 +656,31 @@ __global__ __launch_bounds__(FM_BLOCK) void k_macros_partial(St s, const double*
     }
 }
 
-__global__ __launch_bounds__(64) void k_macros_final(const double* part, int nb, double* out) {
-    int q = threadIdx.x;
-    if (q >= NRED) return;
-    double x = q == 7 ? -1.0e300 : 0.0;
-    for (int b = 0; b < nb; ++b) x = q == 7 ? fmax(x, part[(size_t)b * NRED + q]) : x + part[(size_t)b * NRED + q];
-    out[q] = x;
+// one workgroup: thread t sums partials t, t+256, ... in that order, then a fixed tree over the
+// threads (deterministic; the partial count is the number of k_macros_partial workgroups)
+__global__ __launch_bounds__(256) void k_macros_final(const double* part, int nb, double* out) {
+    __shared__ double red[NRED][256];
+    const int t = threadIdx.x;
+    double x[NRED];
+#pragma unroll
+    for (int q = 0; q < NRED; ++q) x[q] = q == 7 ? -1.0e300 : 0.0;
+    for (int b = t; b < nb; b += 256) {
+#pragma unroll
+        for (int q = 0; q < NRED; ++q) {
+            const double o = part[(size_t)b * NRED + q];
+            x[q] = q == 7 ? fmax(x[q], o) : x[q] + o;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NRED; ++q) red[q][t] = x[q];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h)
+#pragma unroll
+            for (int q = 0; q < NRED; ++q) red[q][t] = q == 7 ? fmax(red[q][t], red[q][t + h]) : red[q][t] + red[q][t + h];
+        __syncthreads();
+    }
+    if (t < NRED) out[t] = red[t][0];
 }
 
 // ---- host-order staging ---------------------------------------------------------------------
@@ -1023,18 +1042,19 @@ int launch_bin(fm_ctx* c, bool keep_acc) {
     return 0;
 }
 
-int launch_macros(fm_ctx* c) {
+int launch_macros(fm_ctx* c, int buf, double* out) {
     int nb = nblocks(c->np);
     if (nb == 0) nb = 1;
+    if (buf < 0) buf = c->scur;
     ktimer_begin(c, KT_MACROS);
     const bool mie = c->p.mie_potential == 1;
     const bool poly = c->p.potential_flag == 1 && c->has_mono;
-    k_macros_partial<<<nb, FM_BLOCK, 0, c->stream>>>(state_of(c->x(), c->cap), c->pe_lj, poly ? c->pe_poly : nullptr,
+    k_macros_partial<<<nb, FM_BLOCK, 0, c->stream>>>(state_of(c->st[buf], c->cap), c->pe_lj, poly ? c->pe_poly : nullptr,
                                                       c->vir, mie ? c->moltype[c->cur] : nullptr,
                                                       c->d_mie + 7 * FM_MAX_TYPES * FM_MAX_TYPES,
                                                       c->np, c->p.integration_algorithm == FM_INTEGRATION_LFV ? 1 : 0,
                                                       c->p.delta_t, c->red);
-    k_macros_final<<<1, 64, 0, c->stream>>>(c->red, nb, c->d_macros);
+    k_macros_final<<<1, 256, 0, c->stream>>>(c->red, nb, out ? out : c->d_macros);
     ktimer_end(c, KT_MACROS);
     LAUNCH_CHECK();
     return 0;

@@ -14,6 +14,7 @@ module flowmol_hip_c
     integer(c_int32_t), parameter :: FM_ENSEMBLE_NVE = 0, FM_ENSEMBLE_TAG_MOVE = 6
     integer(c_int32_t), parameter :: FM_INTEGRATION_LFV = 0, FM_INTEGRATION_VV = 1
     integer(c_int32_t), parameter :: FM_PREC_FP64 = 0, FM_PREC_MIXED = 1
+    integer(c_int), parameter :: FM_EABORT = 8
 
     type, bind(C) :: fm_params
         real(c_double)     :: domain(3)
@@ -135,6 +136,63 @@ module flowmol_hip_c
             import :: c_int, c_ptr, fm_macros
             type(c_ptr), value            :: ctx
             type(fm_macros), intent(out)  :: m
+        end function
+        integer(c_int) function fm_upload_moltype(ctx, moltype) bind(C, name="fm_upload_moltype")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value             :: ctx
+            integer(c_int32_t), intent(in) :: moltype(*)
+        end function
+        integer(c_int) function fm_upload_monomers(ctx, chainID, subchainID, funcy, bin_bflag, glob_no) &
+                bind(C, name="fm_upload_monomers")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value             :: ctx
+            integer(c_int32_t), intent(in) :: chainID(*), subchainID(*), funcy(*), bin_bflag(*), glob_no(*)
+        end function
+        integer(c_int) function fm_download_tags(ctx, rtether) bind(C, name="fm_download_tags")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value          :: ctx
+            real(c_double), intent(out) :: rtether(*)
+        end function
+        ! any argument may be c_null_ptr
+        integer(c_int) function fm_download_labels(ctx, tag, moltype, chainID, subchainID, funcy, bin_bflag, &
+                glob_no) bind(C, name="fm_download_labels")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, tag, moltype, chainID, subchainID, funcy, bin_bflag, glob_no
+        end function
+        integer(c_int) function fm_download_potenergy_parts(ctx, pe_lj, pe_poly) &
+                bind(C, name="fm_download_potenergy_parts")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value          :: ctx
+            real(c_double), intent(out) :: pe_lj(*), pe_poly(*)
+        end function
+        integer(c_int) function fm_download_rfmol(ctx, rfmol) bind(C, name="fm_download_rfmol")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value          :: ctx
+            real(c_double), intent(out) :: rfmol(*)
+        end function
+        integer(c_int) function fm_get_mop(ctx, pxy, nplanes) bind(C, name="fm_get_mop")
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value          :: ctx
+            real(c_double), intent(out) :: pxy(*)
+            integer(c_int32_t), value   :: nplanes
+        end function
+        integer(c_int) function fm_reset_mop(ctx) bind(C, name="fm_reset_mop")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function fm_get_cv_stress(ctx, pxyface, n) bind(C, name="fm_get_cv_stress")
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value          :: ctx
+            real(c_double), intent(out) :: pxyface(*)
+            integer(c_int64_t), value   :: n
+        end function
+        integer(c_int) function fm_reset_cv_stress(ctx) bind(C, name="fm_reset_cv_stress")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function fm_synchronize(ctx) bind(C, name="fm_synchronize")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
         end function
     end interface
 

@@ -20,7 +20,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import FM_MAX_FUNCY, FmMacros, dptr, iptr
+from ._abi import FM_ENSEMBLE_TAG_MOVE, FM_MAX_FUNCY, FmMacros, dptr, iptr
 from ._lib import check, load
 
 
@@ -99,10 +99,10 @@ class MDContext:
         rt = None if rtether is None else _f64(rtether)
         check(load().fm_upload_tags(self._h, iptr(np.ascontiguousarray(tag, np.int32)), dptr(rt)))
 
-    def set_monomers(self, chainID, subchainID, funcy, bin_bflag):
-        c = lambda x: np.ascontiguousarray(x, np.int32)
+    def set_monomers(self, chainID, subchainID, funcy, bin_bflag, glob_no=None):
+        c = lambda x: None if x is None else np.ascontiguousarray(x, np.int32)
         check(load().fm_upload_monomers(self._h, iptr(c(chainID)), iptr(c(subchainID)), iptr(c(funcy)),
-                                        iptr(c(bin_bflag))))
+                                        iptr(c(bin_bflag)), iptr(c(glob_no))))
 
     def state(self):
         n = self._np
@@ -114,6 +114,27 @@ class MDContext:
         out = np.zeros((self._np, 3))
         check(load().fm_download_tags(self._h, dptr(out)))
         return out
+
+    def labels(self):
+        """(tag, moltype, monomer) per molecule in the current numbering; entries the context does
+        not hold are None.  monomer = (chainID, subchainID, funcy, bin_bflag (np, 4), glob_no)."""
+        n = self._np
+        p = self.params
+        tag = np.zeros(n, np.int32) if p.ensemble == FM_ENSEMBLE_TAG_MOVE else None
+        mt = np.zeros(n, np.int32) if p.mie_potential == 1 else None
+        mono = None
+        if p.potential_flag == 1:
+            mono = (np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros((n, 4), np.int32),
+                    np.zeros(n, np.int32))
+        m = mono or (None,) * 5
+        check(load().fm_download_labels(self._h, iptr(tag), iptr(mt), iptr(m[0]), iptr(m[1]), iptr(m[2]), iptr(m[3]),
+                                        iptr(m[4])))
+        return tag, mt, mono
+
+    def potenergy_parts(self):
+        lj, poly = np.zeros(self._np), np.zeros(self._np)
+        check(load().fm_download_potenergy_parts(self._h, dptr(lj), dptr(poly)))
+        return lj, poly
 
     def molecular(self):
         pe, vir = np.zeros(self._np), np.zeros(self._np)

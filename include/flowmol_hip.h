@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FM_ABI_VERSION 3
+#define FM_ABI_VERSION 4
 
 #define FM_MAX_TYPES 4      /* moltype values 0..FM_MAX_TYPES-1 (Mie lookup tables) */
 #define FM_MAX_FUNCY 4      /* max bonds per monomer, src/modules.f90:468 */
@@ -40,6 +40,7 @@ extern "C" {
 #define FM_EOVERFLOW 5      /* neighbour / cell / bond capacity exceeded */
 #define FM_EBOND 6          /* broken FENE bond, src/setup_set_parameters.f90:355 */
 #define FM_ENOMEM 7
+#define FM_EABORT 8         /* max(potenergymol) > 1e4 at a record step, src/simulation_record.f90:1271-1282 */
 
 /* integration_algorithm / ensemble values as in src/modules.f90 */
 #define FM_ENSEMBLE_NVE 0
@@ -170,11 +171,21 @@ int  fm_get_capacity(const fm_ctx* ctx, int32_t* list_entries, int32_t* region_m
 int  fm_upload_state(fm_ctx* ctx, int32_t np, const double* r, const double* v, const double* a);
 int  fm_upload_moltype(fm_ctx* ctx, const int32_t* moltype);
 int  fm_upload_tags(fm_ctx* ctx, const int32_t* tag, const double* rtether);
-/* polymer_info_MD monomer fields (src/modules.f90:472-481): chainID, subchainID, funcy, bin_bflag(4) */
+/* polymer_info_MD monomer fields (src/modules.f90:472-481): chainID, subchainID, funcy,
+   bin_bflag(4), glob_no.  glob_no may be NULL: molecule n then carries n (1-based), as one
+   process numbers them at setup. */
 int  fm_upload_monomers(fm_ctx* ctx, const int32_t* chainID, const int32_t* subchainID,
-                        const int32_t* funcy, const int32_t* bin_bflag /* (4,np) */);
+                        const int32_t* funcy, const int32_t* bin_bflag /* (4,np) */, const int32_t* glob_no);
 int  fm_download_state(fm_ctx* ctx, double* r, double* v, double* a);
 int  fm_download_tags(fm_ctx* ctx, double* rtether);
+/* Per-molecule labels in the current molecule numbering, which sendmols and sort_mols change
+   (src/messenger.MPI.f90:1460-1904, src/linklist.f90:2820-2860): tag codes, moltype (0-based
+   device type), and the monomer fields chainID, subchainID, funcy, bin_bflag (4,np), glob_no.
+   Any pointer may be NULL.  A host mirror reads them back after a rebuild. */
+int  fm_download_labels(fm_ctx* ctx, int32_t* tag, int32_t* moltype, int32_t* chainID, int32_t* subchainID,
+                        int32_t* funcy, int32_t* bin_bflag, int32_t* glob_no);
+/* potenergymol_LJ and potenergymol_POLY (src/modules.f90:373-374) as of the last record step */
+int  fm_download_potenergy_parts(fm_ctx* ctx, double* potenergymol_LJ, double* potenergymol_POLY);
 /* per-molecule potenergymol, virialmol (src/modules.f90:371-378) as of the last record step */
 int  fm_download_molecular(fm_ctx* ctx, double* potenergymol, double* virialmol);
 /* rfmol(np,3,3) when pressure_outflag==1, molecule-major: [9n + 3*i + j] */

@@ -9,9 +9,12 @@ Here it is checked against the oracle at its full size:
     (|da_i|_inf / sum_j |f_ij|), per-molecule PE and virial within 1e-12 relative;
   * a short drop-in trajectory through a forced rebuild with the same rebuild steps;
   * the fused loop (force launches that carry the next step's pass 1 and displacement check)
-    bit-identical to the routine-by-routine sequence over a window whose check fires mid-window.
-The input is the seeded lattice disordered by 200 fused steps on the device; both sides are then
-fed the same positions and velocities.
+    bit-identical to the routine-by-routine sequence over a window whose check fires mid-window,
+    its record-step sums (reduced on the device inside the windows) equal to those of the
+    routine-by-routine sequence.
+The input is the seeded lattice disordered by 200 fused steps on the device, ended by the
+reference's rebuild sequence (sendmols wraps every molecule back into the box); both sides are
+then fed the same positions and velocities.
 """
 import sys
 
@@ -39,6 +42,7 @@ def c2_state():
     g = MDContext(p, r, v)
     g.setup(0)
     g.run_vv(1, 200)
+    g.rebuild()
     r2, v2, a2 = g.state()
     g.close()
     return p, r2, v2, a2
@@ -115,8 +119,14 @@ def _fused_vs_routine(p, r, v, a, nsteps, label):
     f = MDContext(p, r, v, a)
     g.setup(0)
     f.setup(0)
-    rg = [it for it in range(1, nsteps + 1) if g.md_advance_vv(it)]
-    _, nrb = f.run_vv(1, nsteps)
+    rg, mg = [], []
+    for it in range(1, nsteps + 1):
+        if g.md_advance_vv(it):
+            rg.append(it)
+        if it % p.tplot == 0:
+            mg.append(g.evaluate_macroscopic_properties())
+    mf, nrb = f.run_vv(1, nsteps, n_macros=len(mg))
+    assert mf == mg  # record-step sums reduced on the device inside the fused windows
     fired = [it for it in rg if (it + 1) % p.tplot != 0]
     say(f"  {label}: rebuilds at {rg} (displacement-fired {fired})")
     assert nrb == len(rg)
@@ -143,6 +153,7 @@ def test_one_cell_bricks_fused_loop_equals_routine_sequence():
     g = MDContext(p, r, v)
     g.setup(0)
     g.run_vv(1, 100)
+    g.rebuild()
     r2, v2, a2 = g.state()
     g.close()
     _fused_vs_routine(p, r2, v2, a2, 80, "brick 1x1x1")
