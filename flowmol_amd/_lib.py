@@ -36,6 +36,7 @@ SIGNATURES = {
     "fm_download_rfmol": (C.c_int, [_vp, _dp]),
     "fm_get_np": (C.c_int, [_vp, _ip]),
     "fm_simtime": (C.c_double, [_vp]),
+    "fm_set_simtime": (C.c_int, [_vp, C.c_double]),
     "fm_move_particles_vv": (C.c_int, [_vp, _i32]),
     "fm_move_particles_lfv": (C.c_int, [_vp]),
     "fm_checkrebuild": (C.c_int, [_vp, _i32, _ip]),

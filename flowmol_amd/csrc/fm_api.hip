@@ -600,6 +600,13 @@ extern "C" int fm_get_np(const fm_ctx* c, int32_t* np) {
 
 extern "C" double fm_simtime(const fm_ctx* c) { return c ? c->simtime : 0.0; }
 
+// simtime of a context resumed from a checkpoint (src/modules.f90:263; final_state header)
+extern "C" int fm_set_simtime(fm_ctx* c, double simtime) {
+    CHECK_CTX(c);
+    c->simtime = simtime;
+    return 0;
+}
+
 // ---- state transfer ---------------------------------------------------------------------------
 extern "C" int fm_upload_state(fm_ctx* c, int32_t np, const double* r, const double* v, const double* a) {
     CHECK_CTX(c);
@@ -944,6 +951,10 @@ extern "C" int fm_assign_to_cell(fm_ctx* c) {
     c->moved_since_bin = false;
     c->cells_valid = true;
     c->nlist_valid = false;
+    // the binning permutes the slots and the displacement sums are kept per slot: they start
+    // again from zero (a rebuild that simulation_checkrebuild asked for has discarded them
+    // already; the rebuild before a final_state write also lands here)
+    c->rdisp_valid = false;
     return 0;
 }
 

@@ -16,7 +16,7 @@ program md_vv_driver
     type(c_ptr) :: ctx
     integer :: nunits, nsteps, np, n, ix, iy, iz, j, iter
     integer(c_int32_t) :: rebuild
-    real(c_double) :: a_lat, L, c(3), rc(3)
+    real(c_double) :: a_lat, L, c(3), rc(3), mv2
     real(c_double), allocatable :: r(:,:), v(:,:), acc(:,:)
     character(len=32) :: arg
 
@@ -102,9 +102,13 @@ program md_vv_driver
         call check(fm_compute_forces(ctx, int(iter, c_int32_t)))
         call check(fm_move_particles_vv(ctx, 2_c_int32_t))
         if (mod(iter, p%tplot) == 0) then
+            ! simulation_record follows pass 2: the host refresh of a record step takes v(t+dt),
+            ! the velocities the device's sums use
             call check(fm_evaluate_macroscopic_properties(ctx, m))
-            write(*,'(i8,a,5(f19.15,a))') iter, ';', m%kinenergy, ';', m%potenergy, ';', m%totenergy, ';', &
-                m%temperature, ';', m%pressure, ';'
+            call check(fm_download_state(ctx, r, v, acc))
+            mv2 = sum(v(:, 1:np)**2)
+            write(*,'(i8,a,5(f19.15,a),2(es24.16,a))') iter, ';', m%kinenergy, ';', m%potenergy, ';', &
+                m%totenergy, ';', m%temperature, ';', m%pressure, ';', mv2, ';', m%mv2sum, ';'
         end if
     end do
     call check(fm_download_state(ctx, r, v, acc))

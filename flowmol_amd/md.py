@@ -150,6 +150,10 @@ class MDContext:
     def simtime(self):
         return load().fm_simtime(self._h)
 
+    @simtime.setter
+    def simtime(self, t):
+        check(load().fm_set_simtime(self._h, float(t)))
+
     # ---- the reference routines ------------------------------------------------------------------
     def simulation_move_particles_vv(self, pass_num):
         check(load().fm_move_particles_vv(self._h, int(pass_num)))

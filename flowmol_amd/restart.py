@@ -112,8 +112,9 @@ def read_final_state(path):
 
 def write_final_state(path, r, v, *, nunits, density, rcutoff, delta_t, delta_rneighbr, domain, tag=None,
                       rtether=None, moltype=None, monomers=None, iter_=0, tplot=1, simtime=0.0, k_c=0.0,
-                      R_0=0.0, seed=(0, 0), periodic=(1, 1, 1)):
-    """Write a single-rank final_state (npx=npy=npz=1) in the reference's layout."""
+                      R_0=0.0, seed=(0, 0), periodic=(1, 1, 1), nmonomers=0):
+    """Write a single-rank final_state (npx=npy=npz=1) in the reference's layout.  monomers rows
+    are in the file's order: chainID, subchainID, funcy, glob_no, bin_bflag(4)."""
     n = r.shape[0]
     tag = np.zeros(n, np.int32) if tag is None else np.asarray(tag, np.int32)
     recs = []
@@ -131,7 +132,7 @@ def write_final_state(path, r, v, *, nunits, density, rcutoff, delta_t, delta_rn
     body = np.array([x for rec in recs for x in rec], dtype="<f8").tobytes()
     hdr = struct.pack("<i", n) + struct.pack("<3i", *nunits) + struct.pack("<2i", iter_, tplot)
     hdr += struct.pack("<2i", *seed) + struct.pack("<3i", *periodic)
-    hdr += struct.pack("<7i", 1 if monomers is not None else 0, 0, 0, 0, 1, 1, 1)
+    hdr += struct.pack("<7i", 1 if monomers is not None else 0, 0, 0, int(nmonomers), 1, 1, 1)
     hdr += struct.pack("<i", n) + struct.pack("<i", ntether)
     hdr += struct.pack("<3d", *map(float, domain))
     hdr += struct.pack("<11d", density, rcutoff, delta_t, 0.0, simtime, k_c, R_0, 0.0, 0.0, 0.0, delta_rneighbr)
@@ -140,3 +141,28 @@ def write_final_state(path, r, v, *, nunits, density, rcutoff, delta_t, delta_rn
         f.write(body)
         f.write(hdr)
         f.write(struct.pack("<q", len(body)))
+
+
+def write_final_state_from_context(ctx, path, *, nunits, iter_, seed=(0, 0), nmonomers=0):
+    """parallel_io_final_state for one device context (src/parallel_io.MPI2.f90:1495-1704): the
+    rebuild sequence first (linklist_deallocateall, sendmols, sort_mols, assign_to_cell,
+    messenger_updateborders, assign_to_neighbourlist: molecules wrapped into the box and
+    renumbered), then the state, tags, tether sites, molecule types and monomer fields in the
+    current numbering.  The device keeps running from the rebuilt lists, as the reference does
+    after a rescue snapshot."""
+    from ._abi import FM_ENSEMBLE_TAG_MOVE
+
+    p = ctx.params
+    ctx.rebuild()
+    r, v, _ = ctx.state()
+    tag, mt, mono = ctx.labels()
+    rt = ctx.rtether() if p.ensemble == FM_ENSEMBLE_TAG_MOVE else None
+    monomers = None
+    if mono is not None:
+        chain, sub, funcy, bflag, glob = mono
+        monomers = np.column_stack([chain, sub, funcy, glob, bflag]).astype(np.int32)
+    write_final_state(path, r, v, nunits=nunits, density=p.density, rcutoff=p.rcutoff, delta_t=p.delta_t,
+                      delta_rneighbr=p.delta_rneighbr[0], domain=[p.domain[0], p.domain[1], p.domain[2]], tag=tag,
+                      rtether=rt, moltype=mt, monomers=monomers, iter_=iter_, tplot=p.tplot, simtime=ctx.simtime,
+                      k_c=p.k_c if p.potential_flag == 1 else 0.0, R_0=p.R_0 if p.potential_flag == 1 else 0.0,
+                      seed=seed, nmonomers=nmonomers)

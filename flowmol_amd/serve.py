@@ -8,7 +8,10 @@ Continuous batching over libflowmol_hip's replica batches (fm_batch_*):
   * admitted replicas keep their state on the device and advance together, every step one packed
     launch for all of them, in chunks of at most `chunk` steps;
   * a finished replica leaves at the end of its chunk, its results are read back and its
-    memory is released, so waiting requests can enter at the next chunk boundary.
+    memory is released, so waiting requests can enter at the next chunk boundary;
+  * a replica with snapshot_every > 0 checkpoints every that many steps (flowmol_amd.checkpoint);
+    Request.abort() stops a replica at the next chunk boundary with a checkpoint, and
+    Request.resumed() continues a checkpoint, bit-identical to the run that took it.
 Latency of a replica = time its results are available - its arrival time.
 """
 import ctypes as C
@@ -73,7 +76,11 @@ class Batch:
 
 
 class Request:
-    def __init__(self, rid, arrival, params, r, v, nsteps, tag=None, rtether=None):
+    """One replica.  snapshot_every > 0: a checkpoint (flowmol_amd.checkpoint) every that many
+    steps; abort() stops the replica at the next chunk boundary with a final checkpoint; a request
+    made by resumed() continues a checkpoint for the steps left."""
+
+    def __init__(self, rid, arrival, params, r, v, nsteps, tag=None, rtether=None, snapshot_every=0):
         self.rid = rid
         self.arrival = float(arrival)
         self.params = params
@@ -84,6 +91,21 @@ class Request:
         self.nsteps = int(nsteps)
         self.np = r.shape[0]
         self.footprint = memory_footprint(params, self.np)
+        self.snapshot_every = int(snapshot_every)
+        self.snapshot = None      # the checkpoint a resumed request starts from
+        self.iter0 = 1
+        self.aborted = False
+
+    def abort(self):
+        self.aborted = True
+
+    @staticmethod
+    def resumed(rid, arrival, snap, nsteps, snapshot_every=0):
+        q = Request(rid, arrival, snap.params, snap.r, snap.v, nsteps, tag=snap.tag, rtether=snap.rtether,
+                    snapshot_every=snapshot_every)
+        q.snapshot = snap
+        q.iter0 = snap.next_iter
+        return q
 
 
 def couette_request(rid, arrival, n_target, wall_speed, nsteps, seed, density=0.8, tplot=10, precision=0):
@@ -115,10 +137,15 @@ class Active:
         self.req = req
         self.ctx = ctx
         self.remaining = req.nsteps
+        self.done = 0
         self.t_admit = t_admit
+        self.last_snapshot = None
 
 
 def make_context(req):
+    if req.snapshot is not None:
+        from .checkpoint import resume
+        return resume(req.snapshot)
     ctx = MDContext(req.params, req.r, req.v)
     if req.tag is not None:
         ctx.set_tags(req.tag, req.rtether)
@@ -157,7 +184,7 @@ class ReplicaServer:
             while pending and (not realtime or pending[0].arrival <= now) and used + pending[0].footprint <= self.budget:
                 q = pending.pop(0)
                 ctx = make_context(q)
-                self.batch.add(ctx, 1)
+                self.batch.add(ctx, q.iter0)
                 used += q.footprint
                 active.append(Active(q, ctx, time.perf_counter() - t0))
             if not active:
@@ -167,18 +194,50 @@ class ReplicaServer:
                 if pending:
                     raise RuntimeError("a request does not fit the HBM budget on its own")
                 break
+            # replicas aborted while their chunk ran stop here, with a checkpoint to resume from
+            for a in [a for a in active if a.req.aborted]:
+                it = self.batch.iter_of(a.ctx)
+                self.batch.remove(a.ctx)
+                from .checkpoint import take
+                snap = take(a.ctx, it)
+                t_done = time.perf_counter() - t0
+                arrival = a.req.arrival if realtime else 0.0
+                res = dict(rid=a.req.rid, np=a.req.np, status="aborted", steps_done=a.done, snapshot=snap,
+                           latency=t_done - arrival, t_admit=a.t_admit, t_done=t_done, queue_wait=a.t_admit - arrival)
+                results[a.req.rid] = res
+                if on_result:
+                    on_result(res)
+                a.ctx.close()
+                used -= a.req.footprint
+                active.remove(a)
+            if not active:
+                continue
             k = min(self.chunk, min(a.remaining for a in active))
+            for a in active:
+                if a.req.snapshot_every > 0:
+                    k = min(k, a.req.snapshot_every - a.done % a.req.snapshot_every)
             self.batch.run(k)
             atom_steps += k * sum(a.req.np for a in active)
             done = [a for a in active if a.remaining - k == 0]
             for a in active:
                 a.remaining -= k
+                a.done += k
+                if a.req.snapshot_every > 0 and a.done % a.req.snapshot_every == 0 and a.remaining > 0:
+                    # a periodic checkpoint: out of the batch for the rebuild and the read-back
+                    from .checkpoint import take
+                    it = self.batch.iter_of(a.ctx)
+                    self.batch.remove(a.ctx)
+                    a.last_snapshot = take(a.ctx, it)
+                    self.batch.add(a.ctx, it)
             for a in done:
                 res = result_of(a.ctx)
+                if a.req.snapshot is not None and a.req.snapshot.mop is not None and "mop" in res:
+                    res["mop"] = a.req.snapshot.mop + res["mop"]  # the tally before the checkpoint
                 t_done = time.perf_counter() - t0
                 arrival = a.req.arrival if realtime else 0.0
-                res.update(rid=a.req.rid, np=a.req.np, latency=t_done - arrival, t_admit=a.t_admit,
-                           t_done=t_done, queue_wait=a.t_admit - arrival)
+                res.update(rid=a.req.rid, np=a.req.np, status="done", steps_done=a.done, latency=t_done - arrival,
+                           t_admit=a.t_admit, t_done=t_done, queue_wait=a.t_admit - arrival,
+                           last_snapshot=a.last_snapshot)
                 results[a.req.rid] = res
                 if on_result:
                     on_result(res)
@@ -187,8 +246,9 @@ class ReplicaServer:
                 used -= a.req.footprint
                 active.remove(a)
         wall = time.perf_counter() - t0
-        lat = np.array([results[k]["latency"] for k in sorted(results)])
-        stats = {"replicas": len(results), "wall_s": wall, "atom_steps": atom_steps,
+        lat = np.array([results[k]["latency"] for k in sorted(results) if results[k]["status"] == "done"])
+        stats = {"replicas": len(results), "aborted": sum(r["status"] == "aborted" for r in results.values()),
+                 "wall_s": wall, "atom_steps": atom_steps,
                  "atom_steps_per_s": atom_steps / wall if wall > 0 else 0.0,
                  "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
                  "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None}

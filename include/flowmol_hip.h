@@ -192,6 +192,8 @@ int  fm_download_molecular(fm_ctx* ctx, double* potenergymol, double* virialmol)
 int  fm_download_rfmol(fm_ctx* ctx, double* rfmol);
 int  fm_get_np(const fm_ctx* ctx, int32_t* np);
 double fm_simtime(const fm_ctx* ctx);
+/* simtime of a context resumed from a checkpoint (the final_state header carries it) */
+int  fm_set_simtime(fm_ctx* ctx, double simtime);
 
 /* ---- the hot path: one call per reference routine ------------------------------------------ */
 /* simulation_move_particles_vv(pass_num), src/simulation_move_particles_vv.f90:44-193 */

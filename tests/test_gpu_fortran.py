@@ -23,6 +23,10 @@ def test_fortran_driver_matches_python_host():
     assert out.returncode == 0, out.stdout + out.stderr
     rows = [l for l in out.stdout.splitlines() if ";" in l]
     f_macros = np.array([[float(x) for x in l.split(";")[1:6]] for l in rows])
+    # the host refresh after pass 2 of a record step reads the velocities the device's sums use
+    host_mv2 = np.array([float(l.split(";")[6]) for l in rows])
+    dev_mv2 = np.array([float(l.split(";")[7]) for l in rows])
+    assert np.allclose(host_mv2, dev_mv2, rtol=1e-13, atol=0)
     # the same lattice and velocity pattern through the Python mirror
     p, r, _ = config.config_fcc((nunits,) * 3, 0.8, 2.5, seed=0, tplot=10)
     n = np.arange(1, r.shape[0] + 1)

@@ -73,3 +73,79 @@ def test_server_completes_and_reports_latency():
         m = ctx.evaluate_macroscopic_properties()
         assert results[q.rid]["macros"]["totenergy"] == m["totenergy"]
         ctx.close()
+
+
+@pytest.mark.parametrize("kind", ["lj", "couette"])
+def test_checkpoint_resume_bit_identical(kind, tmp_path):
+    """N steps, a checkpoint (the rebuild sequence, then the state read back), N more steps; a
+    context resumed from the checkpoint written to disk and read again runs the same N steps to the
+    same bits (positions, velocities, accelerations, tether sites, simtime, record sums)."""
+    from flowmol_amd import checkpoint
+
+    if kind == "lj":
+        p, r, v = config.config_fcc((8, 8, 8), 0.8, 2.5, seed=71, tplot=10)
+        r = r + np.random.default_rng(71).uniform(-0.02, 0.02, r.shape)
+        req = None
+    else:
+        req = couette_request(0, 0.0, 4 * 8 ** 3, 1.3, 60, seed=72)
+    say(f"case: checkpoint/resume, {kind}, 30 + 30 steps")
+
+    def fresh():
+        if req is not None:
+            return make_context(req)
+        c = MDContext(p, r, v)
+        c.setup(0)
+        return c
+
+    a = fresh()
+    a.run_vv(1, 30)
+    snap = checkpoint.take(a, 31)
+    ma, _ = a.run_vv(31, 30, n_macros=3)
+    path = str(tmp_path / "snap.npz")
+    snap.save(path)
+    b = checkpoint.resume(checkpoint.Snapshot.load(path))
+    assert b.simtime == snap.simtime
+    mb, _ = b.run_vv(31, 30, n_macros=3)
+    for x, y in zip(a.state(), b.state()):
+        assert np.array_equal(x, y)
+    assert ma == mb
+    assert a.simtime == b.simtime
+    if req is not None:
+        assert np.array_equal(a.rtether(), b.rtether())
+    a.close()
+    b.close()
+
+
+def test_server_abort_and_resume():
+    """A replica aborted by the server (at a chunk boundary) leaves a checkpoint; resumed for the
+    steps it had left, its record sums equal those of the same replica run alone with the
+    checkpoint taken at the same step."""
+    from flowmol_amd import checkpoint
+    from flowmol_amd.serve import Request
+
+    short = couette_request(0, 0.0, 4 * 7 ** 3, 0.9, 20, seed=81)
+    long_ = couette_request(1, 0.0, 4 * 8 ** 3, 1.4, 100, seed=82)
+    say("case: server abort at a chunk boundary, resume from the checkpoint")
+    srv = ReplicaServer(chunk=20)
+
+    def on_result(res):
+        if res["rid"] == 0:
+            long_.abort()
+
+    results, stats = srv.serve([short, long_], realtime=False, on_result=on_result)
+    assert stats["aborted"] == 1 and results[1]["status"] == "aborted"
+    done = results[1]["steps_done"]
+    assert done == 20
+    snap = results[1]["snapshot"]
+    assert snap.next_iter == 21
+    res2, _ = srv.serve([Request.resumed(2, 0.0, snap, 100 - done)], realtime=False)
+    srv.close()
+    # the same replica alone, with the checkpoint at the same step
+    c = make_context(long_)
+    c.run_vv(1, done)
+    checkpoint.take(c, done + 1)
+    c.run_vv(done + 1, 100 - done)
+    m = c.evaluate_macroscopic_properties()
+    assert res2[2]["macros"] == m
+    assert np.abs(res2[2]["mop"] - c.mop()).max() <= 1e-12 * max(1.0, np.abs(c.mop()).max())
+    c.close()
