@@ -170,13 +170,17 @@ def drift_main(a):
 
 
 def c4_main(a):
-    """FENE chains of 8 beads on the FCC lattice (64x32x64 units: 524,288 beads, density 0.85,
-    dt 0.002) with WCA between all beads and Mie 14-6 between the beads of the second species
-    (half of the chains); velocity Verlet, displacement rebuild, TPLOT 10, FP64."""
+    """65,536 FENE chains of 8 beads (524,288 beads) grown as seeded random walks in the box of
+    the 64x32x64-unit FCC lattice at density 0.85 (dt 0.002), WCA between all beads and Mie 14-6
+    between the beads of the second species (half of the chains); velocity Verlet, displacement
+    rebuild, TPLOT 10, FP64."""
     from flowmol_amd import config
     from flowmol_amd.md import MDContext
 
-    p, r, v, moltype, mono = config.config_polymer(nunits=(64, 32, 64), density=0.85, nmono=8, seed=4)
+    t_gen = time.perf_counter()
+    p, r, v, moltype, mono = config.config_polymer(nunits=(64, 32, 64), density=0.85, nmono=8, seed=4,
+                                                   init="random_walk")
+    t_gen = time.perf_counter() - t_gen
     N = r.shape[0]
     g = MDContext(p, r, v)
     g.set_moltype(moltype)
@@ -196,9 +200,11 @@ def c4_main(a):
         "value": N * K / dt, "unit": "atom-steps/s", "higher_is_better": True, "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": dt / K * 1e3, "rebuilds_in_timed_steps": nrb,
         "dtype": "f64", "data": "synthetic", "vs_baseline": None,
-        "config": {"workload": "FENE k 30 R0 1.5, 65,536 chains of 8 beads on FCC 64x32x64, density 0.85, "
+        "config": {"workload": "FENE k 30 R0 1.5, 65,536 chains of 8 beads, random-walk initial state from seed 4 "
+                               "(bond 0.97, non-bonded separation 0.8) in the box of FCC 64x32x64 at density 0.85, "
                                "WCA all pairs, Mie 14-6 (rc 2.5) within species 1 (half the chains), dt 0.002, "
-                               "TPLOT 10, displacement rebuild", "molecules": N},
+                               "TPLOT 10, displacement rebuild", "molecules": N,
+                   "init_generation_s": round(t_gen, 1)},
     }), flush=True)
 
 

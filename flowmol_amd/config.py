@@ -199,16 +199,83 @@ def chain_topology(nchains, nmono):
     return chain, sub, funcy, bflag.astype(np.int32)
 
 
+def random_walk_chains(domain, nchains, nmono, seed, bond=0.97, min_sep=0.8, max_tries=200):
+    if not min_sep < bond:
+        raise ValueError("min_sep must be below the bond length")
+    """Seeded random-walk chains in a periodic box.  Chain starts are distinct sites of the FCC
+    lattice that fills the box at the same density (drawn with a seeded permutation); bead s+1
+    lies at distance `bond` from bead s in a uniformly random direction, redrawn until it is at
+    least `min_sep` from every bead placed before it (all chains grow one bead per round, and
+    candidates of one round are also checked against each other).  A bead that finds no such
+    direction in max_tries draws (2-3 % of the beads at density 0.85) is placed with the
+    separation lowered by 0.05 at a time.  Returns (nchains*nmono, 3) positions in [-L/2, L/2) with chain c's beads
+    at rows c*nmono .. c*nmono+nmono-1."""
+    from scipy.spatial import cKDTree
+
+    L = np.asarray(domain, np.float64)
+    rng = np.random.default_rng(seed)
+    ntot = nchains * nmono
+    nunits = [int(round(x)) for x in L * ((ntot / (L[0] * L[1] * L[2])) / 4.0) ** (1.0 / 3.0)]
+    sites = fcc_lattice(nunits, 4.0 * nunits[0] * nunits[1] * nunits[2] / (L[0] * L[1] * L[2]))
+    if sites.shape[0] < nchains:
+        raise ValueError("box too small for the chain starts")
+    pos = np.zeros((nchains, nmono, 3))
+    pos[:, 0] = sites[rng.permutation(sites.shape[0])[:nchains]]
+
+    def wrap(x):
+        return x - L * np.floor((x + 0.5 * L) / L)
+
+    for s in range(1, nmono):
+        tree = cKDTree(wrap(pos[:, :s].reshape(-1, 3)) + 0.5 * L, boxsize=L)
+        extra = np.zeros((0, 3))  # beads of this round placed since the tree was built
+        todo = np.arange(nchains)
+        sep = min_sep
+        while todo.size and sep > 0.5 * min_sep:
+            for _ in range(max_tries):
+                d = rng.standard_normal((todo.size, 3))
+                d /= np.linalg.norm(d, axis=1)[:, None]
+                cand = pos[todo, s - 1] + bond * d
+                cw = wrap(cand) + 0.5 * L
+                cw = np.where(cw >= L, cw - L, cw)
+                # every placed bead (the chain's own previous one lies at `bond` > sep)
+                dn, _ = tree.query(cw)
+                ok = dn >= sep - 1e-12
+                if extra.shape[0]:
+                    de, _ = cKDTree(extra, boxsize=L).query(cw)
+                    ok &= de >= sep - 1e-12
+                # candidates of one draw against each other: the later one of a close pair redraws
+                for a, b in cKDTree(cw, boxsize=L).query_pairs(sep - 1e-12):
+                    ok[max(a, b)] = False
+                pos[todo[ok], s] = cand[ok]
+                extra = np.concatenate([extra, cw[ok]])
+                if extra.shape[0] > 2048:
+                    tree = cKDTree(np.concatenate([tree.data, extra]), boxsize=L)
+                    extra = np.zeros((0, 3))
+                todo = todo[~ok]
+                if todo.size == 0:
+                    break
+            sep -= 0.05
+        if todo.size:
+            raise RuntimeError(f"{todo.size} chains could not place bead {s + 1}")
+    return np.ascontiguousarray(wrap(pos.reshape(ntot, 3)))
+
+
 def config_polymer(nunits=(6, 8, 6), density=0.85, nmono=8, seed=4, k_c=30.0, R_0=1.5, dt=0.002,
-                   mie_fraction=0.5, tplot=10, **kw):
-    """FENE chains of nmono beads on the FCC lattice (4*nunits[1] divisible by nmono) with WCA
-    between all beads, and a second species (whole chains, chosen with a seeded shuffle) whose
-    mutual interaction is Mie 14-6 (eps 1, rc 2.5); species 0 pairs and mixed pairs are WCA."""
-    if (4 * nunits[1]) % nmono:
+                   mie_fraction=0.5, tplot=10, init="lattice", **kw):
+    """FENE chains of nmono beads with WCA between all beads, and a second species (whole chains,
+    chosen with a seeded shuffle) whose mutual interaction is Mie 14-6 (eps 1, rc 2.5); species 0
+    pairs and mixed pairs are WCA.  The box holds the FCC lattice of nunits at the density.
+    init "lattice": consecutive lattice sites form the chains (4*nunits[1] divisible by nmono);
+    init "random_walk": seeded random-walk chains (random_walk_chains)."""
+    if init == "lattice" and (4 * nunits[1]) % nmono:
         raise ValueError("4*nunits_y must be divisible by the chain length")
     domain = fcc_globaldomain(nunits, density)
     r = fcc_lattice(nunits, density)
     n = r.shape[0]
+    if init == "random_walk":
+        r = random_walk_chains(domain, n // nmono, nmono, seed)
+    elif init != "lattice":
+        raise ValueError(f"unknown init {init!r}")
     nchains = n // nmono
     v = seeded_velocities(n, 1.0, seed)
     p = make_params(domain, density, 2.5, dt=dt, tplot=tplot, potential_flag=1, k_c=k_c, R_0=R_0, **kw)

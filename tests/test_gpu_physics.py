@@ -482,3 +482,31 @@ def test_queued_rebuild_grows_capacities(monkeypatch):
     assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-10
     f.close()
     g.close()
+
+
+def test_polymer_random_walk_start():
+    """The C4 initial state (seeded random-walk chains) on a small box: neighbour set, bond lists,
+    forces and a 20-step drop-in trajectory against the oracle."""
+    p, r, v, mt, mono = config.config_polymer(nunits=(6, 8, 6), init="random_walk", seed=4)
+    say("case: random-walk FENE + WCA/Mie14-6 melt 1152 beads")
+    o = Oracle(p, r, v)
+    o.set_moltype(mt)
+    o.set_monomers(*mono)
+    g = MDContext(p, r, v)
+    g.set_moltype(mt)
+    g.set_monomers(*mono)
+    o.setup()
+    g.setup(0)
+    oi, oj, os_ = o.neighbourlist()
+    gi, gj, gs = g.neighbourlist()
+    assert np.array_equal(canonical_pairs(oi, oj, os_), canonical_pairs(gi, gj, gs))
+    ob, obs = o.bonds()
+    gb, gbs = g.bonds()
+    canon = lambda b, s: sorted((i, int(x), int(y)) for i in range(b.shape[0]) for x, y in zip(b[i], s[i]) if x >= 0)
+    assert canon(ob, obs) == canon(gb, gbs)
+    o.compute_forces(0)
+    assert rel_force_err(g.state()[2], o.state()[2], o.force_scale()) < TOL
+    ro, rg = drop_in_steps(o, g, 1, 20)
+    assert ro == rg
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-9
+    g.close()
