@@ -301,7 +301,8 @@ def cpu_baseline(nunits, steps):
     """The CPU restatement of the reference (oracle) on this host's cores, two legs:
     * one core on the same 1M-molecule workload (`steps` steps after its setup);
     * every core: C processes at once, each on a 1/C share of the molecules (an FCC box of
-      64 x 64 x 64/C units, the share one rank of a C-rank domain decomposition holds), aggregate
+      64 x 64/cy x 64/cz units with cy*cz = C, the share one rank of a C-rank domain decomposition
+      holds), aggregate
       atom-steps/s = sum of molecules x steps / the slowest process's time.  No halo exchange is
       modelled, so this leg bounds an MPI run of the reference from above."""
     import multiprocessing as mp
@@ -309,9 +310,18 @@ def cpu_baseline(nunits, steps):
     n1, t1 = _oracle_run(((nunits, nunits, nunits), steps, 2))
     one = n1 * steps / t1
     C = host_cores()
-    while C > 1 and nunits % C:
-        C -= 1
-    share = (nunits, nunits, nunits // C)
+    # C = cy * cz sub-boxes, each at least 12 units (4 cells) along every side
+    split = None
+    while C > 1 and split is None:
+        for cy in range(int(math.isqrt(C)), 0, -1):
+            cz = C // cy
+            if cy * cz == C and nunits % cy == 0 and nunits % cz == 0 and nunits // max(cy, cz) >= 12:
+                split = (cy, cz)
+                break
+        if split is None:
+            C -= 1
+    cy, cz = split if split else (1, 1)
+    share = (nunits, nunits // cy, nunits // cz)
     steps_c = max(steps, 10 * (int(1.0e7 / (4 * nunits * nunits * (nunits // C)) + 9) // 10))
     with mp.get_context("spawn").Pool(C) as pool:
         res = pool.map(_oracle_run, [(share, steps_c, 100 + k) for k in range(C)])

@@ -7,8 +7,9 @@ Bars, stated per case:
   * the FP64 leg against the oracle: positions within 1e-9, tether sites within 1e-12, MOP planes
     within 1e-10 relative, identical rebuild steps (the reference's Couette path is leap-frog with
     tag_move, src/md.f90:109-139);
-  * mixed precision against the FP64 path on the same input: pair-force RMS relative error
-    below 1e-5 on the first step; after K steps the MOP profile (Pxy_plane summed over the steps)
+  * mixed precision against the FP64 path on the same input (a sheared liquid, after FP64 steps
+    from the perturbed lattice): pair-force RMS relative error below 1e-5 on the first step;
+    after K steps the MOP profile (Pxy_plane summed over the steps)
     within 1e-4 of the FP64 profile in relative L2 norm, and the position RMS difference below
     1e-5 (the trajectories separate at FP32 rounding and chaos amplifies it slowly over K dt);
   * the fused loop and the packed replica launch repeat the routine-by-routine and the alone runs
@@ -55,6 +56,26 @@ def gpu_ctx(p, r, v, tag, rt):
     return g
 
 
+def sheared(nunits, steps, lfv=True, tplot=1):
+    """The Couette state after `steps` FP64 steps from the perturbed lattice (a sheared liquid
+    between the walls, where the net forces are not the near-cancelling sums of a lattice):
+    FP64 and mixed parameters, positions, velocities, tags and tether sites in the current
+    numbering."""
+    p64, r, v, tag, rt = couette(nunits, 0, lfv=lfv, tplot=tplot)
+    p32, _, _, _, _ = couette(nunits, FM_PREC_MIXED, lfv=lfv, tplot=tplot)
+    g = gpu_ctx(p64, r, v, tag, rt)
+    if lfv:
+        g.run_lfv(0, steps)
+    else:
+        g.run_vv(1, steps)
+    g.rebuild()
+    r, v, _ = g.state()
+    tag = g.labels()[0]
+    rt = g.rtether()
+    g.close()
+    return p64, p32, r, v, tag, rt
+
+
 def rms_rel(x, ref):
     return float(np.sqrt(((x - ref) ** 2).sum() / (ref ** 2).sum()))
 
@@ -95,8 +116,7 @@ def test_c3_mixed_lfv_vs_fp64():
     """Mixed against FP64, both leap-frog with the walls and MOP every step (the record launch of
     the mixed kernel with the MOP tally), 100 steps; and the mixed fused loop against its own
     routine-by-routine sequence."""
-    p64, r, v, tag, rt = couette((12, 12, 24), 0)
-    p32, _, _, _, _ = couette((12, 12, 24), FM_PREC_MIXED)
+    p64, p32, r, v, tag, rt = sheared((12, 12, 24), 200)
     say("case: C3 mixed vs FP64, 13824 molecules, LFV, MOP every step, 100 steps")
     g64 = gpu_ctx(p64, r, v, tag, rt)
     g32 = gpu_ctx(p32, r, v, tag, rt)
@@ -130,8 +150,7 @@ def test_c3_mixed_lfv_vs_fp64():
 def test_c3_mixed_vv_tethers_vs_fp64():
     """Velocity Verlet with the tethers applied with the pair forces (the mixed kernel with
     tethers and the MOP tally on record steps): mixed against FP64 over 50 steps, TPLOT 5."""
-    p64, r, v, tag, rt = couette((12, 12, 12), 0, lfv=False, tplot=5)
-    p32, _, _, _, _ = couette((12, 12, 12), FM_PREC_MIXED, lfv=False, tplot=5)
+    p64, p32, r, v, tag, rt = sheared((12, 12, 12), 200, lfv=False, tplot=5)
     say("case: C3 mixed vs FP64, 6912 molecules, VV with tethers, 50 steps")
     g64 = gpu_ctx(p64, r, v, tag, rt)
     g32 = gpu_ctx(p32, r, v, tag, rt)
@@ -198,6 +217,7 @@ def test_c3_headline_size():
     assert np.abs(Pg - Po).max() <= 1e-10 * np.abs(Po).max()
     del o
     g64.close()
+    p64, p32, r, v, tag, rt = sheared((64, 64, 64), 100)
     g64 = gpu_ctx(p64, r, v, tag, rt)
     g32 = gpu_ctx(p32, r, v, tag, rt)
     e_force = rms_rel(g32.state()[2], g64.state()[2])
