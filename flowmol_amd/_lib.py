@@ -53,6 +53,8 @@ SIGNATURES = {
     "fm_get_cv_bins": (C.c_int, [_vp, _ip, _ip, _dp]),
     "fm_get_cv_stress": (C.c_int, [_vp, _dp, _i64]),
     "fm_reset_cv_stress": (C.c_int, [_vp]),
+    "fm_macros_sums_noalloc": (C.c_int, [_i32, _vp, _vp, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fm_derive_macros": (C.c_int, [_dp, _i32, C.c_double, _i32, C.POINTER(FmMacros)]),
     "fm_run_vv": (C.c_int, [_vp, _i32, _i32, C.POINTER(FmMacros), _i32, _ip]),
     "fm_run_lfv": (C.c_int, [_vp, _i32, _i32, C.POINTER(FmMacros), _i32, _ip]),
     "fm_download_cells": (C.c_int, [_vp, _ip]),

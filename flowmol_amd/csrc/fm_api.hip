@@ -999,6 +999,32 @@ extern "C" int fm_compute_forces(fm_ctx* c, int32_t iter) {
     return 0;
 }
 
+static void derive_macros_np(int32_t np_, double density, int32_t potential_flag, const double* s, fm_macros* o) {
+    memset(o, 0, sizeof *o);
+    const double np = (double)np_;
+    o->potenergysum_LJ = s[0];
+    o->potenergysum_POLY = s[1];
+    o->potenergysum = (potential_flag == 1) ? s[2] : s[0];
+    o->virial = s[3];
+    o->mv2sum = s[4];
+    o->vsum = s[5];
+    o->msum = s[6];
+    o->max_potenergymol = s[7];
+    o->kinenergy = (0.5 * o->mv2sum) / np;
+    o->potenergy = (0.5 * o->potenergysum) / np;
+    o->totenergy = o->kinenergy + o->potenergy;
+    o->temperature = o->mv2sum / (double)(3 * np_);
+    o->pressure = (density / (double)(np_ * 3)) * (o->mv2sum + o->virial / 2.0);
+}
+
+// The macroscopic properties from the eight sums of fm_macros_sums_noalloc (host arithmetic only).
+extern "C" int fm_derive_macros(const double sums[8], int32_t np, double density, int32_t potential_flag,
+                                fm_macros* out) {
+    if (!sums || !out || np < 1) return fm_set_error(FM_EINVAL, "fm_derive_macros: bad arguments");
+    derive_macros_np(np, density, potential_flag, sums, out);
+    return 0;
+}
+
 static void derive_macros(const fm_ctx* c, const double* s, fm_macros* o) {
     memset(o, 0, sizeof *o);
     const double np = (double)c->np;

@@ -9,6 +9,8 @@
 #include <limits.h>
 #include <math.h>
 
+#include <mutex>
+
 #pragma clang fp contract(off)
 
 namespace {
@@ -1119,6 +1121,95 @@ int launch_pass1_check_disp(fm_ctx* c, bool cond, int step) {
 int launch_pass1_multi(hipStream_t st, const P1Desc* d_descs, const int2* d_table, int nblk) {
     if (nblk == 0) return 0;
     k_pass1_multi<<<nblk, FM_BLOCK, 0, st>>>(d_descs, d_table);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// =============================================================================================
+// Record-step sums of caller-held device arrays without any allocation (fm_macros_sums_noalloc).
+// The partial sums live in a file-scope device array; each device's copy is found once with
+// hipGetSymbolAddress and its address is what the kernels receive.
+#define FM_NOALLOC_MAX_NP (1 << 24)
+#define FM_NOALLOC_BLOCKS ((FM_NOALLOC_MAX_NP + FM_BLOCK - 1) / FM_BLOCK)
+static __device__ __align__(256) double scratch[FM_NOALLOC_BLOCKS * NRED];
+
+namespace {
+// the sums of k_macros_partial for (3,np) arrays in molecule order; a (leap-frog) shifts v by
+// half a step, m = per-molecule masses (null: 1)
+__global__ __launch_bounds__(FM_BLOCK) void k_macros_partial_aos(const double* v, const double* a, double dt,
+                                                                 const double* pe_lj, const double* pe_poly,
+                                                                 const double* vir, const double* mass, int np,
+                                                                 double* part) {
+    __shared__ double red[NRED][FM_BLOCK / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s[NRED] = {0, 0, 0, 0, 0, 0, 0, -1.0e300};
+    if (i < np) {
+        const double m = mass ? mass[i] : 1.0;
+        const double pl = pe_lj[i], pp = pe_poly ? pe_poly[i] : 0.0;
+        s[0] = pl;
+        s[1] = pp;
+        s[2] = pl + pp;
+        s[3] = vir[i];
+        double vx = v[3 * (size_t)i], vy = v[3 * (size_t)i + 1], vz = v[3 * (size_t)i + 2];
+        if (a) {
+            vx = vx + 0.5 * a[3 * (size_t)i] * dt;
+            vy = vy + 0.5 * a[3 * (size_t)i + 1] * dt;
+            vz = vz + 0.5 * a[3 * (size_t)i + 2] * dt;
+        }
+        s[4] = (m * (vx * vx) + m * (vy * vy)) + m * (vz * vz);
+        s[5] = (m * vx + m * vy) + m * vz;
+        s[6] = m;
+        s[7] = pl + pp;
+    }
+    for (int q = 0; q < NRED; ++q) {
+        double x = s[q];
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_down(x, off, 64);
+            x = q == 7 ? fmax(x, o) : x + o;
+        }
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NRED) {
+        const int q = threadIdx.x;
+        double x = red[q][0];
+        for (int w = 1; w < FM_BLOCK / 64; ++w) x = q == 7 ? fmax(x, red[q][w]) : x + red[q][w];
+        part[(size_t)blockIdx.x * NRED + q] = x;
+    }
+}
+}  // namespace
+
+// device address of `scratch` per device, looked up at each device's first call
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[64] = {nullptr};
+
+extern "C" int fm_macros_sums_noalloc(int32_t np, const double* v, const double* a, double delta_t,
+                                      const double* potenergymol_LJ, const double* potenergymol_POLY,
+                                      const double* virialmol, const double* mass, double* sums, void* stream) {
+    if (np < 1 || np > FM_NOALLOC_MAX_NP)
+        return fm_set_error(FM_EINVAL, "fm_macros_sums_noalloc: np %d outside 1..%d (the size of its static scratch)",
+                            np, FM_NOALLOC_MAX_NP);
+    if (!v || !potenergymol_LJ || !virialmol || !sums) return fm_set_error(FM_EINVAL, "fm_macros_sums_noalloc: null array");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        return fm_set_error(FM_EHIP, "fm_macros_sums_noalloc: no current device");
+    double* ptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return fm_set_error(FM_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            g_scratch_ptr[dev] = (double*)p;
+        }
+        ptr = g_scratch_ptr[dev];
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    const int nb = (np + FM_BLOCK - 1) / FM_BLOCK;
+    k_macros_partial_aos<<<nb, FM_BLOCK, 0, st>>>(v, a, delta_t, potenergymol_LJ, potenergymol_POLY, virialmol, mass,
+                                                 np, ptr);
+    k_macros_final<<<1, 256, 0, st>>>(ptr, nb, sums);
     LAUNCH_CHECK();
     return 0;
 }

@@ -235,6 +235,22 @@ int  fm_get_cv_bins(const fm_ctx* ctx, int32_t nbins[3], int32_t nhb[3], double 
 int  fm_get_cv_stress(fm_ctx* ctx, double* pxyface, int64_t n);
 int  fm_reset_cv_stress(fm_ctx* ctx);
 
+/* ---- record-step sums without allocation ----------------------------------------------------
+   No counterpart in the reference: a variant of fm_evaluate_macroscopic_properties for callers
+   that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  It takes caller-held device arrays in molecule order
+   (v, and a for the leap-frog half-step shift or NULL; (3,np) like arrays_MD), the per-molecule
+   potenergymol_LJ, potenergymol_POLY (or NULL), virialmol and masses (or NULL for 1), and writes
+   the eight sums (PE LJ, PE POLY, PE, virial, sum m v^2, sum m v, sum m, max PE) to sums[8] in
+   device memory, stream-ordered on `stream` (a hipStream_t, NULL for the default stream), with no
+   host synchronisation.  Its partial sums use a static device buffer for at most 16,777,216
+   molecules (larger np is refused); every call on one device shares it, so calls that overlap on
+   one device race.  fm_derive_macros turns the sums into fm_macros on the host. */
+int  fm_macros_sums_noalloc(int32_t np, const double* v, const double* a, double delta_t,
+                            const double* potenergymol_LJ, const double* potenergymol_POLY,
+                            const double* virialmol, const double* mass, double* sums, void* stream);
+int  fm_derive_macros(const double sums[8], int32_t np, double density, int32_t potential_flag, fm_macros* out);
+
 /* ---- fused time loop: md_advance_vv (src/md.f90:143-163) nsteps times from iter0, without a
    host round trip per step.  Same kernels and arithmetic as the call-per-routine sequence.
    If macros != NULL, record-step macros are written to macros[k] for k-th record step
