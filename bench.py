@@ -201,7 +201,8 @@ def c4_main(a):
         "steps": K, "warmup": W, "ms_per_step": dt / K * 1e3, "rebuilds_in_timed_steps": nrb,
         "dtype": "f64", "data": "synthetic", "vs_baseline": None,
         "config": {"workload": "FENE k 30 R0 1.5, 65,536 chains of 8 beads, random-walk initial state from seed 4 "
-                               "(bond 0.97, non-bonded separation 0.8) in the box of FCC 64x32x64 at density 0.85, "
+                               "(bond 0.97; close non-bonded pairs relaxed to >= 0.8) in the box of FCC 64x32x64 "
+                               "at density 0.85, "
                                "WCA all pairs, Mie 14-6 (rc 2.5) within species 1 (half the chains), dt 0.002, "
                                "TPLOT 10, displacement rebuild", "molecules": N,
                    "init_generation_s": round(t_gen, 1)},

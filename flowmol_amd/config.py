@@ -260,20 +260,70 @@ def random_walk_chains(domain, nchains, nmono, seed, bond=0.97, min_sep=0.8, max
     return np.ascontiguousarray(wrap(pos.reshape(ntot, 3)))
 
 
+def relax_overlaps(r, domain, nmono, bond=0.97, target=0.85, floor=0.8, max_iter=200):
+    """Move beads of chains (nmono consecutive rows each) until no two non-bonded beads are closer
+    than `floor`: every non-bonded pair closer than `target` is pushed apart symmetrically to
+    `target`, then the bonds are pulled back towards `bond` (a few symmetric sweeps), repeatedly.
+    Ends with every bond within 0.05 of `bond`; raises if max_iter sweeps do not get there."""
+    from scipy.spatial import cKDTree
+
+    L = np.asarray(domain, np.float64)
+    r = np.array(r, dtype=np.float64)
+    n = r.shape[0]
+    bi = np.arange(n - 1)
+    bi = bi[(bi % nmono) != nmono - 1]  # bead i bonded to i+1
+
+    def wrap(x):
+        return x - L * np.floor((x + 0.5 * L) / L)
+
+    def min_image(d):
+        return d - L * np.round(d / L)
+
+    for _ in range(max_iter):
+        pairs = cKDTree(wrap(r) + 0.5 * L, boxsize=L).query_pairs(target, output_type="ndarray")
+        if pairs.size:
+            i, j = pairs[:, 0], pairs[:, 1]
+            keep = ~((j == i + 1) & ((i % nmono) != nmono - 1))
+            i, j = i[keep], j[keep]
+        else:
+            i = j = np.zeros(0, np.int64)
+        d = min_image(r[i] - r[j])
+        dist = np.linalg.norm(d, axis=1)
+        bd = np.linalg.norm(min_image(r[bi + 1] - r[bi]), axis=1)
+        if (dist.size == 0 or dist.min() >= floor) and np.abs(bd - bond).max() <= 0.05:
+            return wrap(r)
+        push = (0.5 * (target - dist) / np.maximum(dist, 1e-12))[:, None] * d
+        dr = np.zeros_like(r)
+        np.add.at(dr, i, push)
+        np.add.at(dr, j, -push)
+        r = r + dr
+        for _ in range(4):
+            d = min_image(r[bi + 1] - r[bi])
+            ln = np.linalg.norm(d, axis=1)
+            c = (0.5 * (ln - bond) / ln)[:, None] * d
+            db = np.zeros_like(r)
+            np.add.at(db, bi, c)
+            np.add.at(db, bi + 1, -c)
+            r = r + db
+    raise RuntimeError("overlaps remain after relaxation")
+
+
 def config_polymer(nunits=(6, 8, 6), density=0.85, nmono=8, seed=4, k_c=30.0, R_0=1.5, dt=0.002,
                    mie_fraction=0.5, tplot=10, init="lattice", **kw):
     """FENE chains of nmono beads with WCA between all beads, and a second species (whole chains,
     chosen with a seeded shuffle) whose mutual interaction is Mie 14-6 (eps 1, rc 2.5); species 0
     pairs and mixed pairs are WCA.  The box holds the FCC lattice of nunits at the density.
     init "lattice": consecutive lattice sites form the chains (4*nunits[1] divisible by nmono);
-    init "random_walk": seeded random-walk chains (random_walk_chains)."""
+    init "random_walk": seeded random-walk chains (random_walk_chains), then the close non-bonded
+    pairs its densest spots leave pushed apart to at least 0.8 (relax_overlaps): a start a FENE
+    chain survives (pairs nearer than that release enough energy to stretch bonds past R0)."""
     if init == "lattice" and (4 * nunits[1]) % nmono:
         raise ValueError("4*nunits_y must be divisible by the chain length")
     domain = fcc_globaldomain(nunits, density)
     r = fcc_lattice(nunits, density)
     n = r.shape[0]
     if init == "random_walk":
-        r = random_walk_chains(domain, n // nmono, nmono, seed)
+        r = relax_overlaps(random_walk_chains(domain, n // nmono, nmono, seed), domain, nmono)
     elif init != "lattice":
         raise ValueError(f"unknown init {init!r}")
     nchains = n // nmono

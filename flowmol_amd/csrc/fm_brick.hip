@@ -964,28 +964,34 @@ __device__ __forceinline__ void lj_word_rec(const P4& p, int left, double xi, do
 }
 
 // Mixed precision: the FP64 coordinate difference converted to FP32, then the same law in FP32
-// (rcp without refinement); an entry that does not count selects 1/r^2 = 0.
-__device__ __forceinline__ void lj_word_mixed(const P4& p, int left, double xi, double yi, double zi, float rc2,
+// (rcp without refinement); an entry that does not count selects 1/r^2 = 0.  Whether a pair
+// counts is decided on the FP64 r^2, as the FP64 path decides it: an FP32 test would move pairs
+// within rounding of rc across the cutoff, a discrete error of the whole pair force.
+__device__ __forceinline__ void lj_word_mixed(const P4& p, int left, double xi, double yi, double zi, double rc2,
                                               float& fx, float& fy, float& fz) {
-#pragma clang fp contract(fast)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const float rx = (float)(xi - p.x[q]), ry = (float)(yi - p.y[q]), rz = (float)(zi - p.z[q]);
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        float x = __builtin_amdgcn_rcpf(r2);
-        x = (r2 < rc2 && q < left) ? x : 0.0f;
-        const float x2 = x * x;
-        const float acc = (x2 * x2) * (2.0f * (x2 * x) - 1.0f);
-        fx = fx + acc * rx;
-        fy = fy + acc * ry;
-        fz = fz + acc * rz;
+        const double dx = xi - p.x[q], dy = yi - p.y[q], dz = zi - p.z[q];
+        const bool in = ((dx * dx + dy * dy) + dz * dz) < rc2 && q < left;  // FP64, no contraction
+        {
+#pragma clang fp contract(fast)
+            const float rx = (float)dx, ry = (float)dy, rz = (float)dz;
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            float x = __builtin_amdgcn_rcpf(r2);
+            x = in ? x : 0.0f;
+            const float x2 = x * x;
+            const float acc = (x2 * x2) * (2.0f * (x2 * x) - 1.0f);
+            fx = fx + acc * rx;
+            fy = fy + acc * ry;
+            fz = fz + acc * rz;
+        }
     }
 }
 template <typename T>
 __device__ __forceinline__ void lj_word_t(const P4& p, int left, double xi, double yi, double zi, double rc2, T& fx,
                                           T& fy, T& fz) {
     if constexpr (sizeof(T) == 8) lj_word(p, left, xi, yi, zi, rc2, fx, fy, fz);
-    else lj_word_mixed(p, left, xi, yi, zi, (float)rc2, fx, fy, fz);
+    else lj_word_mixed(p, left, xi, yi, zi, rc2, fx, fy, fz);
 }
 
 // MIXED: the pair law and the per-molecule sum in FP32 (lj_word_mixed), the result in FP64.
@@ -1361,7 +1367,6 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
         __syncthreads();
     }
     const size_t cap = a.cap;
-    const float rc2 = (float)a.rc2;
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
         int rci, li, i;
         imol(r, a.start, q, rci, li, i);
@@ -1390,12 +1395,13 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
             if (kw + q >= n) break;
             const int j = jq[q];
             const double rxd = rxq[q], ryd = ryq[q], rzd = rzq[q];
+            // the cutoff decided on the FP64 r^2, as the FP64 path decides it
+            if (!(((rxd * rxd + ryd * ryd) + rzd * rzd) < a.rc2)) continue;
             const float rx = (float)rxd, ry = (float)ryd, rz = (float)rzd;
             float acc, inv;
             {
 #pragma clang fp contract(fast)
                 const float r2 = rx * rx + ry * ry + rz * rz;
-                if (!(r2 < rc2)) continue;
                 inv = __builtin_amdgcn_rcpf(r2);
                 const float i2 = inv * inv, i4 = i2 * i2, i3 = i2 * inv;
                 acc = 48.f * (i4 * i3 - 0.5f * i4);

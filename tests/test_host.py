@@ -31,10 +31,10 @@ def test_brick_candidates_at_the_1m_workload():
 
 
 def test_random_walk_chains():
-    """Seeded random-walk chains: every bond exactly 0.97 long (minimum image), chain starts on
-    distinct lattice sites, non-bonded beads at least the requested separation apart except those
-    placed with a lowered one (about 3 % of the pairs at density 0.85, none below 0.6), inside
-    the box, and the same chains for the same seed."""
+    """Seeded random-walk chains: raw walks have every bond exactly 0.97 long (minimum image);
+    after the overlap relaxation of the initial state no two non-bonded beads are closer than
+    0.8 and every bond is within 0.05 of 0.97; beads inside the box; the same chains for the
+    same seed."""
     from scipy.spatial import cKDTree
 
     from flowmol_amd import config
@@ -43,15 +43,19 @@ def test_random_walk_chains():
     L = np.array([p.domain[0], p.domain[1], p.domain[2]])
     n = r.shape[0]
     assert n == 2048 and np.all(r >= -0.5 * L) and np.all(r < 0.5 * L)
-    d = r[1:] - r[:-1]
-    d -= L * np.round(d / L)
-    bl = np.linalg.norm(d, axis=1)
+    raw = config.random_walk_chains(L, n // 8, 8, 9)
     bonded = (np.arange(n - 1) % 8) != 7
-    assert np.abs(bl[bonded] - 0.97).max() < 1e-12
-    pairs = cKDTree(r + 0.5 * L, boxsize=L).query_pairs(0.8 - 1e-9, output_type="ndarray")
+
+    def bonds(x):
+        d = x[1:] - x[:-1]
+        d -= L * np.round(d / L)
+        return np.linalg.norm(d, axis=1)[bonded]
+
+    assert np.abs(bonds(raw) - 0.97).max() < 1e-12
+    assert np.abs(bonds(r) - 0.97).max() <= 0.05
+    pairs = cKDTree(r + 0.5 * L, boxsize=L).query_pairs(0.8, output_type="ndarray")
     close = [(i, j) for i, j in pairs if not (j == i + 1 and i % 8 != 7)]
-    assert len(close) <= 0.05 * n
-    assert all(np.linalg.norm((r[i] - r[j]) - L * np.round((r[i] - r[j]) / L)) > 0.6 for i, j in close)
+    assert not close
     chain, sub, funcy, bflag = mono
     assert list(sub[:8]) == list(range(1, 9)) and list(funcy[:8]) == [1, 2, 2, 2, 2, 2, 2, 1]
     _, r2, _, _, _ = config.config_polymer(nunits=(8, 8, 8), init="random_walk", seed=9)
