@@ -563,6 +563,11 @@ struct NlqArgs {
     int lo, hi;
     double cs[3], hd[3];
     const int* order;
+    // bond discovery (POLY): monomer fields in slot order, the bond lists, molecule numbers
+    const int* mono;
+    uint32_t* bond;
+    int* bondn;
+    const int* ext;
 };
 
 __device__ inline int quantise(double x, double o, double inv_h) {
@@ -571,7 +576,11 @@ __device__ inline int quantise(double x, double o, double inv_h) {
 }
 
 // registers held to what lets three workgroups share a CU (its LDS region is a third of the
-// force kernels')
+// force kernels').  POLY: the bonds of the chains are found among the list entries (a bonded
+// partner lies within R0 <= r_n, so it is an entry), as check_update_adjacentbeadinfo finds them
+// during the reference's scan (src/linklist.f90:2542-2584), from the chain ids staged with the
+// positions.
+template <bool POLY>
 __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(3 * FB_BLOCK / 256))) void
 k_nlist_quant(NlqArgs a) {
     extern __shared__ __align__(16) char smem[];
@@ -582,6 +591,13 @@ k_nlist_quant(NlqArgs a) {
     r.pos = nullptr;
     r.lslot = nullptr;
     int* ip = ls + a.maxreg;
+    int* lchain = nullptr;                  // POLY: chain and subchain id of each staged molecule
+    int* lsub = nullptr;
+    if (POLY) {
+        lchain = ip;
+        lsub = lchain + a.maxreg;
+        ip = lsub + a.maxreg;
+    }
     r.rq = (uint16_t*)ip;
     ip += (a.maxreg + 1) / 2;
     r.off = ip;
@@ -602,6 +618,7 @@ k_nlist_quant(NlqArgs a) {
     const double oy = -a.hd[1] + (by * g.B[1] - 1) * a.cs[1] - 2.0 * a.h;
     const double oz = -a.hd[2] + (bz * g.B[2] - 1) * a.cs[2] - 2.0 * a.h;
     constexpr int SU = 6;  // molecules per thread and pass, loads first (as stage_region)
+    const size_t cap0 = a.cap;
     for (int q0 = threadIdx.x; q0 < r.total; q0 += SU * blockDim.x) {
         int sl[SU], sc[SU];
         double px[SU], py[SU], pz[SU];
@@ -626,6 +643,10 @@ k_nlist_quant(NlqArgs a) {
             const int zq = quantise(img(pz[k], sz, g.L[2]), oz, a.inv_h);
             qv[q] = make_uint2((uint32_t)xq | ((uint32_t)yq << 16), (uint32_t)zq);
             ls[q] = (sl[k] << 5) | sc[k];
+            if (POLY) {
+                lchain[q] = a.mono[sl[k]];
+                lsub[q] = a.mono[sl[k] + cap0];
+            }
         }
     }
     __syncthreads();
@@ -646,6 +667,11 @@ k_nlist_quant(NlqArgs a) {
             const uint2 wi = qv[li];
             const s16x2 ixy = __builtin_bit_cast(s16x2, wi.x);
             const int izq = (int)wi.y;
+            int nb = 0, chain = 0, funcy = 0;
+            if (POLY) {
+                chain = lchain[li];
+                funcy = a.mono[slot + 2 * cap];
+            }
             // faces of the molecule's cell in y and z, as in k_nlist_brick
             const int vy = by * g.B[1] + ry - 1, vz = bz * g.B[2] + rz - 1;
             const double ylo = -a.hd[1] + vy * a.cs[1], zlo = -a.hd[2] + vz * a.cs[2];
@@ -743,11 +769,19 @@ k_nlist_quant(NlqArgs a) {
                         word = (word >> 16) | ((uint64_t)(uint32_t)j << 48);
                         if ((cnt & 3) == 3 && cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word;
                         ++cnt;
+                        if (POLY && lchain[j] == chain && get_bondflag(a.mono, a.cap, slot, lsub[j])) {
+                            if (nb < funcy && nb < FM_MAX_FUNCY) a.bond[(size_t)nb * cap + slot] = (uint32_t)ls[j];
+                            ++nb;
+                        }
                     }
                 }
             }
             if ((cnt & 3) != 0 && cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word >> (16 * (4 - (cnt & 3)));
             a.nbn[slot] = cnt < a.maxnb ? cnt : a.maxnb;
+            if (POLY) {
+                a.bondn[slot] = nb < funcy ? nb : funcy;
+                if (nb > funcy) set_err(a.flags, FM_EBOND, a.ext[slot]);
+            }
         }
         int m = active ? cnt : 0;
         for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
@@ -1291,6 +1325,127 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
     }
 }
 
+// pressure_tensor_forces_MOP for one list entry of the record pass below, with the same
+// contributions as mop_entry: the reference loops over the planes between the ones nearest to
+// ri(2) and rj(2) (nint of a division); only planes strictly between the two molecules add (the
+// sign factor is zero elsewhere), so the loop here runs over that range widened by one plane on
+// each side, found with a multiplication, and the sign factor decides exactly as the reference's.
+// The intercept factors are 1 whenever both molecules lie strictly inside the box in x and z
+// (the intercept lies between them); otherwise they are computed with the reference's divisions.
+__device__ inline void mop_entry_fast(const FbArgs& a, const double ri[3], const double rj[3], const double rij[3],
+                                      double acc, double* spxy, const double* planes, double inv_spacing,
+                                      bool inside) {
+    const double ti = (ri[1] + a.hp - a.shift) * inv_spacing, tj = (rj[1] + a.hp - a.shift) * inv_spacing;
+    int n0 = (int)floor(fmin(ti, tj) + 0.5), n1 = (int)floor(fmax(ti, tj) + 0.5) + 2;
+    n0 = n0 < 1 ? 1 : n0;
+    n1 = n1 > a.nplanes ? a.nplanes : n1;
+    for (int n = n0; n <= n1; ++n) {
+        const double plane = planes[n - 1];
+        const double fx = copysign(1.0, ri[1] - plane) - copysign(1.0, rj[1] - plane);
+        if (fx == 0.0) continue;
+        double hxx = 1.0, hzz = 1.0;
+        if (!inside) {
+            const double pyb0 = ri[0] + (rij[0] / rij[1]) * (plane - ri[1]);
+            const double pyb2 = ri[2] + (rij[2] / rij[1]) * (plane - ri[1]);
+            hxx = dp_heaviside(a.hx - pyb0) - dp_heaviside(-a.hx - pyb0);
+            hzz = dp_heaviside(a.hz - pyb2) - dp_heaviside(-a.hz - pyb2);
+            if (hxx == 0.0 || hzz == 0.0) continue;
+        }
+        for (int d = 0; d < 3; ++d) atomicAdd(&spxy[3 * (n - 1) + d], acc * rij[d] * fx * hxx * hzz);
+    }
+}
+
+// Record step of an LJ domain with the MOP tally and nothing else (no Mie tables, bonds, tethers
+// in the forces, CV faces or rfmol): the forces through the branch-free loop of plain steps
+// (MIXED: its FP32 pair law), then a second pass over the list for the tallies, each entry in
+// FP64 as the reference's loop computes it (src/simulation_compute_forces.f90:683-701): PE, the
+// virial, and the MOP planes of the entries the reference visits (mop_owner).
+template <bool MIXED>
+__device__ __forceinline__ void forces_body_mop(const FbArgs& a, int brick, char* smem) {
+    Region r;
+    carve(r, smem, a.g, a.maxreg, true);
+    // after the region: the plane tallies, the plane positions, and each region cell's
+    // coordinates packed x | y << 10 | z << 20 (the owner rule without integer divisions)
+    double* spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
+    double* planes = spxy + 3 * a.nplanes;
+    int* rcxyz = (int*)(planes + a.nplanes);
+    for (int t = threadIdx.x; t < a.nplanes; t += blockDim.x) planes[t] = a.planes[t];
+    for (int t = threadIdx.x; t < a.g.nreg; t += blockDim.x)
+        rcxyz[t] = (t % a.g.R[0]) | (((t / a.g.R[0]) % a.g.R[1]) << 10) | ((t / (a.g.R[0] * a.g.R[1])) << 20);
+    if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags,
+                      nullptr, a.ext))
+        return;
+    for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
+    __syncthreads();
+    const size_t cap = a.cap;
+    const double inv_spacing = 1.0 / a.spacing;
+    const double bx = a.hx * (1.0 - 1e-12), bz = a.hz * (1.0 - 1e-12);
+    for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
+        int rci, li, i;
+        imol(r, a.start, q, rci, li, i);
+        const double xi = r.pos[3 * li], yi = r.pos[3 * li + 1], zi = r.pos[3 * li + 2];
+        const int n = a.nbn[i];
+        const uint64_t* row = a.nbl + i;
+        double fx, fy, fz;
+        lj_plain<MIXED>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz);
+        double pe = 0.0, vir = 0.0;
+        const int exti = a.ext[i];
+        const bool in_i = fabs(xi) < bx && fabs(zi) < bz;
+        const bool shi = r.sh[rci] != 13;
+        const int ci = rcxyz[rci];
+        uint64_t wnext = n > 0 ? row[0] : 0;
+        for (int kw = 0; kw < n; kw += 4) {
+            const uint64_t word = wnext;
+            if (kw + 4 < n) wnext = row[(size_t)((kw >> 2) + 1) * cap];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (kw + e >= n) break;
+                const int j = (int)((word >> (16 * e)) & 0xffffu);
+                const double xj = r.pos[3 * j], yj = r.pos[3 * j + 1], zj = r.pos[3 * j + 2];
+                const double rx = xi - xj, ry = yi - yj, rz = zi - zj;
+                const double r2 = (rx * rx + ry * ry) + rz * rz;
+                if (!(r2 < a.rc2)) continue;
+                const double inv = recip(r2);
+                const double acc = 48.0 * (p7(inv) - 0.5 * p4(inv));
+                pe = pe + (4.0 * (p6(inv) - p3(inv)) - a.potshift);
+                vir = vir + acc * r2;
+                // mop_owner: entries against an image always; unshifted pairs once, from the
+                // molecule whose cell scans the other's as a forward cell, or in one cell from the
+                // higher molecule number
+                const int rcj = region_cell_of(r, a.g.nreg, j);
+                bool own = shi || r.sh[rcj] != 13;
+                if (!own) {
+                    if (rcj == rci) {
+                        own = exti > r.lslot[j];
+                    } else {
+                        const int cj = rcxyz[rcj];
+                        own = is_forward((cj & 1023) - (ci & 1023), ((cj >> 10) & 1023) - ((ci >> 10) & 1023),
+                                         (cj >> 20) - (ci >> 20));
+                    }
+                }
+                if (own) {
+                    const double ri[3] = {xi, yi, zi}, rj[3] = {xj, yj, zj}, rij[3] = {rx, ry, rz};
+                    mop_entry_fast(a, ri, rj, rij, acc, spxy, planes, inv_spacing,
+                                   in_i && fabs(xj) < bx && fabs(zj) < bz);
+                }
+            }
+        }
+        a.s.ax[i] = fx;
+        a.s.ay[i] = fy;
+        a.s.az[i] = fz;
+        if (a.kick) {
+            a.s.vx[i] = a.s.vx[i] + a.hdt * fx;
+            a.s.vy[i] = a.s.vy[i] + a.hdt * fy;
+            a.s.vz[i] = a.s.vz[i] + a.hdt * fz;
+        }
+        a.vir[i] = vir;
+        a.pe_lj[i] = pe;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x)
+        if (spxy[t] != 0.0) atomicAdd(&a.pxy[t], spxy[t]);
+}
+
 // A conditional launch does nothing when a rebuild is pending for its own step or an earlier one,
 // and passes the halt on to the launches queued after it.  A force launch that carries the next
 // step's pass 1 may itself raise FLAG_REBUILD for that next step while its later workgroups are
@@ -1323,6 +1478,16 @@ k_forces_brick_lj(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt, a.step)) return;
     forces_body<false, RECORD, false, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+}
+
+// record steps with the MOP tally only (forces_body_mop): two workgroups per CU, as the LDS of
+// two regions allows
+template <bool MIXED>
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128))) void
+k_forces_brick_mop(FbArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    if (halted(a.halt, a.step)) return;
+    forces_body_mop<MIXED>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // LJ steps without tallies, bonds or tethers (the common step): registers held to what lets two
@@ -1568,7 +1733,10 @@ size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind) {
     const bool mie = c->p.mie_potential == 1;
     const size_t tab = mie ? FM_MIE_WORDS * sizeof(double) : 0;  // Mie tables
     if (kind == 1) return region_lds_bytes(g, maxreg, mie || c->p.vflux_outflag == 4, mie) + tab;
-    return region_lds_bytes(g, maxreg, true, mie) + 3 * (size_t)c->p.nplanes * sizeof(double) + tab;
+    // kind 2: the plane tallies, and for the MOP record kernel the plane positions and the packed
+    // region-cell coordinates
+    return region_lds_bytes(g, maxreg, true, mie) + 4 * (size_t)c->p.nplanes * sizeof(double) +
+           (size_t)g.nreg * sizeof(int) + tab;
 }
 
 // measure the largest region (after binning); result in d_flags[FLAG_MAX_REGION]
@@ -1605,9 +1773,13 @@ int launch_nlist(fm_ctx* c) {
         a.cs[d] = c->cs[d];
         a.hd[d] = c->half[d];
     }
-    if (!a.mono && a.xsorted && c->cap < (1 << 26)) {
+    if (a.xsorted && c->cap < (1 << 26)) {
         // quantised prefilter (slots packed with the shift code in 31 bits)
         NlqArgs q;
+        q.mono = a.mono;
+        q.bond = a.bond;
+        q.bondn = a.bondn;
+        q.ext = a.ext;
         q.s = a.s;
         q.g = g;
         q.start = a.start;
@@ -1634,10 +1806,11 @@ int launch_nlist(fm_ctx* c) {
         q.lo = (int)floor(t * t - 2.0 * r3 * eps * t - 1e-9 * t * t) - 2;
         q.hi = (int)ceil(t * t * (1.0 + 3e-9) + 2.0 * r3 * eps * t * (1.0 + 1e-9) + 3.0 * eps * eps) + 2;
         const int nbmax = g.B[0] * g.B[1] * g.B[2];
-        const size_t lds = (size_t)c->maxreg * (sizeof(uint2) + sizeof(int)) +
+        const size_t lds = (size_t)c->maxreg * (sizeof(uint2) + sizeof(int) + (a.mono ? 2 * sizeof(int) : 0)) +
                            (size_t)((c->maxreg + 1) / 2 + 4 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int);
         ktimer_begin(c, KT_NLIST);
-        k_nlist_quant<<<brick_count(c), FB_BLOCK, lds, c->stream>>>(q);
+        if (a.mono) k_nlist_quant<true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(q);
+        else k_nlist_quant<false><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(q);
         LAUNCH_CHECK();
         ktimer_end(c, KT_NLIST);
         return 0;
@@ -1761,6 +1934,14 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
     }
     ktimer_begin(c, KT_FORCES);
     dim3 grid(brick_count(c)), blk(FB_BLOCK);
+    // a record step whose only tally beyond PE and virial is MOP: the two-pass kernel
+    if ((kind == 0 || kind == 1) && a.rec && a.pxy != nullptr && a.cv == nullptr && a.rf == nullptr) {
+        if (kind == 1) k_forces_brick_mop<true><<<grid, blk, lds, c->stream>>>(a);
+        else k_forces_brick_mop<false><<<grid, blk, lds, c->stream>>>(a);
+        ktimer_end(c, KT_FORCES);
+        LAUNCH_CHECK();
+        return 0;
+    }
 #define FB(M, R, E) \
     (R ? k_forces_brick<M, true, E, true><<<grid, blk, lds, c->stream>>>(a) \
        : (a.ev ? k_forces_brick<M, false, E, true><<<grid, blk, lds, c->stream>>>(a) \
