@@ -1326,19 +1326,17 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
 }
 
 // pressure_tensor_forces_MOP for one list entry of the record pass below, with the same
-// contributions as mop_entry: the reference loops over the planes between the ones nearest to
-// ri(2) and rj(2) (nint of a division); only planes strictly between the two molecules add (the
-// sign factor is zero elsewhere), so the loop here runs over that range widened by one plane on
-// each side, found with a multiplication, and the sign factor decides exactly as the reference's.
-// The intercept factors are 1 whenever both molecules lie strictly inside the box in x and z
-// (the intercept lies between them); otherwise they are computed with the reference's divisions.
+// contributions as mop_entry.  The reference loops over the planes between the ones nearest to
+// ri(2) and rj(2); only planes strictly between the two molecules add (the sign factor is zero
+// elsewhere).  With t = (y + halfdomain - shift)/planespacing, plane n sits at t = n - 1, so the
+// candidates here are k0 <= n - 1 <= k1, the whole numbers between t_i and t_j widened by 1e-9
+// (the t's come from a multiplication, within 1e-14 of the quotient), and the sign factor decides
+// exactly as the reference's.  The intercept factors are 1 whenever both molecules lie strictly
+// inside the box in x and z (the intercept lies between them); otherwise they are computed with
+// the reference's divisions.
 __device__ inline void mop_entry_fast(const FbArgs& a, const double ri[3], const double rj[3], const double rij[3],
-                                      double acc, double* spxy, const double* planes, double inv_spacing,
-                                      bool inside) {
-    const double ti = (ri[1] + a.hp - a.shift) * inv_spacing, tj = (rj[1] + a.hp - a.shift) * inv_spacing;
-    int n0 = (int)floor(fmin(ti, tj) + 0.5), n1 = (int)floor(fmax(ti, tj) + 0.5) + 2;
-    n0 = n0 < 1 ? 1 : n0;
-    n1 = n1 > a.nplanes ? a.nplanes : n1;
+                                      double acc, double* spxy, const double* planes, int k0, int k1, bool inside) {
+    const int n0 = k0 < 0 ? 1 : k0 + 1, n1 = k1 >= a.nplanes ? a.nplanes : k1 + 1;
     for (int n = n0; n <= n1; ++n) {
         const double plane = planes[n - 1];
         const double fx = copysign(1.0, ri[1] - plane) - copysign(1.0, rj[1] - plane);
@@ -1356,10 +1354,11 @@ __device__ inline void mop_entry_fast(const FbArgs& a, const double ri[3], const
 }
 
 // Record step of an LJ domain with the MOP tally and nothing else (no Mie tables, bonds, tethers
-// in the forces, CV faces or rfmol): the forces through the branch-free loop of plain steps
-// (MIXED: its FP32 pair law), then a second pass over the list for the tallies, each entry in
-// FP64 as the reference's loop computes it (src/simulation_compute_forces.f90:683-701): PE, the
-// virial, and the MOP planes of the entries the reference visits (mop_owner).
+// in the forces, CV faces or rfmol): the forces through the branch-free loop of plain steps (FP64
+// with PE and virial; MIXED: its FP32 pair law), then a second pass over the list for the MOP
+// planes of the entries the reference visits (mop_owner) that cross a plane, in FP64 as the
+// reference's loop computes them (src/simulation_compute_forces.f90:683-701), and with MIXED the
+// FP64 PE and virial of every entry inside rc.
 template <bool MIXED>
 __device__ __forceinline__ void forces_body_mop(const FbArgs& a, int brick, char* smem) {
     Region r;
@@ -1387,28 +1386,50 @@ __device__ __forceinline__ void forces_body_mop(const FbArgs& a, int brick, char
         const int n = a.nbn[i];
         const uint64_t* row = a.nbl + i;
         double fx, fy, fz;
-        lj_plain<MIXED>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz);
         double pe = 0.0, vir = 0.0;
+        // FP64: PE and virial with the forces in the branch-free loop; the pass below only tallies
+        // MOP.  Mixed: FP32 forces there, the FP64 PE and virial below.
+        if constexpr (MIXED) lj_plain<true>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz);
+        else lj_plain<false, true>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz, a.potshift, &pe, &vir);
         const int exti = a.ext[i];
         const bool in_i = fabs(xi) < bx && fabs(zi) < bz;
         const bool shi = r.sh[rci] != 13;
         const int ci = rcxyz[rci];
+        const double ti = (yi + a.hp - a.shift) * inv_spacing;
         uint64_t wnext = n > 0 ? row[0] : 0;
         for (int kw = 0; kw < n; kw += 4) {
             const uint64_t word = wnext;
             if (kw + 4 < n) wnext = row[(size_t)((kw >> 2) + 1) * cap];
+            // the four entries' positions read together
+            double px[4], py[4], pz[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int j = (int)((word >> (16 * e)) & 0xffffu);
+                px[e] = r.pos[3 * j];
+                py[e] = r.pos[3 * j + 1];
+                pz[e] = r.pos[3 * j + 2];
+            }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 if (kw + e >= n) break;
                 const int j = (int)((word >> (16 * e)) & 0xffffu);
-                const double xj = r.pos[3 * j], yj = r.pos[3 * j + 1], zj = r.pos[3 * j + 2];
+                const double xj = px[e], yj = py[e], zj = pz[e];
+                // the whole numbers strictly between t_i and t_j, widened (see mop_entry_fast)
+                const double tj = (yj + a.hp - a.shift) * inv_spacing;
+                const int k0 = (int)ceil(fmin(ti, tj) - 1e-9), k1 = (int)floor(fmax(ti, tj) + 1e-9);
+                const bool cross = !(k0 > k1 || k1 < 0 || k0 >= a.nplanes);
+                if (!MIXED && !cross) continue;
                 const double rx = xi - xj, ry = yi - yj, rz = zi - zj;
                 const double r2 = (rx * rx + ry * ry) + rz * rz;
                 if (!(r2 < a.rc2)) continue;
-                const double inv = recip(r2);
-                const double acc = 48.0 * (p7(inv) - 0.5 * p4(inv));
-                pe = pe + (4.0 * (p6(inv) - p3(inv)) - a.potshift);
-                vir = vir + acc * r2;
+                double acc = 0.0;
+                if constexpr (MIXED) {
+                    const double inv = recip(r2);
+                    acc = 48.0 * (p7(inv) - 0.5 * p4(inv));
+                    pe = pe + (4.0 * (p6(inv) - p3(inv)) - a.potshift);
+                    vir = vir + acc * r2;
+                }
+                if (!cross) continue;
                 // mop_owner: entries against an image always; unshifted pairs once, from the
                 // molecule whose cell scans the other's as a forward cell, or in one cell from the
                 // higher molecule number
@@ -1424,9 +1445,12 @@ __device__ __forceinline__ void forces_body_mop(const FbArgs& a, int brick, char
                     }
                 }
                 if (own) {
+                    if constexpr (!MIXED) {
+                        const double inv = recip(r2);
+                        acc = 48.0 * (p7(inv) - 0.5 * p4(inv));
+                    }
                     const double ri[3] = {xi, yi, zi}, rj[3] = {xj, yj, zj}, rij[3] = {rx, ry, rz};
-                    mop_entry_fast(a, ri, rj, rij, acc, spxy, planes, inv_spacing,
-                                   in_i && fabs(xj) < bx && fabs(zj) < bz);
+                    mop_entry_fast(a, ri, rj, rij, acc, spxy, planes, k0, k1, in_i && fabs(xj) < bx && fabs(zj) < bz);
                 }
             }
         }
