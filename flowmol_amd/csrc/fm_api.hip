@@ -1140,6 +1140,30 @@ static int rebuild_lists(fm_ctx* c) {
 
 int rebuild_lists_internal(fm_ctx* c) { return rebuild_lists(c); }
 
+static int rebuild_lists_queued(fm_ctx* c);
+
+// The rebuild of a fused loop whose list one force call reads, at the positions it is built from
+// (the next step rebuilds before its force, and that step belongs to this loop): the pairs that
+// call counts are those inside rc, so the list keeps only those (r^2 below rc^2, with a relative
+// 1e-12 margin over the force kernels' own test on the same values).  The entries keep the order
+// of the full list (the cell windows and the runs are those of the list radius), and an entry
+// beyond rc adds exactly zero to every force, PE and virial sum, so each sum is the full list's,
+// bit for bit.  Only for pairs that all end at rc: no bonds, no Mie tables.
+static int rebuild_lists_for(fm_ctx* c, bool read_once, bool queued) {
+    const bool skinless = read_once && c->p.potential_flag == 0 && c->p.mie_potential == 0;
+    c->list_ra2 = skinless ? c->rc2 * (1.0 + 1e-12) : 0.0;
+    const int rc = queued ? rebuild_lists_queued(c) : rebuild_lists(c);
+    c->list_ra2 = 0.0;
+    return rc;
+}
+
+// Whether step `it` + 1 rebuilds its lists whatever the displacement or velocity checks find (its
+// record step is next, or the fixed interval)
+static bool next_rebuild_certain(const fm_ctx* c, int it) {
+    const bool fixed = c->p.rebuild_criteria != FM_REBUILD_DISPLACEMENT && c->p.rebuild_criteria != FM_REBUILD_VMAX;
+    return ((it + 2) % c->p.tplot) == 0 || (fixed && (it + 1) % c->p.fixed_rebuild == 0);
+}
+
 // The same rebuild queued without waiting on the device (fused loops): the list build uses the
 // region and list capacities of the last measured build, and a region or list beyond them raises
 // FLAG_NEED and FLAG_HALT, so the conditional launches queued after it do nothing until
@@ -1253,7 +1277,7 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
                 }
                 scur_p1[0] = c->scur;
                 c->rdisp_valid = false;
-                TRY(rebuild_lists_queued(c));
+                TRY(rebuild_lists_for(c, iter + 1 < end && host_rebuild(iter + 1), true));
                 ++*nrb;
                 lists = iter;
                 scur_lists = c->scur;
@@ -1375,7 +1399,7 @@ extern "C" int fm_run_vv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* ma
                 TRY(launch_fill_f64(c, c->d_rmax, 0.0, 1));
                 TRY(launch_fill_i32(c, c->d_flags + FLAG_REBUILD, 0, 1));
             }
-            TRY(rebuild_lists(c));
+            TRY(rebuild_lists_for(c, iter + 1 < iter0 + nsteps && next_rebuild_certain(c, iter), false));
             ++nrb;
         }
         TRY(launch_forces(c, iter, true));
@@ -1451,7 +1475,7 @@ extern "C" int fm_run_lfv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* m
                 TRY(launch_fill_f64(c, c->d_rmax, 0.0, 1));
                 TRY(launch_fill_i32(c, c->d_flags + FLAG_REBUILD, 0, 1));
             }
-            TRY(rebuild_lists(c));
+            TRY(rebuild_lists_for(c, iter + 1 < iter0 + nsteps && next_rebuild_certain(c, iter), false));
             ++nrb;
         }
     }

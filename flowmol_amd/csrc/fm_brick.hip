@@ -342,7 +342,8 @@ struct NlbArgs {
     int* bondn;
     int* flags;
     int cap, maxnb, maxreg;
-    double rn2;
+    double rn2;      // the list radius squared: the x windows, so the order of the entries
+    double ra2;      // pairs kept: r^2 < ra2 (rn2; rc^2 for a list read once, at the build's positions)
     int xsorted;     // cells are sorted by x (positions unchanged since binning)
     double xwin;     // window half-width: above sqrt(rn2), so no pair inside rn2 is skipped
     double cs[3], hd[3];  // cell side and half domain (cell faces in image coordinates)
@@ -468,7 +469,7 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                             const double ddx = xi - r.pos[3 * j], ddy = yi - r.pos[3 * j + 1],
                                          ddz = zi - r.pos[3 * j + 2];
                             const double r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
-                            m |= (r2 < a.rn2) ? (1u << u) : 0u;
+                            m |= (r2 < a.ra2) ? (1u << u) : 0u;
                         }
                         if (li >= j0 && li < j0 + nn) m &= ~(1u << (li - j0));
                         // entries enter the word from the top: after four, the first is in
@@ -489,7 +490,7 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                     for (int j = jb; j < je; ++j) {
                         const double ddx = xi - r.pos[3 * j], ddy = yi - r.pos[3 * j + 1], ddz = zi - r.pos[3 * j + 2];
                         const double r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
-                        const bool hit = (r2 < a.rn2) && (j != li);
+                        const bool hit = (r2 < a.ra2) && (j != li);
                         const uint32_t sh = (uint32_t)(cnt & 3) << 4;
                         word |= hit ? ((uint64_t)(uint32_t)j << sh) : 0ull;
                         if (hit && (cnt & 3) == 3) {
@@ -559,7 +560,7 @@ struct NlqArgs {
     int* nbn;
     int* flags;
     int cap, maxnb, maxreg;
-    double rn2, xwin, h, inv_h;
+    double rn2, ra2, xwin, h, inv_h;  // ra2, lo, hi: the radius of the pairs kept (as NlbArgs)
     int lo, hi;
     double cs[3], hd[3];
     const int* order;
@@ -760,7 +761,7 @@ k_nlist_quant(NlqArgs a) {
                         const double ddx = xi - img(a.s.x[sl], sx, g.L[0]), ddy = yi - img(a.s.y[sl], sy, g.L[1]),
                                      ddz = zi - img(a.s.z[sl], sz, g.L[2]);
                         const double r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
-                        if (r2 < a.rn2) m |= 1u << u;
+                        if (r2 < a.ra2) m |= 1u << u;
                     }
                     // entries enter the word from the top: after four, the first is in bits 0-15
                     while (m) {
@@ -1354,106 +1355,128 @@ __device__ inline void mop_entry_fast(const FbArgs& a, const double ri[3], const
 }
 
 // Record step of an LJ domain with the MOP tally and nothing else (no Mie tables, bonds, tethers
-// in the forces, CV faces or rfmol): the forces through the branch-free loop of plain steps (FP64
-// with PE and virial; MIXED: its FP32 pair law), then a second pass over the list for the MOP
-// planes of the entries the reference visits (mop_owner) that cross a plane, in FP64 as the
-// reference's loop computes them (src/simulation_compute_forces.f90:683-701), and with MIXED the
-// FP64 PE and virial of every entry inside rc.
-template <bool MIXED>
+// in the forces, CV faces or rfmol), in FP64 for both precisions (every pair term the tallies need
+// is an FP64 term, so the FP32 pair law would only add work).  One pass over the list: the forces,
+// PE and virial as the branch-free record loop computes them, and per entry the MOP test; the
+// entries inside rc that cross a plane and that this end tallies add their contribution, in FP64
+// as the reference's loop computes it (src/simulation_compute_forces.f90:683-701).
+//
+// Which end tallies: the reference visits every pair with both molecules in the domain once (its
+// half list) and every pair with a halo image from the molecule's side (so twice, once per end).
+// A pair's contribution is symmetric in its two ends (r_ij and the sign factor both change sign,
+// the crossed planes and the intercept are the same), so any rule that picks one end of each
+// unshifted pair tallies the same sum: the end with the higher molecule number.  Each staged
+// molecule's key is its molecule number << 1 | (its region cell is an image).
+struct P4K {
+    double x[4], y[4], z[4];
+    int k[4];
+};
+__device__ __forceinline__ void lj_stage_k(P4K& p, const double* pos, const int* key, uint64_t w) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int j = (int)((w >> (16 * q)) & 0xffffu);
+        p.x[q] = pos[3 * j];
+        p.y[q] = pos[3 * j + 1];
+        p.z[q] = pos[3 * j + 2];
+        p.k[q] = key[j];
+    }
+}
+struct MopI {             // the i-molecule's side of the MOP test
+    double ti;            // (y_i + halfdomain - shift) / planespacing
+    double hps, inv_sp;   // halfdomain - shift, 1 / planespacing
+    double bx, bz;        // the box's half widths in x and z, less 1e-12 relative
+    int keyi, nplanes;
+    bool in_i;            // molecule i strictly inside the box in x and z
+    double* spxy;
+    const double* planes;
+};
+__device__ __forceinline__ void lj_word_mop(const FbArgs& a, const P4K& p, int left, double xi, double yi,
+                                            double zi, const MopI& m, double& fx, double& fy, double& fz,
+                                            double& pe, double& vt) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double rx = xi - p.x[q], ry = yi - p.y[q], rz = zi - p.z[q];
+        const double r2 = (rx * rx + ry * ry) + rz * rz;
+        const bool in = r2 < a.rc2 && q < left;
+        const double xr = recip(r2);
+        const double x = in ? xr : __hiloint2double(0, __double2loint(xr));
+        const double x2 = x * x;
+        const double x3 = x * x2;
+        const double x4 = x2 * x2;
+        const double t = x4 * fmaf64(2.0, x3, -1.0);
+        fx = fmaf64(t, rx, fx);
+        fy = fmaf64(t, ry, fy);
+        fz = fmaf64(t, rz, fz);
+        vt = fmaf64(t, r2, vt);
+        const double e = 4.0 * (x3 * x3 - x3) - a.potshift;
+        pe = pe + (in ? e : 0.0);
+        // the whole numbers strictly between t_i and t_j, widened (see mop_entry_fast)
+        const double tj = (p.y[q] + m.hps) * m.inv_sp;
+        const int k0 = (int)ceil(fmin(m.ti, tj) - 1e-9), k1 = (int)floor(fmax(m.ti, tj) + 1e-9);
+        const int kj = p.k[q];
+        const bool own = ((m.keyi | kj) & 1) != 0 || (m.keyi >> 1) > (kj >> 1);
+        if (in && own && k0 <= k1 && k1 >= 0 && k0 < m.nplanes) {
+            const double acc = 48.0 * (p7(xr) - 0.5 * p4(xr));
+            const double ri[3] = {xi, yi, zi}, rj[3] = {p.x[q], p.y[q], p.z[q]}, rij[3] = {rx, ry, rz};
+            mop_entry_fast(a, ri, rj, rij, acc, m.spxy, m.planes, k0, k1,
+                           m.in_i && fabs(p.x[q]) < m.bx && fabs(p.z[q]) < m.bz);
+        }
+    }
+}
+
+// The list loop with the MOP test per entry: one word at a time, the next word's index in flight.
+__device__ __forceinline__ void lj_plain_mop(const FbArgs& a, const double* pos, const int* key,
+                                             const uint64_t* row, int n, size_t cap, double xi, double yi,
+                                             double zi, const MopI& m, double& fx, double& fy, double& fz,
+                                             double& pe_out, double& vir_out) {
+    double sx = 0, sy = 0, sz = 0, pe = 0.0, vt = 0.0;
+    uint64_t wn = row[0];
+    for (int kw = 0; kw < n; kw += 4) {
+        const uint64_t w = wn;
+        if (kw + 4 < n) wn = row[(size_t)((kw >> 2) + 1) * cap];
+        P4K p;
+        lj_stage_k(p, pos, key, w);
+        lj_word_mop(a, p, n - kw, xi, yi, zi, m, sx, sy, sz, pe, vt);
+    }
+    fx = 24.0 * sx;
+    fy = 24.0 * sy;
+    fz = 24.0 * sz;
+    pe_out = pe;
+    vir_out = 24.0 * vt;
+}
+
 __device__ __forceinline__ void forces_body_mop(const FbArgs& a, int brick, char* smem) {
     Region r;
     carve(r, smem, a.g, a.maxreg, true);
-    // after the region: the plane tallies, the plane positions, and each region cell's
-    // coordinates packed x | y << 10 | z << 20 (the owner rule without integer divisions)
+    // after the region: the plane tallies and the plane positions
     double* spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
     double* planes = spxy + 3 * a.nplanes;
-    int* rcxyz = (int*)(planes + a.nplanes);
     for (int t = threadIdx.x; t < a.nplanes; t += blockDim.x) planes[t] = a.planes[t];
-    for (int t = threadIdx.x; t < a.g.nreg; t += blockDim.x)
-        rcxyz[t] = (t % a.g.R[0]) | (((t / a.g.R[0]) % a.g.R[1]) << 10) | ((t / (a.g.R[0] * a.g.R[1])) << 20);
     if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags,
                       nullptr, a.ext))
         return;
     for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
+    // the staged molecule numbers become the owner keys
+    for (int t = threadIdx.x; t < r.total; t += blockDim.x) r.lslot[t] = (r.lslot[t] << 1) | (r.sh[r.rq[t]] != 13);
     __syncthreads();
     const size_t cap = a.cap;
-    const double inv_spacing = 1.0 / a.spacing;
-    const double bx = a.hx * (1.0 - 1e-12), bz = a.hz * (1.0 - 1e-12);
+    MopI m;
+    m.hps = a.hp - a.shift;
+    m.inv_sp = 1.0 / a.spacing;
+    m.bx = a.hx * (1.0 - 1e-12);
+    m.bz = a.hz * (1.0 - 1e-12);
+    m.nplanes = a.nplanes;
+    m.spxy = spxy;
+    m.planes = planes;
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
         int rci, li, i;
         imol(r, a.start, q, rci, li, i);
         const double xi = r.pos[3 * li], yi = r.pos[3 * li + 1], zi = r.pos[3 * li + 2];
-        const int n = a.nbn[i];
-        const uint64_t* row = a.nbl + i;
-        double fx, fy, fz;
-        double pe = 0.0, vir = 0.0;
-        // FP64: PE and virial with the forces in the branch-free loop; the pass below only tallies
-        // MOP.  Mixed: FP32 forces there, the FP64 PE and virial below.
-        if constexpr (MIXED) lj_plain<true>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz);
-        else lj_plain<false, true>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz, a.potshift, &pe, &vir);
-        const int exti = a.ext[i];
-        const bool in_i = fabs(xi) < bx && fabs(zi) < bz;
-        const bool shi = r.sh[rci] != 13;
-        const int ci = rcxyz[rci];
-        const double ti = (yi + a.hp - a.shift) * inv_spacing;
-        uint64_t wnext = n > 0 ? row[0] : 0;
-        for (int kw = 0; kw < n; kw += 4) {
-            const uint64_t word = wnext;
-            if (kw + 4 < n) wnext = row[(size_t)((kw >> 2) + 1) * cap];
-            // the four entries' positions read together
-            double px[4], py[4], pz[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int j = (int)((word >> (16 * e)) & 0xffffu);
-                px[e] = r.pos[3 * j];
-                py[e] = r.pos[3 * j + 1];
-                pz[e] = r.pos[3 * j + 2];
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                if (kw + e >= n) break;
-                const int j = (int)((word >> (16 * e)) & 0xffffu);
-                const double xj = px[e], yj = py[e], zj = pz[e];
-                // the whole numbers strictly between t_i and t_j, widened (see mop_entry_fast)
-                const double tj = (yj + a.hp - a.shift) * inv_spacing;
-                const int k0 = (int)ceil(fmin(ti, tj) - 1e-9), k1 = (int)floor(fmax(ti, tj) + 1e-9);
-                const bool cross = !(k0 > k1 || k1 < 0 || k0 >= a.nplanes);
-                if (!MIXED && !cross) continue;
-                const double rx = xi - xj, ry = yi - yj, rz = zi - zj;
-                const double r2 = (rx * rx + ry * ry) + rz * rz;
-                if (!(r2 < a.rc2)) continue;
-                double acc = 0.0;
-                if constexpr (MIXED) {
-                    const double inv = recip(r2);
-                    acc = 48.0 * (p7(inv) - 0.5 * p4(inv));
-                    pe = pe + (4.0 * (p6(inv) - p3(inv)) - a.potshift);
-                    vir = vir + acc * r2;
-                }
-                if (!cross) continue;
-                // mop_owner: entries against an image always; unshifted pairs once, from the
-                // molecule whose cell scans the other's as a forward cell, or in one cell from the
-                // higher molecule number
-                const int rcj = region_cell_of(r, a.g.nreg, j);
-                bool own = shi || r.sh[rcj] != 13;
-                if (!own) {
-                    if (rcj == rci) {
-                        own = exti > r.lslot[j];
-                    } else {
-                        const int cj = rcxyz[rcj];
-                        own = is_forward((cj & 1023) - (ci & 1023), ((cj >> 10) & 1023) - ((ci >> 10) & 1023),
-                                         (cj >> 20) - (ci >> 20));
-                    }
-                }
-                if (own) {
-                    if constexpr (!MIXED) {
-                        const double inv = recip(r2);
-                        acc = 48.0 * (p7(inv) - 0.5 * p4(inv));
-                    }
-                    const double ri[3] = {xi, yi, zi}, rj[3] = {xj, yj, zj}, rij[3] = {rx, ry, rz};
-                    mop_entry_fast(a, ri, rj, rij, acc, spxy, planes, k0, k1, in_i && fabs(xj) < bx && fabs(zj) < bz);
-                }
-            }
-        }
+        m.ti = (yi + m.hps) * m.inv_sp;
+        m.keyi = r.lslot[li];
+        m.in_i = fabs(xi) < m.bx && fabs(zi) < m.bz;
+        double fx, fy, fz, pe, vir;
+        lj_plain_mop(a, r.pos, r.lslot, a.nbl + i, a.nbn[i], cap, xi, yi, zi, m, fx, fy, fz, pe, vir);
         a.s.ax[i] = fx;
         a.s.ay[i] = fy;
         a.s.az[i] = fz;
@@ -1506,12 +1529,11 @@ k_forces_brick_lj(FbArgs a) {
 
 // record steps with the MOP tally only (forces_body_mop): two workgroups per CU, as the LDS of
 // two regions allows
-template <bool MIXED>
 __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128))) void
 k_forces_brick_mop(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt, a.step)) return;
-    forces_body_mop<MIXED>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+    forces_body_mop(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // LJ steps without tallies, bonds or tethers (the common step): registers held to what lets two
@@ -1757,10 +1779,8 @@ size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind) {
     const bool mie = c->p.mie_potential == 1;
     const size_t tab = mie ? FM_MIE_WORDS * sizeof(double) : 0;  // Mie tables
     if (kind == 1) return region_lds_bytes(g, maxreg, mie || c->p.vflux_outflag == 4, mie) + tab;
-    // kind 2: the plane tallies, and for the MOP record kernel the plane positions and the packed
-    // region-cell coordinates
-    return region_lds_bytes(g, maxreg, true, mie) + 4 * (size_t)c->p.nplanes * sizeof(double) +
-           (size_t)g.nreg * sizeof(int) + tab;
+    // kind 2: the plane tallies, and for the MOP record kernel the plane positions
+    return region_lds_bytes(g, maxreg, true, mie) + 4 * (size_t)c->p.nplanes * sizeof(double) + tab;
 }
 
 // measure the largest region (after binning); result in d_flags[FLAG_MAX_REGION]
@@ -1790,6 +1810,7 @@ int launch_nlist(fm_ctx* c) {
     a.maxnb = c->maxnb;
     a.maxreg = c->maxreg;
     a.rn2 = c->rn2;
+    a.ra2 = c->list_ra2 > 0.0 ? c->list_ra2 : c->rn2;
     a.xsorted = c->moved_since_bin ? 0 : 1;
     a.xwin = sqrt(c->rn2) * (1.0 + 1e-9) + 1e-12;
     a.order = c->brick_order;
@@ -1814,6 +1835,7 @@ int launch_nlist(fm_ctx* c) {
         q.maxnb = a.maxnb;
         q.maxreg = a.maxreg;
         q.rn2 = a.rn2;
+        q.ra2 = a.ra2;
         q.xwin = a.xwin;
         q.order = a.order;
         double ext = 0.0;
@@ -1826,7 +1848,7 @@ int launch_nlist(fm_ctx* c) {
         // squared distances of neighbouring-cell candidates then stay below 2^31
         q.h = ext / 32700.0;
         q.inv_h = 1.0 / q.h;
-        const double t = sqrt(c->rn2) / q.h, eps = 1.0 + 1e-6, r3 = sqrt(3.0);
+        const double t = sqrt(a.ra2) / q.h, eps = 1.0 + 1e-6, r3 = sqrt(3.0);
         q.lo = (int)floor(t * t - 2.0 * r3 * eps * t - 1e-9 * t * t) - 2;
         q.hi = (int)ceil(t * t * (1.0 + 3e-9) + 2.0 * r3 * eps * t * (1.0 + 1e-9) + 3.0 * eps * eps) + 2;
         const int nbmax = g.B[0] * g.B[1] * g.B[2];
@@ -1958,10 +1980,10 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
     }
     ktimer_begin(c, KT_FORCES);
     dim3 grid(brick_count(c)), blk(FB_BLOCK);
-    // a record step whose only tally beyond PE and virial is MOP: the two-pass kernel
+    // a record step whose only tally beyond PE and virial is MOP: the one-pass FP64 kernel (for
+    // both precisions)
     if ((kind == 0 || kind == 1) && a.rec && a.pxy != nullptr && a.cv == nullptr && a.rf == nullptr) {
-        if (kind == 1) k_forces_brick_mop<true><<<grid, blk, lds, c->stream>>>(a);
-        else k_forces_brick_mop<false><<<grid, blk, lds, c->stream>>>(a);
+        k_forces_brick_mop<<<grid, blk, lds, c->stream>>>(a);
         ktimer_end(c, KT_FORCES);
         LAUNCH_CHECK();
         return 0;

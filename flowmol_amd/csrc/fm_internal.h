@@ -58,6 +58,9 @@ struct fm_ctx {
     int nc[3] = {0, 0, 0};
     int ncell = 0;
     double half[3], cs[3], rn2, rc2, potshift, dmin;
+    // > 0: the next list build keeps only the pairs with r^2 below this (a list that one force
+    // call reads at the build's positions needs no skin); 0: the list radius
+    double list_ra2 = 0.0;
 
     // state, slot order (two copies for the permutation at rebuild)
     double* st[2] = {nullptr, nullptr};   // each: 9*cap doubles: x y z vx vy vz ax ay az

@@ -340,3 +340,41 @@ def test_tuned_brick_keeps_sets_and_forces():
         g.md_advance_vv(it)
     assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-10
     g.close()
+
+
+@pytest.mark.parametrize("integ", ["vv", "lfv"])
+def test_read_once_lists_keep_every_sum(integ):
+    """With TPLOT 1 every step rebuilds, so inside a fused loop each list is read by one force
+    call at the positions it was built from, and the loop keeps only the pairs inside rc.  After
+    30 steps positions, velocities and accelerations, and every record step's sums, are
+    bit-identical to the routine-by-routine sequence (full lists); the list the loop leaves behind
+    is the full list."""
+    from flowmol_amd._abi import FM_INTEGRATION_LFV
+
+    p, r, v = config.config_fcc((8, 8, 8), 0.8, 2.5, seed=31, tplot=1)
+    r = r + np.random.default_rng(31).uniform(-0.05, 0.05, r.shape)
+    if integ == "lfv":
+        p.integration_algorithm = FM_INTEGRATION_LFV
+    say(f"case: read-once lists, 2048 LJ, {integ}, TPLOT 1, 30 steps")
+    g1 = MDContext(p, r, v)
+    g2 = MDContext(p, r, v)
+    g1.setup(0)
+    g2.setup(0)
+    m1 = []
+    if integ == "vv":
+        for it in range(1, 31):
+            g1.md_advance_vv(it)
+            m1.append(g1.evaluate_macroscopic_properties())
+        m2, nrb = g2.run_vv(1, 30, n_macros=30)
+    else:
+        for it in range(0, 30):
+            m1.append(g1.md_advance_lfv(it, macros=True)[1])
+        m2, nrb = g2.run_lfv(0, 30, n_macros=30)
+    assert nrb == 30
+    for x, y in zip(g1.state(), g2.state()):
+        assert np.array_equal(x, y)
+    assert m1 == m2
+    for x, y in zip(g1.neighbourlist(), g2.neighbourlist()):
+        assert np.array_equal(x, y)
+    g1.close()
+    g2.close()
