@@ -342,10 +342,10 @@ struct NlbArgs {
     int* bondn;
     int* flags;
     int cap, maxnb, maxreg;
-    double rn2;      // the list radius squared: the x windows, so the order of the entries
-    double ra2;      // pairs kept: r^2 < ra2 (rn2; rc^2 for a list read once, at the build's positions)
+    double ra2;      // pairs kept: r^2 < ra2 (the list radius; rc^2 for a list read once, at the
+                     // positions it is built from)
     int xsorted;     // cells are sorted by x (positions unchanged since binning)
-    double xwin;     // window half-width: above sqrt(rn2), so no pair inside rn2 is skipped
+    double xwin;     // window half-width: above sqrt(ra2), so no pair inside ra2 is skipped
     double cs[3], hd[3];  // cell side and half domain (cell faces in image coordinates)
     const int* order;     // brick order (brick_of_block)
 };
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                 if (!a.xsorted) return;
                 const double gy = dy < 0 ? gyl : (dy > 0 ? gyh : 0.0);
                 const double gz = dz < 0 ? gzl : (dz > 0 ? gzh : 0.0);
-                const double w2 = a.rn2 - (gy * gy + gz * gz);
+                const double w2 = a.ra2 - (gy * gy + gz * gz);
                 if (!(w2 > 0.0)) {
                     je = jb;
                     return;
@@ -438,27 +438,35 @@ __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
                 je = lo;
             };
             if (!POLY) {
-                // All nine windows first, then the runs longest first: the lanes of a wavefront
+                // All nine windows first, then the runs nearest first: the lanes of a wavefront
                 // then walk runs of similar length together (the order of a list's entries is
-                // free; the set is the reference's).  Packed as size:20 | jb:20 | je:20.
+                // free; the set is the reference's).  The order is that of the distance g^2 from
+                // the molecule to the run's y and z faces (its top 28 bits), then of the run
+                // number, which does not depend on the radius: a list of a smaller radius is the
+                // larger one's with the farther entries left out.  Packed as g2:28 | k:4 | jb:16 |
+                // je:16 (a region's LDS holds far fewer than 65,536 molecules).
                 uint64_t w[9];
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
                     int jb, je;
                     run_window(k, jb, je);
-                    w[k] = ((uint64_t)(je - jb) << 40) | ((uint64_t)jb << 20) | (uint64_t)je;
+                    const int dz = k / 3 - 1, dy = k % 3 - 1;
+                    const double gy = dy < 0 ? gyl : (dy > 0 ? gyh : 0.0);
+                    const double gz = dz < 0 ? gzl : (dz > 0 ? gzh : 0.0);
+                    w[k] = ((uint64_t)(__float_as_uint((float)(gy * gy + gz * gz)) >> 4) << 36) | ((uint64_t)k << 32) |
+                           ((uint64_t)jb << 16) | (uint64_t)je;
                 }
 #pragma unroll
-                for (int round = 0; round < 9; ++round)  // odd-even transposition sort, descending
+                for (int round = 0; round < 9; ++round)  // odd-even transposition sort, ascending
 #pragma unroll
                     for (int k = round & 1; k + 1 < 9; k += 2) {
                         const uint64_t x = w[k], y = w[k + 1];
-                        w[k] = x > y ? x : y;
-                        w[k + 1] = x > y ? y : x;
+                        w[k] = x < y ? x : y;
+                        w[k + 1] = x < y ? y : x;
                     }
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
-                    const int jb = (int)((w[k] >> 20) & 0xfffffu), je = (int)(w[k] & 0xfffffu);
+                    const int jb = (int)((w[k] >> 16) & 0xffffu), je = (int)(w[k] & 0xffffu);
                     // 32 candidates at a time: a hit mask first (distance tests only), then the
                     // hits in candidate order into the list words
                     for (int j0 = jb; j0 < je; j0 += 32) {
@@ -560,7 +568,7 @@ struct NlqArgs {
     int* nbn;
     int* flags;
     int cap, maxnb, maxreg;
-    double rn2, ra2, xwin, h, inv_h;  // ra2, lo, hi: the radius of the pairs kept (as NlbArgs)
+    double ra2, xwin, h, inv_h;  // ra2, xwin, lo, hi: the radius of the pairs kept (as NlbArgs)
     int lo, hi;
     double cs[3], hd[3];
     const int* order;
@@ -678,8 +686,11 @@ k_nlist_quant(NlqArgs a) {
             const double ylo = -a.hd[1] + vy * a.cs[1], zlo = -a.hd[2] + vz * a.cs[2];
             const double gyl = fmax(yi - ylo - 1e-9, 0.0), gyh = fmax(ylo + a.cs[1] - yi - 1e-9, 0.0);
             const double gzl = fmax(zi - zlo - 1e-9, 0.0), gzh = fmax(zlo + a.cs[2] - zi - 1e-9, 0.0);
-            // nine runs of three x-consecutive region cells; each x window widened by a grid step
-            // on both sides of its quantised bounds, runs longest first
+            // nine runs of three x-consecutive region cells; each x window (of the radius of the
+            // pairs kept) widened by a grid step on both sides of its quantised bounds.  The runs
+            // go nearest first: by the distance g^2 from the molecule to the run's y and z faces
+            // (its top 28 bits), then by run number.  That order does not depend on the radius,
+            // so a list of a smaller radius is the larger one's with the farther entries left out.
             uint64_t w[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
@@ -688,7 +699,9 @@ k_nlist_quant(NlqArgs a) {
                 int jb = r.off[c0], je = r.off[c0 + 3];
                 const double gy = dy < 0 ? gyl : (dy > 0 ? gyh : 0.0);
                 const double gz = dz < 0 ? gzl : (dz > 0 ? gzh : 0.0);
-                const double w2 = a.rn2 - (gy * gy + gz * gz);
+                const double g2 = gy * gy + gz * gz;
+                const double w2 = a.ra2 - g2;
+                const uint64_t order = ((uint64_t)(__float_as_uint((float)g2) >> 4) << 36) | ((uint64_t)k << 32);
                 if (!(w2 > 0.0)) {
                     je = jb;
                 } else {
@@ -710,20 +723,20 @@ k_nlist_quant(NlqArgs a) {
                     }
                     je = lo;
                 }
-                w[k] = ((uint64_t)(je - jb) << 40) | ((uint64_t)jb << 20) | (uint64_t)je;
+                w[k] = order | ((uint64_t)jb << 16) | (uint64_t)je;
             }
 #pragma unroll
-            for (int sr = 0; sr < 9; ++sr)  // odd-even transposition sort, descending
+            for (int sr = 0; sr < 9; ++sr)  // odd-even transposition sort, ascending
 #pragma unroll
                 for (int k = sr & 1; k + 1 < 9; k += 2) {
                     const uint64_t x = w[k], y = w[k + 1];
-                    w[k] = x > y ? x : y;
-                    w[k + 1] = x > y ? y : x;
+                    w[k] = x < y ? x : y;
+                    w[k + 1] = x < y ? y : x;
                 }
             uint64_t word = 0;
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                const int jb = (int)((w[k] >> 20) & 0xfffffu), je = (int)(w[k] & 0xfffffu);
+                const int jb = (int)((w[k] >> 16) & 0xffffu), je = (int)(w[k] & 0xffffu);
                 for (int j0 = jb; j0 < je; j0 += 32) {
                     const int nn = min(32, je - j0);
                     // each test shifts its two bits in from the bottom (the first candidate ends
@@ -1416,7 +1429,8 @@ __device__ __forceinline__ void lj_word_mop(const FbArgs& a, const P4K& p, int l
         const int kj = p.k[q];
         const bool own = ((m.keyi | kj) & 1) != 0 || (m.keyi >> 1) > (kj >> 1);
         if (in && own && k0 <= k1 && k1 >= 0 && k0 < m.nplanes) {
-            const double acc = 48.0 * (p7(xr) - 0.5 * p4(xr));
+            // the pair law's magnitude 48 (x^7 - x^4/2) as the force sum takes it (within 1e-15)
+            const double acc = 24.0 * t;
             const double ri[3] = {xi, yi, zi}, rj[3] = {p.x[q], p.y[q], p.z[q]}, rij[3] = {rx, ry, rz};
             mop_entry_fast(a, ri, rj, rij, acc, m.spxy, m.planes, k0, k1,
                            m.in_i && fabs(p.x[q]) < m.bx && fabs(p.z[q]) < m.bz);
@@ -1809,10 +1823,9 @@ int launch_nlist(fm_ctx* c) {
     a.cap = c->cap;
     a.maxnb = c->maxnb;
     a.maxreg = c->maxreg;
-    a.rn2 = c->rn2;
     a.ra2 = c->list_ra2 > 0.0 ? c->list_ra2 : c->rn2;
     a.xsorted = c->moved_since_bin ? 0 : 1;
-    a.xwin = sqrt(c->rn2) * (1.0 + 1e-9) + 1e-12;
+    a.xwin = sqrt(a.ra2) * (1.0 + 1e-9) + 1e-12;
     a.order = c->brick_order;
     for (int d = 0; d < 3; ++d) {
         a.cs[d] = c->cs[d];
@@ -1834,7 +1847,6 @@ int launch_nlist(fm_ctx* c) {
         q.cap = a.cap;
         q.maxnb = a.maxnb;
         q.maxreg = a.maxreg;
-        q.rn2 = a.rn2;
         q.ra2 = a.ra2;
         q.xwin = a.xwin;
         q.order = a.order;
