@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--replicas", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=6.0, help="serve: Poisson arrival rate (1/s)")
     ap.add_argument("--replica-steps", type=int, default=2000, help="serve: steps per replica")
+    ap.add_argument("--serve-graphs", choices=["on", "off"], default="off",
+                    help="serve: replay each window step's packed launches as HIP graphs")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a plain device-to-device copy of this size (PMC byte calibration)")
     return ap.parse_args()
@@ -90,7 +92,7 @@ def serve_main(a):
     reqs = poisson_workload(a.replicas, a.rate, seed=5, nsteps=a.replica_steps)
     # warm-up: one small replica through the same path (kernel loading, graph capture)
     warm = poisson_workload(1, 100.0, seed=99, nmin=4096, nmax=4096, nsteps=50)
-    srv = ReplicaServer(chunk=50)
+    srv = ReplicaServer(chunk=50, graphs=a.serve_graphs == "on")
     srv.serve(warm, realtime=False)
     drift = []
 
@@ -111,6 +113,8 @@ def serve_main(a):
         "aggregate_atom_steps_per_s": stats["atom_steps_per_s"],
         "wall_s": stats["wall_s"],
         "replicas": stats["replicas"],
+        "batch_steps": stats["batch_steps"],
+        "host_syncs_per_batch_step": stats["host_syncs"] / max(1, stats["batch_steps"]),
         "dtype": "f64",
         "data": "synthetic",
         "vs_baseline": None,

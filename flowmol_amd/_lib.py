@@ -66,6 +66,7 @@ SIGNATURES = {
     "fm_batch_add": (C.c_int, [_vp, _vp, _i32]),
     "fm_batch_remove": (C.c_int, [_vp, _vp]),
     "fm_batch_run": (C.c_int, [_vp, _i32, _ip]),
+    "fm_batch_counters": (C.c_int, [_vp, _lp, _lp]),
     "fm_batch_size": (C.c_int, [_vp, _ip]),
     "fm_batch_get_iter": (C.c_int, [_vp, _vp, _ip]),
     "fm_memory_footprint": (C.c_int, [C.POINTER(FmParams), _i32, _lp]),

@@ -117,10 +117,17 @@ static int dalloc(T** p, size_t n) {
     return 0;
 }
 
+static int check_flags(fm_ctx* c);
+
 // read the device flag words; report the first device-side error
 static int sync_flags(fm_ctx* c) {
     FM_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, FLAG_COUNT * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     FM_HIP(hipStreamSynchronize(c->stream));
+    return check_flags(c);
+}
+
+// the first device-side error in the flag words read last (h_flags)
+static int check_flags(fm_ctx* c) {
     int code = c->h_flags[FLAG_ERROR];
     if (code) {
         int mol = c->h_flags[FLAG_ERROR_MOL];
@@ -141,6 +148,7 @@ static int sync_flags(fm_ctx* c) {
 }
 
 int ctx_sync_flags(fm_ctx* c) { return sync_flags(c); }
+int check_flags_internal(fm_ctx* c) { return check_flags(c); }
 
 static void free_ctx(fm_ctx* c) {
     if (!c) return;
@@ -1187,6 +1195,8 @@ static int rebuild_lists_queued(fm_ctx* c) {
     return 0;
 }
 
+int rebuild_lists_queued_internal(fm_ctx* c) { return rebuild_lists_queued(c); }
+
 // A synchronisation found FLAG_NEED set by the queued list build: grow the region and list
 // capacities to what it met and build the lists again (the cells are unchanged, and every launch
 // queued after the build did nothing).
@@ -1203,6 +1213,8 @@ static int redo_lists(fm_ctx* c) {
     }
     return c->h_flags[FLAG_NEED] ? fm_set_error(FM_EOVERFLOW, "list build still beyond its capacities") : 0;
 }
+
+int redo_lists_internal(fm_ctx* c) { return redo_lists(c); }
 
 // bookkeeping of a completed step of the fused loops; ring: the step's record sums, taken on the
 // device when the step was queued (null: evaluate them now, the state being that of this step)

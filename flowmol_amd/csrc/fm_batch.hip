@@ -2,12 +2,24 @@
 //
 // Each step is two packed launches over every member: VV pass 1 with the displacement check
 // (one block per 256 molecules of a member), and the forces with the fused VV pass 2 (one block
-// per brick of a member).  Per-member arguments live in device descriptor arrays, rebuilt per
+// per brick of a member).  Per-member arguments live in device descriptor arrays, written per
 // step from pinned host staging; block -> (member, offset) tables change only when the
-// membership or a member's brick count changes.  One host synchronisation per step reads all
-// members' rebuild flags; only flagged members rebuild their lists.  Both packed launches are
-// captured as HIP graphs (copy nodes and kernel nodes only) and re-instantiated when a table,
-// the launch size or the LDS size changes.
+// membership or a member's brick count changes.
+//
+// Steps are queued in windows of up to BATCH_WINDOW steps with one host synchronisation per
+// window.  A member's steps in a window end before its next rebuild that the host knows in
+// advance (the record step's forced rebuild, or the fixed interval); such a rebuild opens the
+// member's next window, queued without host waits.  A displacement check that fires inside the
+// window records its step in the member's flag words, and the member's later launches of the
+// window do nothing (the packed kernels read each member's flags); at the window's
+// synchronisation the host reads every member's flags, and the member's next window starts
+// with that step's rebuild and force.  A queued list build beyond the capacities halts the
+// member the same way and is repeated, with grown capacities, at the synchronisation.  Each
+// completed step has the arithmetic of the member's own step-by-step loop.  A window's
+// descriptors go to the device in three copies: pass 1 of its first step, then (after the
+// rebuilds that open members' windows, which change their state pointers) everything else.
+// Optionally the launches after the rebuilds are replayed as one HIP graph per window length
+// (kernel nodes only), re-instantiated when a table, the launch sizes or the LDS size change.
 #include "fm_internal.h"
 
 #include <string.h>
@@ -15,61 +27,64 @@
 #include <algorithm>
 #include <vector>
 
+#define BATCH_WINDOW 16
+
 struct Member {
     fm_ctx* c;
     int iter;                 // iteration number of the member's next step
     hipStream_t own_stream;   // restored when the member leaves
+    int fired;                // step whose displacement check fired (its pass 1 has run), or -1
+};
+
+struct WindowGraph {          // the launches of a window of W steps after its rebuilds
+    hipGraphExec_t g = nullptr;
+    int nblk1 = -1, nblk2 = -1;
+    size_t lds = 0;
 };
 
 struct fm_batch {
     std::vector<Member> m;
     hipStream_t stream = nullptr;
     int kind = -1;
-    // device
+    // device: BATCH_WINDOW slots of cap_m descriptors each
     P1Desc* d_p1 = nullptr;
-    int* d_bflags = nullptr;
     char* d_fargs = nullptr;
     int2* d_t1 = nullptr;
     int2* d_t2 = nullptr;
     size_t cap_m = 0, cap_t1 = 0, cap_t2 = 0;
-    // pinned host staging
+    // pinned host staging, the same slots
     P1Desc* h_p1 = nullptr;
-    int* h_bflags = nullptr;
     char* h_fargs = nullptr;
     std::vector<int2> t1, t2;
     bool dirty = true;
     int nblk1 = 0, nblk2 = 0;
     // graphs
     bool use_graph = true;
-    hipGraphExec_t g1 = nullptr, g2 = nullptr;
-    int g1_nblk = -1, g2_nblk = -1, g2_kind = -1;
-    size_t g1_n = 0, g2_n = 0;          // member counts baked into the graphs' copy nodes
-    size_t g2_lds = 0;
+    WindowGraph wg[BATCH_WINDOW + 1];
     int64_t rebuilds = 0;
+    int64_t windows = 0;
 };
 
 
 static void free_graphs(fm_batch* b) {
-    if (b->g1) hipGraphExecDestroy(b->g1);
-    if (b->g2) hipGraphExecDestroy(b->g2);
-    b->g1 = b->g2 = nullptr;
-    b->g1_nblk = b->g2_nblk = -1;
+    for (WindowGraph& w : b->wg) {
+        if (w.g) hipGraphExecDestroy(w.g);
+        w = WindowGraph();
+    }
 }
 
 static void free_buffers(fm_batch* b) {
     free_graphs(b);
-    void* d[] = {b->d_p1, b->d_bflags, b->d_fargs, b->d_t1, b->d_t2};
+    void* d[] = {b->d_p1, b->d_fargs, b->d_t1, b->d_t2};
     for (void* p : d)
         if (p) hipFree(p);
-    void* h[] = {b->h_p1, b->h_bflags, b->h_fargs};
+    void* h[] = {b->h_p1, b->h_fargs};
     for (void* p : h)
         if (p) hipHostFree(p);
     b->d_p1 = nullptr;
-    b->d_bflags = nullptr;
     b->d_fargs = nullptr;
     b->d_t1 = b->d_t2 = nullptr;
     b->h_p1 = nullptr;
-    b->h_bflags = nullptr;
     b->h_fargs = nullptr;
     b->cap_m = b->cap_t1 = b->cap_t2 = 0;
 }
@@ -79,17 +94,19 @@ static int ensure_member_buffers(fm_batch* b, size_t n) {
     size_t cap = std::max<size_t>(n, 2 * b->cap_m + 8);
     free_graphs(b);
     if (b->d_p1) hipFree(b->d_p1);
-    if (b->d_bflags) hipFree(b->d_bflags);
     if (b->d_fargs) hipFree(b->d_fargs);
     if (b->h_p1) hipHostFree(b->h_p1);
-    if (b->h_bflags) hipHostFree(b->h_bflags);
     if (b->h_fargs) hipHostFree(b->h_fargs);
-    FM_HIP(hipMalloc((void**)&b->d_p1, cap * sizeof(P1Desc)));
-    FM_HIP(hipMalloc((void**)&b->d_bflags, cap * sizeof(int)));
-    FM_HIP(hipMalloc((void**)&b->d_fargs, cap * fargs_bytes()));
-    FM_HIP(hipHostMalloc((void**)&b->h_p1, cap * sizeof(P1Desc), hipHostMallocDefault));
-    FM_HIP(hipHostMalloc((void**)&b->h_bflags, cap * sizeof(int), hipHostMallocDefault));
-    FM_HIP(hipHostMalloc((void**)&b->h_fargs, cap * fargs_bytes(), hipHostMallocDefault));
+    b->d_p1 = nullptr;
+    b->d_fargs = nullptr;
+    b->h_p1 = nullptr;
+    b->h_fargs = nullptr;
+    b->cap_m = 0;
+    const size_t W = BATCH_WINDOW;
+    FM_HIP(hipMalloc((void**)&b->d_p1, W * cap * sizeof(P1Desc)));
+    FM_HIP(hipMalloc((void**)&b->d_fargs, W * cap * fargs_bytes()));
+    FM_HIP(hipHostMalloc((void**)&b->h_p1, W * cap * sizeof(P1Desc), hipHostMallocDefault));
+    FM_HIP(hipHostMalloc((void**)&b->h_fargs, W * cap * fargs_bytes(), hipHostMallocDefault));
     b->cap_m = cap;
     return 0;
 }
@@ -127,35 +144,46 @@ static int upload_tables(fm_batch* b) {
     return 0;
 }
 
-// the work of one packed launch, enqueued on the batch stream (also used under capture)
-static int enqueue_pass1(fm_batch* b) {
-    const size_t n = b->m.size();
-    FM_HIP(hipMemcpyAsync(b->d_p1, b->h_p1, n * sizeof(P1Desc), hipMemcpyHostToDevice, b->stream));
-    FM_HIP(hipMemcpyAsync(b->d_bflags, b->h_bflags, n * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    if (int rc = launch_pass1_multi(b->stream, b->d_p1, b->d_t1, b->nblk1)) return rc;
-    FM_HIP(hipMemcpyAsync(b->h_bflags, b->d_bflags, n * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+static P1Desc* slot_p1(fm_batch* b, P1Desc* base, int j) { return base + (size_t)j * b->cap_m; }
+static char* slot_fargs(fm_batch* b, char* base, int j) { return base + (size_t)j * b->cap_m * fargs_bytes(); }
+
+// the launches of window steps j0.. of a W-step window: pass 1 (except at step 0, launched
+// before the rebuilds) and the forces of each step, from the descriptors already on the device
+static int enqueue_window(fm_batch* b, int W, size_t lds) {
+    for (int j = 0; j < W; ++j) {
+        if (j > 0) TRY(launch_pass1_multi(b->stream, slot_p1(b, b->d_p1, j), b->d_t1, b->nblk1));
+        TRY(launch_forces_multi(b->stream, b->kind, slot_fargs(b, b->d_fargs, j), b->d_t2, b->nblk2, lds));
+    }
     return 0;
 }
 
-static int enqueue_forces(fm_batch* b, size_t lds) {
-    const size_t n = b->m.size();
-    FM_HIP(hipMemcpyAsync(b->d_fargs, b->h_fargs, n * fargs_bytes(), hipMemcpyHostToDevice, b->stream));
-    return launch_forces_multi(b->stream, b->kind, b->d_fargs, b->d_t2, b->nblk2, lds);
-}
-
-static int capture(fm_batch* b, hipGraphExec_t* exec, int which, size_t lds) {
-    hipGraph_t g = nullptr;
-    FM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = which == 1 ? enqueue_pass1(b) : enqueue_forces(b, lds);
-    hipError_t e = hipStreamEndCapture(b->stream, &g);
-    if (rc) {
-        if (g) hipGraphDestroy(g);
-        return rc;
+static int launch_window(fm_batch* b, int W, size_t lds) {
+    if (!b->use_graph) return enqueue_window(b, W, lds);
+    WindowGraph& w = b->wg[W];
+    // a graph's LDS size is fixed: keep it until a member needs more, with headroom
+    if (!w.g || w.nblk1 != b->nblk1 || w.nblk2 != b->nblk2 || lds > w.lds) {
+        if (w.g) hipGraphExecDestroy(w.g);
+        w.g = nullptr;
+        size_t glds = std::max(lds, lds + lds / 8);
+        size_t limit = b->m[0].c->lds_limit;
+        if (glds > limit) glds = std::max(lds, limit);
+        hipGraph_t g = nullptr;
+        FM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_window(b, W, glds);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fm_set_error(FM_EHIP, "graph capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&w.g, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return fm_set_error(FM_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+        w.nblk1 = b->nblk1;
+        w.nblk2 = b->nblk2;
+        w.lds = glds;
     }
-    if (e != hipSuccess) return fm_set_error(FM_EHIP, "graph capture: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    if (e != hipSuccess) return fm_set_error(FM_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+    FM_HIP(hipGraphLaunch(w.g, b->stream));
     return 0;
 }
 
@@ -204,9 +232,12 @@ extern "C" int fm_batch_add(fm_batch* b, fm_ctx* c, int32_t iter0) {
     if (b->kind >= 0 && kind != b->kind)
         return fm_set_error(FM_EINVAL, "batch members must share precision, pair law and bond/tether kind");
     if (int rc = ensure_member_buffers(b, b->m.size() + 1)) return rc;
+    // no rebuild pending from the member's own loops (the batch's halt rule reads these words)
+    static const int reset[] = {FLAG_REBUILD, FLAG_HALT, FLAG_NEED};
+    if (int rc = launch_reset_flags(c, reset, 3)) return rc;
     FM_HIP(hipStreamSynchronize(c->stream));
     b->kind = kind;
-    b->m.push_back({c, iter0, c->stream});
+    b->m.push_back({c, iter0, c->stream, -1});
     c->stream = b->stream;
     b->dirty = true;
     return 0;
@@ -232,6 +263,13 @@ extern "C" int fm_batch_size(const fm_batch* b, int32_t* n) {
     return 0;
 }
 
+extern "C" int fm_batch_counters(const fm_batch* b, int64_t* rebuilds, int64_t* host_syncs) {
+    if (!b) return fm_set_error(FM_EINVAL, "null batch");
+    if (rebuilds) *rebuilds = b->rebuilds;
+    if (host_syncs) *host_syncs = b->windows;
+    return 0;
+}
+
 extern "C" int fm_batch_get_iter(const fm_batch* b, const fm_ctx* c, int32_t* iter) {
     if (!b || !c || !iter) return fm_set_error(FM_EINVAL, "null argument");
     for (auto& mm : b->m)
@@ -242,7 +280,13 @@ extern "C" int fm_batch_get_iter(const fm_batch* b, const fm_ctx* c, int32_t* it
     return fm_set_error(FM_EINVAL, "context not in the batch");
 }
 
-// md_advance_vv (src/md.f90:143-163) nsteps times for every member
+// Does member c rebuild at step it whatever its displacement check finds (the record step's
+// forced rebuild, or the fixed interval)?
+static bool host_rebuild(const fm_ctx* c, int it) {
+    return ((it + 1) % c->p.tplot) == 0 || (c->p.rebuild_criteria == FM_REBUILD_FIXED && it % c->p.fixed_rebuild == 0);
+}
+
+// md_advance_vv (src/md.f90:143-163) nsteps times for every member, in windows of queued steps
 extern "C" int fm_batch_run(fm_batch* b, int32_t nsteps, int32_t* n_rebuilds) {
     if (!b) return fm_set_error(FM_EINVAL, "null batch");
     const size_t n = b->m.size();
@@ -253,96 +297,125 @@ extern "C" int fm_batch_run(fm_batch* b, int32_t nsteps, int32_t* n_rebuilds) {
     }
     if (b->dirty) TRY(upload_tables(b));
     const size_t fsz = fargs_bytes();
-    for (int step = 0; step < nsteps; ++step) {
-        // ---- VV pass 1 and the displacement check, all members in one launch
+    std::vector<int> rem(n, nsteps), len(n), it0(n);
+    std::vector<char> rb0(n);
+    for (;;) {
+        // ---- the window: each member's steps up to its next host-known rebuild
+        int W = 0;
         for (size_t k = 0; k < n; ++k) {
             Member& mm = b->m[k];
-            fm_ctx* c = mm.c;
-            const bool forced = ((mm.iter + 1) % c->p.tplot) == 0;
-            P1Desc& d = b->h_p1[k];
-            d.st = c->x();
-            d.rd = c->rdisp;
-            d.flag_out = b->d_bflags + k;
-            d.np = c->np;
-            d.cap = c->cap;
-            d.check = (c->p.rebuild_criteria == FM_REBUILD_DISPLACEMENT && !forced) ? 1 : 0;
-            d.fresh = c->rdisp_valid ? 0 : 1;
-            d.dt = c->p.delta_t;
-            d.hdt2 = 0.5 * (c->p.delta_t * c->p.delta_t);
-            d.hdt = 0.5 * c->p.delta_t;
-            d.thresh = 0.5 * c->dmin;
-            b->h_bflags[k] = 0;
-            c->moved_since_bin = true;
+            it0[k] = mm.iter;
+            len[k] = 0;
+            rb0[k] = 0;
+            if (rem[k] <= 0) continue;
+            rb0[k] = mm.fired == mm.iter || host_rebuild(mm.c, mm.iter);
+            int l = 1;
+            while (l < BATCH_WINDOW && l < rem[k] && !host_rebuild(mm.c, mm.iter + l)) ++l;
+            len[k] = l;
+            W = std::max(W, l);
         }
-        if (b->use_graph) {
-            if (!b->g1 || b->g1_nblk != b->nblk1 || b->g1_n != n) {
-                if (b->g1) hipGraphExecDestroy(b->g1);
-                b->g1 = nullptr;
-                TRY(capture(b, &b->g1, 1, 0));
-                b->g1_nblk = b->nblk1;
-                b->g1_n = n;
+        if (W == 0) break;
+        const size_t n_ = n;
+        size_t lds = 0;
+        for (int j = 0; j < W; ++j) {
+            // ---- VV pass 1 and the displacement check of every member's step j
+            for (size_t k = 0; k < n_; ++k) {
+                Member& mm = b->m[k];
+                fm_ctx* c = mm.c;
+                const int it = it0[k] + j;
+                P1Desc& d = slot_p1(b, b->h_p1, j)[k];
+                d.active = j < len[k] && !(j == 0 && mm.fired == it0[k]);
+                d.st = c->x();
+                d.rd = c->rdisp;
+                d.flags = c->d_flags;
+                d.np = c->np;
+                d.cap = c->cap;
+                d.step = it;
+                d.check = (c->p.rebuild_criteria == FM_REBUILD_DISPLACEMENT && !host_rebuild(c, it)) ? 1 : 0;
+                d.fresh = c->rdisp_valid ? 0 : 1;
+                d.dt = c->p.delta_t;
+                d.hdt2 = 0.5 * (c->p.delta_t * c->p.delta_t);
+                d.hdt = 0.5 * c->p.delta_t;
+                d.thresh = 0.5 * c->dmin;
+                if (d.active) {
+                    c->moved_since_bin = true;
+                    if (d.check) c->rdisp_valid = true;
+                }
             }
-            FM_HIP(hipGraphLaunch(b->g1, b->stream));
-        } else {
-            TRY(enqueue_pass1(b));
+            if (j == 0) {
+                // pass 1 of the first step, then the rebuilds that open members' windows, queued
+                // without host waits (they move members' state to their other buffer)
+                FM_HIP(hipMemcpyAsync(b->d_p1, b->h_p1, n_ * sizeof(P1Desc), hipMemcpyHostToDevice, b->stream));
+                TRY(launch_pass1_multi(b->stream, b->d_p1, b->d_t1, b->nblk1));
+                for (size_t k = 0; k < n_; ++k) {
+                    if (len[k] == 0 || !rb0[k]) continue;
+                    fm_ctx* c = b->m[k].c;
+                    c->rdisp_valid = false;
+                    const int old_bricks = brick_count(c);
+                    TRY(rebuild_lists_queued_internal(c));
+                    if (brick_count(c) != old_bricks) b->dirty = true;
+                    ++nrb;
+                }
+                if (b->dirty) TRY(upload_tables(b));
+            }
+            // ---- forces with the fused VV pass 2 of every member's step j
+            for (size_t k = 0; k < n_; ++k) {
+                size_t l = 0;
+                int kind = 0;
+                char* fa = slot_fargs(b, b->h_fargs, j) + k * fsz;
+                TRY(make_fargs(b->m[k].c, it0[k] + j, true, fa, &l, &kind));
+                fargs_set_queued(fa, b->m[k].c->d_flags, j < len[k]);
+                lds = std::max(lds, l);
+            }
+        }
+        // the window's other descriptors in two copies, then its launches
+        if (W > 1)
+            FM_HIP(hipMemcpyAsync(slot_p1(b, b->d_p1, 1), slot_p1(b, b->h_p1, 1), (size_t)(W - 1) * b->cap_m * sizeof(P1Desc),
+                                  hipMemcpyHostToDevice, b->stream));
+        FM_HIP(hipMemcpyAsync(b->d_fargs, b->h_fargs, (size_t)W * b->cap_m * fsz, hipMemcpyHostToDevice, b->stream));
+        TRY(launch_window(b, W, lds));
+        // ---- one synchronisation: every member's flag words
+        for (size_t k = 0; k < n; ++k) {
+            fm_ctx* c = b->m[k].c;
+            FM_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, FLAG_COUNT * sizeof(int), hipMemcpyDeviceToHost, b->stream));
         }
         FM_HIP(hipStreamSynchronize(b->stream));
-        // ---- per-member rebuild decisions and list rebuilds
+        ++b->windows;
         for (size_t k = 0; k < n; ++k) {
+            if (len[k] == 0) continue;
             Member& mm = b->m[k];
             fm_ctx* c = mm.c;
-            const P1Desc& d = b->h_p1[k];
-            if (d.check) c->rdisp_valid = true;
-            int rb = ((mm.iter + 1) % c->p.tplot) == 0;
-            if (d.check && b->h_bflags[k]) rb = 1;
-            if (c->p.rebuild_criteria == FM_REBUILD_FIXED && mm.iter % c->p.fixed_rebuild == 0) rb = 1;
-            if (rb) {
-                c->rdisp_valid = false;
-                const int old_bricks = brick_count(c);
-                TRY(rebuild_lists_internal(c));
-                if (brick_count(c) != old_bricks) b->dirty = true;
-                ++nrb;
+            TRY(check_flags_internal(c));
+            int done = len[k];
+            mm.fired = -1;
+            if (c->h_flags[FLAG_NEED]) {
+                // the window's list build met a region or list beyond the capacities: every launch
+                // of the member after it did nothing; build again, then the step's force
+                if (!rb0[k]) return fm_set_error(FM_ESTATE, "capacity request without a queued list build");
+                TRY(redo_lists_internal(c));
+                TRY(launch_forces(c, it0[k], true));
+                done = 1;
+            } else if (c->h_flags[FLAG_REBUILD]) {
+                const int s = c->h_flags[FLAG_RB_STEP];
+                if (s < it0[k] || s >= it0[k] + len[k] || (s == it0[k] && rb0[k]))
+                    return fm_set_error(FM_ESTATE, "rebuild requested at step %d outside %d..%d", s, it0[k],
+                                        it0[k] + len[k] - 1);
+                // steps before s are complete; s has had its pass 1 and opens the next window
+                done = s - it0[k];
+                mm.fired = s;
             }
-        }
-        if (b->dirty) TRY(upload_tables(b));
-        // ---- forces with the fused VV pass 2, all members in one launch
-        size_t lds = 0;
-        for (size_t k = 0; k < n; ++k) {
-            size_t l = 0;
-            int kind = 0;
-            TRY(make_fargs(b->m[k].c, b->m[k].iter, true, b->h_fargs + k * fsz, &l, &kind));
-            lds = std::max(lds, l);
-        }
-        if (b->use_graph) {
-            // a graph's LDS size is fixed: keep it until a member needs more, with headroom
-            if (!b->g2 || b->g2_nblk != b->nblk2 || b->g2_n != n || lds > b->g2_lds) {
-                if (b->g2) hipGraphExecDestroy(b->g2);
-                b->g2 = nullptr;
-                size_t glds = std::max(lds, lds + lds / 8);
-                size_t limit = b->m[0].c->lds_limit;
-                if (glds > limit) glds = std::max(lds, limit);
-                TRY(capture(b, &b->g2, 2, glds));
-                b->g2_nblk = b->nblk2;
-                b->g2_lds = glds;
-                b->g2_n = n;
+            for (int q = 0; q < done; ++q) {
+                const int it = it0[k] + q;
+                const bool rec = (it % c->p.tplot) == 0;
+                const bool ev = c->p.teval > 0 && (it % c->p.teval) == 0;
+                c->pe_valid = rec;
+                c->vir_valid = rec || ev;
+                c->simtime = c->simtime + c->p.delta_t;
             }
-            FM_HIP(hipGraphLaunch(b->g2, b->stream));
-        } else {
-            TRY(enqueue_forces(b, lds));
-        }
-        for (size_t k = 0; k < n; ++k) {
-            Member& mm = b->m[k];
-            fm_ctx* c = mm.c;
-            const bool rec = (mm.iter % c->p.tplot) == 0;
-            const bool ev = c->p.teval > 0 && (mm.iter % c->p.teval) == 0;
-            c->pe_valid = rec;
-            c->vir_valid = rec || ev;
-            c->simtime = c->simtime + c->p.delta_t;
-            mm.iter += 1;
+            mm.iter = it0[k] + done;
+            rem[k] -= done;
         }
     }
-    FM_HIP(hipStreamSynchronize(b->stream));
-    for (auto& mm : b->m) TRY(ctx_sync_flags(mm.c));
     b->rebuilds += nrb;
     if (n_rebuilds) *n_rebuilds = nrb;
     return 0;

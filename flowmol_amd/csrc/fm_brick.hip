@@ -849,6 +849,7 @@ struct FbArgs {
     const int* order;      // brick order of one-domain launches (brick_of_block)
     int* halt;             // flag words of a conditional launch (null: unconditional)
     int step;              // step whose forces this launch computes (the halt rule compares it)
+    int active;            // packed launches: 0 = this domain has no step in this launch
     double* cv;            // faces 1..3 of the control-volume tally, or null
     int cvnbo[3], cvnhb[3];
     double cvbs[3], cvhd[3];
@@ -1562,12 +1563,23 @@ k_forces_brick_plain(FbArgs a) {
     forces_body<false, REC, false, REC, P1, MIXED>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
+// A domain's step in a packed launch does nothing when the domain has no step in this launch, or
+// when a rebuild is pending for this step or an earlier one (the rule of halted(), read only:
+// a packed launch's first workgroup belongs to one domain, so none raises FLAG_HALT).
+__device__ inline bool skip_domain(const FbArgs& a) {
+    if (!a.active) return true;
+    const int* f = a.halt;
+    return f && (f[FLAG_HALT] != 0 || (__atomic_load_n(&f[FLAG_REBUILD], __ATOMIC_RELAXED) != 0 &&
+                                       __atomic_load_n(&f[FLAG_RB_STEP], __ATOMIC_RELAXED) <= a.step));
+}
+
 // packed launch over many domains: block b works on brick table[b].y of domain table[b].x
 template <bool MIE, bool EXTRA>
 __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_multi(const FbArgs* descs, const int2* table) {
     extern __shared__ __align__(16) char smem[];
     const int2 t = table[blockIdx.x];
     const FbArgs& a = descs[t.x];
+    if (skip_domain(a)) return;
     if (a.rec) forces_body<MIE, true, EXTRA>(a, t.y, smem);
     else if (a.ev) forces_body<MIE, false, EXTRA, true>(a, t.y, smem);
     else forces_body<MIE, false, EXTRA>(a, t.y, smem);
@@ -1698,6 +1710,7 @@ __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_mixed_multi(const FbA
     extern __shared__ __align__(16) char smem[];
     const int2 t = table[blockIdx.x];
     const FbArgs& a = descs[t.x];
+    if (skip_domain(a)) return;
     if (a.rec) forces_body_mixed<true, EXTRA>(a, t.y, smem);
     else if (a.ev) forces_body_mixed<false, EXTRA, true>(a, t.y, smem);
     else forces_body_mixed<false, EXTRA>(a, t.y, smem);
@@ -1884,6 +1897,13 @@ int launch_nlist(fm_ctx* c) {
 
 size_t fargs_bytes() { return sizeof(FbArgs); }
 
+// a domain's force arguments in a queued packed launch: halted by its own flag words, or idle
+void fargs_set_queued(void* fargs, int* flags, bool active) {
+    FbArgs* a = (FbArgs*)fargs;
+    a->halt = flags;
+    a->active = active ? 1 : 0;
+}
+
 // Arguments of the force kernel for one domain at step iter.  kind: bit0 mixed precision,
 // bit1 Mie tables, bit2 bonds or tethers present.
 int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* kind_out) {
@@ -1939,6 +1959,7 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     a.mie = mie ? c->d_mie : nullptr;
     a.order = c->brick_order;
     a.halt = nullptr;
+    a.active = 1;
     a.cv = (a.rec && c->cvface) ? c->cvface : nullptr;
     for (int d = 0; d < 3; ++d) {
         a.cvnbo[d] = c->cv_nbo[d];

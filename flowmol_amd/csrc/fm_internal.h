@@ -216,17 +216,22 @@ int launch_slot_to_ext_i32(fm_ctx* c, int* dst, const int* src, int nf);
 struct P1Desc {               // VV pass 1 (+ displacement check) of one domain
     double* st;               // state block (9*cap doubles)
     double* rd;               // rdisp (3*cap)
-    int* flag_out;            // rebuild flag of this domain
+    int* flags;               // the domain's flag words (halt rule, FLAG_REBUILD / FLAG_RB_STEP)
     int np, cap;
     int check, fresh;         // apply the displacement update / start it from zero
+    int active, step;         // the domain has a step in this launch; its number
     double dt, hdt2, hdt, thresh;
 };
 int launch_pass1_multi(hipStream_t st, const P1Desc* d_descs, const int2* d_table, int nblk);
 size_t fargs_bytes();
+void fargs_set_queued(void* fargs, int* flags, bool active);
 // force-kernel arguments of one domain for step iter; kind: bit0 mixed, bit1 Mie, bit2 bonds/tethers
 int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds, int* kind);
 int launch_forces_multi(hipStream_t st, int kind, const void* d_descs, const int2* d_table, int nblk, size_t lds);
 int rebuild_lists_internal(fm_ctx* c);
+int rebuild_lists_queued_internal(fm_ctx* c);  // the rebuild without host waits (capacity requests in FLAG_NEED)
+int redo_lists_internal(fm_ctx* c);            // after FLAG_NEED: grow the capacities, build the list again
+int check_flags_internal(fm_ctx* c);           // report a device-side error in the flag words read last
 
 // LDS-brick kernels (fm_brick.hip)
 int brick_count(const fm_ctx* c);

@@ -739,10 +739,19 @@ __global__ __launch_bounds__(FM_BLOCK) void k_slot_to_ext(T* dst, const T* src, 
 }
 
 // packed VV pass 1 (+ displacement check) over many domains: block b handles molecules
-// table[b].y*FM_BLOCK .. of domain table[b].x (same operations as k_pass1 / k_pass1_disp)
+// table[b].y*FM_BLOCK .. of domain table[b].x (same operations as k_pass1 / k_pass1_disp).  A
+// domain's pass does nothing when it has no step in this launch, or when a rebuild is pending
+// for an earlier step (FLAG_HALT, or FLAG_REBUILD with FLAG_RB_STEP below this step: a check of
+// this same launch stores this step, so its other workgroups still run); a check that fires
+// records this step.
 __global__ __launch_bounds__(FM_BLOCK) void k_pass1_multi(const P1Desc* descs, const int2* table) {
     const int2 t = table[blockIdx.x];
     const P1Desc& d = descs[t.x];
+    if (!d.active) return;
+    int* f = d.flags;
+    if (f[FLAG_HALT] != 0 || (__atomic_load_n(&f[FLAG_REBUILD], __ATOMIC_RELAXED) != 0 &&
+                              __atomic_load_n(&f[FLAG_RB_STEP], __ATOMIC_RELAXED) < d.step))
+        return;
     const int i = t.y * FM_BLOCK + threadIdx.x;
     bool hit = false;
     if (i < d.np) {
@@ -774,8 +783,10 @@ __global__ __launch_bounds__(FM_BLOCK) void k_pass1_multi(const P1Desc* descs, c
     }
     unsigned long long m = __ballot(hit);
     if (m != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1 &&
-        __atomic_load_n(d.flag_out, __ATOMIC_RELAXED) == 0)
-        atomicOr(d.flag_out, 1);
+        __atomic_load_n(&f[FLAG_REBUILD], __ATOMIC_RELAXED) == 0) {
+        atomicOr(&f[FLAG_REBUILD], 1);
+        atomicExch(&f[FLAG_RB_STEP], d.step);
+    }
 }
 
 }  // namespace

@@ -56,6 +56,12 @@ class Batch:
         check(load().fm_batch_run(self._h, int(nsteps), C.byref(nrb)))
         return nrb.value
 
+    def counters(self):
+        """(member list rebuilds, host synchronisations) since the batch was created."""
+        rb, ns = C.c_int64(), C.c_int64()
+        check(load().fm_batch_counters(self._h, C.byref(rb), C.byref(ns)))
+        return rb.value, ns.value
+
     def iter_of(self, ctx):
         it = C.c_int32()
         check(load().fm_batch_get_iter(self._h, ctx._h, C.byref(it)))
@@ -178,6 +184,8 @@ class ReplicaServer:
         results = {}
         t0 = time.perf_counter()
         atom_steps = 0
+        batch_steps = 0
+        _, syncs0 = self.batch.counters()
         while pending or active:
             now = time.perf_counter() - t0
             # admit what has arrived, in arrival order, while the budget allows
@@ -217,6 +225,7 @@ class ReplicaServer:
                 if a.req.snapshot_every > 0:
                     k = min(k, a.req.snapshot_every - a.done % a.req.snapshot_every)
             self.batch.run(k)
+            batch_steps += k
             atom_steps += k * sum(a.req.np for a in active)
             done = [a for a in active if a.remaining - k == 0]
             for a in active:
@@ -251,7 +260,8 @@ class ReplicaServer:
                  "wall_s": wall, "atom_steps": atom_steps,
                  "atom_steps_per_s": atom_steps / wall if wall > 0 else 0.0,
                  "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
-                 "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None}
+                 "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,
+                 "batch_steps": batch_steps, "host_syncs": self.batch.counters()[1] - syncs0}
         return results, stats
 
     def close(self):

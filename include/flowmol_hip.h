@@ -289,7 +289,11 @@ int  fm_batch_use_graphs(fm_batch* b, int32_t enable);
 /* ctx must have its initial neighbour list and forces; iter0 = iteration number of its next step */
 int  fm_batch_add(fm_batch* b, fm_ctx* ctx, int32_t iter0);
 int  fm_batch_remove(fm_batch* b, fm_ctx* ctx);
+/* nsteps velocity Verlet steps of every member, queued in windows of up to 16 steps with one
+   host synchronisation per window (rebuild decisions on the device; see fm_batch.hip) */
 int  fm_batch_run(fm_batch* b, int32_t nsteps, int32_t* n_rebuilds);
+/* totals since fm_batch_create: member list rebuilds, and host synchronisations (windows) */
+int  fm_batch_counters(const fm_batch* b, int64_t* rebuilds, int64_t* host_syncs);
 int  fm_batch_size(const fm_batch* b, int32_t* n);
 int  fm_batch_get_iter(const fm_batch* b, const fm_ctx* ctx, int32_t* iter);
 /* device bytes fm_init would allocate (admission control by HBM footprint) */

@@ -42,6 +42,10 @@ def test_batch_members_match_alone(graphs):
     b.run(20)
     b.add(ctx[2], 1)      # arrives while the others are on the GPU
     b.run(40)
+    # rebuild decisions on the device: one host synchronisation per window of queued steps
+    nrb, syncs = b.counters()
+    say(f"  60 steps: {nrb} member rebuilds, {syncs} host synchronisations")
+    assert 0 < syncs <= 30 and nrb > 0
     for k, (c, q) in enumerate(zip(ctx, reqs)):
         r, v, _ = c.state()
         assert np.array_equal(r, ref[k][0]), k
