@@ -49,7 +49,11 @@ def parse():
     ap.add_argument("--rate", type=float, default=6.0, help="serve: Poisson arrival rate (1/s)")
     ap.add_argument("--replica-steps", type=int, default=2000, help="serve: steps per replica")
     ap.add_argument("--serve-graphs", choices=["on", "off"], default="off",
-                    help="serve: replay each window step's packed launches as HIP graphs")
+                    help="serve: replay each window's packed launches (after its rebuilds) as one HIP graph")
+    ap.add_argument("--serve-tune", choices=["on", "off"], default="on",
+                    help="serve: brick shape per replica grid from the brick tuner, before the stream starts")
+    ap.add_argument("--c4-tune", choices=["on", "off"], default="on",
+                    help="c4: brick shape from the brick tuner (fresh contexts) first")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a plain device-to-device copy of this size (PMC byte calibration)")
     return ap.parse_args()
@@ -93,6 +97,11 @@ def serve_main(a):
     # warm-up: one small replica through the same path (kernel loading, graph capture)
     warm = poisson_workload(1, 100.0, seed=99, nmin=4096, nmax=4096, nsteps=50)
     srv = ReplicaServer(chunk=50, graphs=a.serve_graphs == "on")
+    t_tune = 0.0
+    if a.serve_tune == "on":
+        t_tune = time.perf_counter()
+        srv.tune(reqs)
+        t_tune = time.perf_counter() - t_tune
     srv.serve(warm, realtime=False)
     drift = []
 
@@ -114,6 +123,7 @@ def serve_main(a):
         "wall_s": stats["wall_s"],
         "replicas": stats["replicas"],
         "batch_steps": stats["batch_steps"],
+        "brick_tuning": {"on": a.serve_tune == "on", "grids": len(srv.bricks), "seconds_before_stream": t_tune},
         "host_syncs_per_batch_step": stats["host_syncs"] / max(1, stats["batch_steps"]),
         "dtype": "f64",
         "data": "synthetic",
@@ -186,9 +196,20 @@ def c4_main(a):
                                                    init="random_walk")
     t_gen = time.perf_counter() - t_gen
     N = r.shape[0]
+
+    def prepare(ctx):
+        ctx.set_moltype(moltype)
+        ctx.set_monomers(*mono)
+
+    brick = "default"
+    if a.autotune or a.c4_tune == "on":
+        from flowmol_amd.md import tune_brick
+        best, tuned = tune_brick(p, r, v, prepare=prepare, max_candidates=6)
+        p.brick[0], p.brick[1], p.brick[2] = best
+        brick = "autotuned %dx%dx%d (%s)" % (best + (", ".join("%dx%dx%d %.3f ms" % (b + (t * 1e3,))
+                                                               for b, t in sorted(tuned.items(), key=lambda x: x[1])),))
     g = MDContext(p, r, v)
-    g.set_moltype(moltype)
-    g.set_monomers(*mono)
+    prepare(g)
     g.setup(0)
     K = a.c4_steps
     W = max(1, K // 10)
@@ -209,7 +230,7 @@ def c4_main(a):
                                "at density 0.85, "
                                "WCA all pairs, Mie 14-6 (rc 2.5) within species 1 (half the chains), dt 0.002, "
                                "TPLOT 10, displacement rebuild", "molecules": N,
-                   "init_generation_s": round(t_gen, 1)},
+                   "init_generation_s": round(t_gen, 1), "brick": brick},
     }), flush=True)
 
 

@@ -337,10 +337,12 @@ def brick_candidates(ncells, nmol, max_candidates=4):
     return [b for _, b in sorted(out, key=lambda t: -t[0])][:max(1, int(max_candidates))]
 
 
-def tune_brick(params, r, v, a=None, steps=20, warmup=5, max_candidates=4):
+def tune_brick(params, r, v, a=None, steps=20, warmup=5, max_candidates=4, prepare=None):
     """Time `steps` fused velocity Verlet steps (after `warmup` more, list rebuilds included) of the
     same state on a fresh context per candidate brick shape; returns (fastest shape, {shape:
-    seconds per step}).  `params` is not changed: set params.brick to the shape to use it."""
+    seconds per step}).  `params` is not changed: set params.brick to the shape to use it.
+    prepare(ctx), when given, loads what the state needs beyond r, v, a (molecule types, monomer
+    data, tags) into each fresh context before its setup."""
     import time
 
     probe = MDContext(params, r, v, a)
@@ -352,6 +354,8 @@ def tune_brick(params, r, v, a=None, steps=20, warmup=5, max_candidates=4):
         p.brick[0], p.brick[1], p.brick[2] = b
         g = MDContext(p, r, v, a)
         try:
+            if prepare is not None:
+                prepare(g)
             g.setup(0)
             g.run_vv(1, warmup)
             g.synchronize()

@@ -148,11 +148,24 @@ class Active:
         self.last_snapshot = None
 
 
-def make_context(req):
+def grid_key(req):
+    """The parameters that fix a request's cell grid (the brick tuner's table key)."""
+    p = req.params
+    return (tuple(round(float(p.domain[d]), 12) for d in range(3)), round(float(p.rcutoff), 12),
+            round(float(p.delta_rneighbr[0]), 12), req.np)
+
+
+def make_context(req, brick=None):
+    """A request's context with its initial lists and forces; brick: the brick shape to use
+    (None: the library's default for its grid)."""
     if req.snapshot is not None:
         from .checkpoint import resume
         return resume(req.snapshot)
-    ctx = MDContext(req.params, req.r, req.v)
+    params = req.params
+    if brick is not None:
+        params = type(params).from_buffer_copy(params)
+        params.brick[0], params.brick[1], params.brick[2] = brick
+    ctx = MDContext(params, req.r, req.v)
     if req.tag is not None:
         ctx.set_tags(req.tag, req.rtether)
     ctx.setup(0)
@@ -170,10 +183,30 @@ def result_of(ctx):
 class ReplicaServer:
     """Continuous-batching request loop for replicas (see the module docstring)."""
 
-    def __init__(self, hbm_budget_bytes=200 * 2**30, chunk=50, graphs=True):
+    def __init__(self, hbm_budget_bytes=200 * 2**30, chunk=50, graphs=True, bricks=None):
         self.budget = int(hbm_budget_bytes)
         self.chunk = int(chunk)
         self.batch = Batch(graphs=graphs)
+        self.bricks = {} if bricks is None else dict(bricks)   # grid_key -> brick shape
+
+    def tune(self, requests, steps=10, warmup=3, max_candidates=3):
+        """Brick shapes for the requests' cell grids, chosen by the brick tuner (flowmol_amd.md.
+        tune_brick) on fresh contexts of one request per grid, kept for the contexts the server
+        creates; returns the table (grid_key -> shape)."""
+        from .md import tune_brick
+
+        for q in requests:
+            if q.snapshot is not None or grid_key(q) in self.bricks:
+                continue
+
+            def prepare(ctx, q=q):
+                if q.tag is not None:
+                    ctx.set_tags(q.tag, q.rtether)
+
+            best, _ = tune_brick(q.params, q.r, q.v, steps=steps, warmup=warmup, max_candidates=max_candidates,
+                                 prepare=prepare)
+            self.bricks[grid_key(q)] = best
+        return dict(self.bricks)
 
     def serve(self, requests, realtime=True, on_result=None):
         """Run the requests to completion.  realtime: arrivals follow the wall clock from the
@@ -191,7 +224,7 @@ class ReplicaServer:
             # admit what has arrived, in arrival order, while the budget allows
             while pending and (not realtime or pending[0].arrival <= now) and used + pending[0].footprint <= self.budget:
                 q = pending.pop(0)
-                ctx = make_context(q)
+                ctx = make_context(q, self.bricks.get(grid_key(q)))
                 self.batch.add(ctx, q.iter0)
                 used += q.footprint
                 active.append(Active(q, ctx, time.perf_counter() - t0))
