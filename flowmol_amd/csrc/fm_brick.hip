@@ -359,6 +359,14 @@ __device__ inline int get_bondflag(const int* mono, int cap, int i, int scID) {
     int w = mono[i + (size_t)(2 + group) * cap];
     return (w >> expo) & 1;
 }
+// the same from the molecule's four bin_bflag words held in registers
+__device__ inline int bondflag_of(const int (&bf)[4], int scID) {
+    const int group = (int)ceilf((float)scID / 31.0f);
+    int expo = scID % 31 - 1;
+    if (expo == -1) expo = 30;
+    const int w = group == 1 ? bf[0] : group == 2 ? bf[1] : group == 3 ? bf[2] : bf[3];
+    return (group >= 1 && group <= 4) ? (w >> expo) & 1 : 0;
+}
 
 template <bool POLY>
 __global__ __launch_bounds__(FB_BLOCK) void k_nlist_brick(NlbArgs a) {
@@ -588,9 +596,13 @@ __device__ inline int quantise(double x, double o, double inv_h) {
 // force kernels').  POLY: the bonds of the chains are found among the list entries (a bonded
 // partner lies within R0 <= r_n, so it is an entry), as check_update_adjacentbeadinfo finds them
 // during the reference's scan (src/linklist.f90:2542-2584), from the chain ids staged with the
-// positions.
+// positions: the low 17 bits of each chain id ride in the spare bits of the quantised z, so the
+// candidate test also marks the candidates of the molecule's own chain, and only hits among
+// those read the full chain and subchain ids and test the molecule's bond flags (held in
+// registers).  With the chain ids the region's LDS allows two workgroups per CU, so POLY has
+// the registers of two.
 template <bool POLY>
-__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(3 * FB_BLOCK / 256))) void
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(POLY ? FB_BLOCK / 128 : 3 * FB_BLOCK / 256))) void
 k_nlist_quant(NlqArgs a) {
     extern __shared__ __align__(16) char smem[];
     const BG& g = a.g;
@@ -650,7 +662,9 @@ k_nlist_quant(NlqArgs a) {
             const int xq = quantise(img(px[k], sx, g.L[0]), ox, a.inv_h);
             const int yq = quantise(img(py[k], sy, g.L[1]), oy, a.inv_h);
             const int zq = quantise(img(pz[k], sz, g.L[2]), oz, a.inv_h);
-            qv[q] = make_uint2((uint32_t)xq | ((uint32_t)yq << 16), (uint32_t)zq);
+            // (POLY: z | low 17 bits of the chain id << 15)
+            qv[q] = make_uint2((uint32_t)xq | ((uint32_t)yq << 16),
+                               (uint32_t)zq | (POLY ? ((uint32_t)a.mono[sl[k]] & 0x1ffffu) << 15 : 0u));
             ls[q] = (sl[k] << 5) | sc[k];
             if (POLY) {
                 lchain[q] = a.mono[sl[k]];
@@ -675,11 +689,15 @@ k_nlist_quant(NlqArgs a) {
             const double xi = a.s.x[slot], yi = a.s.y[slot], zi = a.s.z[slot];
             const uint2 wi = qv[li];
             const s16x2 ixy = __builtin_bit_cast(s16x2, wi.x);
-            const int izq = (int)wi.y;
+            const int izq = (int)(wi.y & 0x7fffu);
+            const uint32_t ich = wi.y >> 15;
             int nb = 0, chain = 0, funcy = 0;
+            int bf[4] = {0, 0, 0, 0};
             if (POLY) {
                 chain = lchain[li];
                 funcy = a.mono[slot + 2 * cap];
+#pragma unroll
+                for (int g4 = 0; g4 < 4; ++g4) bf[g4] = a.mono[slot + (size_t)(3 + g4) * cap];
             }
             // faces of the molecule's cell in y and z, as in k_nlist_brick
             const int vy = by * g.B[1] + ry - 1, vz = bz * g.B[2] + rz - 1;
@@ -741,14 +759,15 @@ k_nlist_quant(NlqArgs a) {
                     const int nn = min(32, je - j0);
                     // each test shifts its two bits in from the bottom (the first candidate ends
                     // highest); one bit reversal afterwards puts candidate u at bit u
-                    uint32_t m = 0, b = 0;
+                    uint32_t m = 0, b = 0, sc = 0;
                     auto test = [&](int j) {
                         const uint2 wj = qv[j];
                         const s16x2 d1 = ixy - __builtin_bit_cast(s16x2, wj.x);
-                        const int dz = izq - (int)wj.y;
+                        const int dz = izq - (int)(POLY ? wj.y & 0x7fffu : wj.y);
                         const int r2q = __builtin_amdgcn_sdot2(d1, d1, __mul24(dz, dz), false);
                         m = m + m + (uint32_t)(r2q < a.lo);
                         b = b + b + (uint32_t)(r2q < a.hi);
+                        if (POLY) sc = sc + sc + (uint32_t)((wj.y >> 15) == ich);
                     };
                     int u = 0;
                     for (; u + 4 <= nn; u += 4) {
@@ -760,6 +779,7 @@ k_nlist_quant(NlqArgs a) {
                     for (; u < nn; ++u) test(j0 + u);
                     m = __builtin_bitreverse32(m) >> (32 - nn);
                     b = __builtin_bitreverse32(b) >> (32 - nn);
+                    if (POLY) sc = __builtin_bitreverse32(sc) >> (32 - nn);
                     if (li >= j0 && li < j0 + nn) {
                         m &= ~(1u << (li - j0));
                         b &= ~(1u << (li - j0));
@@ -777,15 +797,22 @@ k_nlist_quant(NlqArgs a) {
                         if (r2 < a.ra2) m |= 1u << u;
                     }
                     // entries enter the word from the top: after four, the first is in bits 0-15
+                    // (POLY: the hits of the molecule's own chain, for the bond test below)
+                    const uint32_t same = POLY ? m & sc : 0u;
                     while (m) {
                         const int j = j0 + __builtin_ctz(m);
                         m &= m - 1;
                         word = (word >> 16) | ((uint64_t)(uint32_t)j << 48);
                         if ((cnt & 3) == 3 && cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word;
                         ++cnt;
-                        if (POLY && lchain[j] == chain && get_bondflag(a.mono, a.cap, slot, lsub[j])) {
-                            if (nb < funcy && nb < FM_MAX_FUNCY) a.bond[(size_t)nb * cap + slot] = (uint32_t)ls[j];
-                            ++nb;
+                    }
+                    if (POLY) {
+                        for (uint32_t h = same; h; h &= h - 1) {
+                            const int j = j0 + __builtin_ctz(h);
+                            if (lchain[j] == chain && bondflag_of(bf, lsub[j])) {
+                                if (nb < funcy && nb < FM_MAX_FUNCY) a.bond[(size_t)nb * cap + slot] = (uint32_t)ls[j];
+                                ++nb;
+                            }
                         }
                     }
                 }
