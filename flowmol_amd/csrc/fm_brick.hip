@@ -595,30 +595,22 @@ __device__ inline int quantise(double x, double o, double inv_h) {
 // registers held to what lets three workgroups share a CU (its LDS region is a third of the
 // force kernels').  POLY: the bonds of the chains are found among the list entries (a bonded
 // partner lies within R0 <= r_n, so it is an entry), as check_update_adjacentbeadinfo finds them
-// during the reference's scan (src/linklist.f90:2542-2584), from the chain ids staged with the
-// positions: the low 17 bits of each chain id ride in the spare bits of the quantised z, so the
-// candidate test also marks the candidates of the molecule's own chain, and only hits among
-// those read the full chain and subchain ids and test the molecule's bond flags (held in
-// registers).  With the chain ids the region's LDS allows two workgroups per CU, so POLY has
-// the registers of two.
+// during the reference's scan (src/linklist.f90:2542-2584): the low 17 bits of each chain id
+// ride in the spare bits of the quantised z, so the candidate test also marks the candidates of
+// the molecule's own chain, and only hits among those read the full chain and subchain ids (from
+// HBM, by the staged slot) and test the molecule's bond flags (held in registers).  The region's
+// LDS is then that of the plain build, and so are its three workgroups per CU.
 template <bool POLY>
-__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(POLY ? FB_BLOCK / 128 : 3 * FB_BLOCK / 256))) void
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(3 * FB_BLOCK / 256))) void
 k_nlist_quant(NlqArgs a) {
     extern __shared__ __align__(16) char smem[];
     const BG& g = a.g;
-    uint2* qv = (uint2*)smem;               // maxreg: x | y << 16, z
+    uint2* qv = (uint2*)smem;               // maxreg: x | y << 16, z (POLY: | chain id << 15)
     int* ls = (int*)(qv + a.maxreg);        // maxreg: slot << 5 | image shift code
     Region r;
     r.pos = nullptr;
     r.lslot = nullptr;
     int* ip = ls + a.maxreg;
-    int* lchain = nullptr;                  // POLY: chain and subchain id of each staged molecule
-    int* lsub = nullptr;
-    if (POLY) {
-        lchain = ip;
-        lsub = lchain + a.maxreg;
-        ip = lsub + a.maxreg;
-    }
     r.rq = (uint16_t*)ip;
     ip += (a.maxreg + 1) / 2;
     r.off = ip;
@@ -639,7 +631,6 @@ k_nlist_quant(NlqArgs a) {
     const double oy = -a.hd[1] + (by * g.B[1] - 1) * a.cs[1] - 2.0 * a.h;
     const double oz = -a.hd[2] + (bz * g.B[2] - 1) * a.cs[2] - 2.0 * a.h;
     constexpr int SU = 6;  // molecules per thread and pass, loads first (as stage_region)
-    const size_t cap0 = a.cap;
     for (int q0 = threadIdx.x; q0 < r.total; q0 += SU * blockDim.x) {
         int sl[SU], sc[SU];
         double px[SU], py[SU], pz[SU];
@@ -666,10 +657,6 @@ k_nlist_quant(NlqArgs a) {
             qv[q] = make_uint2((uint32_t)xq | ((uint32_t)yq << 16),
                                (uint32_t)zq | (POLY ? ((uint32_t)a.mono[sl[k]] & 0x1ffffu) << 15 : 0u));
             ls[q] = (sl[k] << 5) | sc[k];
-            if (POLY) {
-                lchain[q] = a.mono[sl[k]];
-                lsub[q] = a.mono[sl[k] + cap0];
-            }
         }
     }
     __syncthreads();
@@ -694,7 +681,7 @@ k_nlist_quant(NlqArgs a) {
             int nb = 0, chain = 0, funcy = 0;
             int bf[4] = {0, 0, 0, 0};
             if (POLY) {
-                chain = lchain[li];
+                chain = a.mono[slot];
                 funcy = a.mono[slot + 2 * cap];
 #pragma unroll
                 for (int g4 = 0; g4 < 4; ++g4) bf[g4] = a.mono[slot + (size_t)(3 + g4) * cap];
@@ -809,7 +796,8 @@ k_nlist_quant(NlqArgs a) {
                     if (POLY) {
                         for (uint32_t h = same; h; h &= h - 1) {
                             const int j = j0 + __builtin_ctz(h);
-                            if (lchain[j] == chain && bondflag_of(bf, lsub[j])) {
+                            const int sj = ls[j] >> 5;  // the candidate's slot: its chain and subchain ids
+                            if (a.mono[sj] == chain && bondflag_of(bf, a.mono[sj + cap])) {
                                 if (nb < funcy && nb < FM_MAX_FUNCY) a.bond[(size_t)nb * cap + slot] = (uint32_t)ls[j];
                                 ++nb;
                             }
@@ -1904,7 +1892,7 @@ int launch_nlist(fm_ctx* c) {
         q.lo = (int)floor(t * t - 2.0 * r3 * eps * t - 1e-9 * t * t) - 2;
         q.hi = (int)ceil(t * t * (1.0 + 3e-9) + 2.0 * r3 * eps * t * (1.0 + 1e-9) + 3.0 * eps * eps) + 2;
         const int nbmax = g.B[0] * g.B[1] * g.B[2];
-        const size_t lds = (size_t)c->maxreg * (sizeof(uint2) + sizeof(int) + (a.mono ? 2 * sizeof(int) : 0)) +
+        const size_t lds = (size_t)c->maxreg * (sizeof(uint2) + sizeof(int)) +
                            (size_t)((c->maxreg + 1) / 2 + 4 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int);
         ktimer_begin(c, KT_NLIST);
         if (a.mono) k_nlist_quant<true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(q);
