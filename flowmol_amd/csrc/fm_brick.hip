@@ -1418,12 +1418,25 @@ struct MopI {             // the i-molecule's side of the MOP test
     bool in_i;            // molecule i strictly inside the box in x and z
     double* spxy;
     const double* planes;
+    // SLOTS: the (at most three) planes a pair of molecule i can cross, kb .. kb+2 (0-based plane
+    // numbers; a slot outside 0 .. nplanes-1 holds 1e300, where no pair crosses), their y, the
+    // sign of y_i against each, and the thread's sums of the contributions to each
+    int kb, sneg;          // sneg bit s: y_i lies below slot s's plane
+    double pl[3];
+    double c[3][3];
 };
+// SLOTS (when 2 rc < 3 planespacing, so a pair inside rc crosses at most three planes near
+// molecule i): the MOP tally without branches.  For each slot the reference's sign factor
+// sign(y_i - plane) - sign(y_j - plane) is zero unless the plane lies between the two molecules,
+// so every entry adds (acc r_ij) x (sign factor) to each slot's sums, with acc zero for an entry
+// beyond rc or not tallied from this end; the entries with a molecule near the box's x or z
+// faces (whose intercept factors need the reference's divisions) go through mop_entry_fast.
+template <bool SLOTS, int NQ = 4>
 __device__ __forceinline__ void lj_word_mop(const FbArgs& a, const P4K& p, int left, double xi, double yi,
-                                            double zi, const MopI& m, double& fx, double& fy, double& fz,
+                                            double zi, MopI& m, double& fx, double& fy, double& fz,
                                             double& pe, double& vt) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
         const double rx = xi - p.x[q], ry = yi - p.y[q], rz = zi - p.z[q];
         const double r2 = (rx * rx + ry * ry) + rz * rz;
         const bool in = r2 < a.rc2 && q < left;
@@ -1439,11 +1452,24 @@ __device__ __forceinline__ void lj_word_mop(const FbArgs& a, const P4K& p, int l
         vt = fmaf64(t, r2, vt);
         const double e = 4.0 * (x3 * x3 - x3) - a.potshift;
         pe = pe + (in ? e : 0.0);
+        const int kj = p.k[q];
+        const bool own = ((m.keyi | kj) & 1) != 0 || (m.keyi >> 1) > (kj >> 1);
+        if constexpr (SLOTS) {
+            const bool ins = m.in_i && fabs(p.x[q]) < m.bx && fabs(p.z[q]) < m.bz;
+            // (t is zero for an entry beyond rc)
+            const double af = (own && ins) ? 24.0 * t : 0.0;
+            const double v[3] = {af * rx, af * ry, af * rz};
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                const double f = (((m.sneg >> s) & 1) ? -1.0 : 1.0) - copysign(1.0, p.y[q] - m.pl[s]);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) m.c[s][d] = fmaf64(v[d], f, m.c[s][d]);
+            }
+            if (!(in && own && !ins)) continue;
+        }
         // the whole numbers strictly between t_i and t_j, widened (see mop_entry_fast)
         const double tj = (p.y[q] + m.hps) * m.inv_sp;
         const int k0 = (int)ceil(fmin(m.ti, tj) - 1e-9), k1 = (int)floor(fmax(m.ti, tj) + 1e-9);
-        const int kj = p.k[q];
-        const bool own = ((m.keyi | kj) & 1) != 0 || (m.keyi >> 1) > (kj >> 1);
         if (in && own && k0 <= k1 && k1 >= 0 && k0 < m.nplanes) {
             // the pair law's magnitude 48 (x^7 - x^4/2) as the force sum takes it (within 1e-15)
             const double acc = 24.0 * t;
@@ -1455,18 +1481,34 @@ __device__ __forceinline__ void lj_word_mop(const FbArgs& a, const P4K& p, int l
 }
 
 // The list loop with the MOP test per entry: one word at a time, the next word's index in flight.
+// With SLOTS the word's entries go one at a time (the nine plane sums hold registers that four
+// staged entries would need).
+template <bool SLOTS>
 __device__ __forceinline__ void lj_plain_mop(const FbArgs& a, const double* pos, const int* key,
                                              const uint64_t* row, int n, size_t cap, double xi, double yi,
-                                             double zi, const MopI& m, double& fx, double& fy, double& fz,
+                                             double zi, MopI& m, double& fx, double& fy, double& fz,
                                              double& pe_out, double& vir_out) {
     double sx = 0, sy = 0, sz = 0, pe = 0.0, vt = 0.0;
     uint64_t wn = row[0];
     for (int kw = 0; kw < n; kw += 4) {
         const uint64_t w = wn;
         if (kw + 4 < n) wn = row[(size_t)((kw >> 2) + 1) * cap];
-        P4K p;
-        lj_stage_k(p, pos, key, w);
-        lj_word_mop(a, p, n - kw, xi, yi, zi, m, sx, sy, sz, pe, vt);
+        if constexpr (SLOTS) {
+#pragma unroll 1
+            for (int q = 0; q < 4; ++q) {
+                P4K p;
+                const int j = (int)((w >> (16 * q)) & 0xffffu);
+                p.x[0] = pos[3 * j];
+                p.y[0] = pos[3 * j + 1];
+                p.z[0] = pos[3 * j + 2];
+                p.k[0] = key[j];
+                lj_word_mop<true, 1>(a, p, n - kw - q, xi, yi, zi, m, sx, sy, sz, pe, vt);
+            }
+        } else {
+            P4K p;
+            lj_stage_k(p, pos, key, w);
+            lj_word_mop<SLOTS>(a, p, n - kw, xi, yi, zi, m, sx, sy, sz, pe, vt);
+        }
     }
     fx = 24.0 * sx;
     fy = 24.0 * sy;
@@ -1475,6 +1517,7 @@ __device__ __forceinline__ void lj_plain_mop(const FbArgs& a, const double* pos,
     vir_out = 24.0 * vt;
 }
 
+template <bool SLOTS>
 __device__ __forceinline__ void forces_body_mop(const FbArgs& a, int brick, char* smem) {
     Region r;
     carve(r, smem, a.g, a.maxreg, true);
@@ -1498,6 +1541,7 @@ __device__ __forceinline__ void forces_body_mop(const FbArgs& a, int brick, char
     m.nplanes = a.nplanes;
     m.spxy = spxy;
     m.planes = planes;
+    const double rct = sqrt(a.rc2) * m.inv_sp;  // rc in plane spacings
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
         int rci, li, i;
         imol(r, a.start, q, rci, li, i);
@@ -1505,8 +1549,30 @@ __device__ __forceinline__ void forces_body_mop(const FbArgs& a, int brick, char
         m.ti = (yi + m.hps) * m.inv_sp;
         m.keyi = r.lslot[li];
         m.in_i = fabs(xi) < m.bx && fabs(zi) < m.bz;
+        if constexpr (SLOTS) {
+            // the planes strictly within rc of y_i: t in (t_i - rc/spacing, t_i + rc/spacing)
+            m.kb = (int)floor(m.ti - rct - 1e-9) + 1;
+            m.sneg = 0;
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                const int k = m.kb + s;
+                m.pl[s] = (k >= 0 && k < a.nplanes) ? planes[k] : 1e300;
+                m.sneg |= (copysign(1.0, yi - m.pl[s]) < 0.0 ? 1 : 0) << s;
+                m.c[s][0] = m.c[s][1] = m.c[s][2] = 0.0;
+            }
+        }
         double fx, fy, fz, pe, vir;
-        lj_plain_mop(a, r.pos, r.lslot, a.nbl + i, a.nbn[i], cap, xi, yi, zi, m, fx, fy, fz, pe, vir);
+        lj_plain_mop<SLOTS>(a, r.pos, r.lslot, a.nbl + i, a.nbn[i], cap, xi, yi, zi, m, fx, fy, fz, pe, vir);
+        if constexpr (SLOTS) {
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                const int k = m.kb + s;
+                if (k >= 0 && k < a.nplanes)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        if (m.c[s][d] != 0.0) atomicAdd(&spxy[3 * k + d], m.c[s][d]);
+            }
+        }
         a.s.ax[i] = fx;
         a.s.ay[i] = fy;
         a.s.az[i] = fz;
@@ -1559,11 +1625,12 @@ k_forces_brick_lj(FbArgs a) {
 
 // record steps with the MOP tally only (forces_body_mop): two workgroups per CU, as the LDS of
 // two regions allows
+template <bool SLOTS>
 __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128))) void
 k_forces_brick_mop(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt, a.step)) return;
-    forces_body_mop(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+    forces_body_mop<SLOTS>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // LJ steps without tallies, bonds or tethers (the common step): registers held to what lets two
@@ -2031,7 +2098,9 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
     // a record step whose only tally beyond PE and virial is MOP: the one-pass FP64 kernel (for
     // both precisions)
     if ((kind == 0 || kind == 1) && a.rec && a.pxy != nullptr && a.cv == nullptr && a.rf == nullptr) {
-        k_forces_brick_mop<<<grid, blk, lds, c->stream>>>(a);
+        // a pair inside rc crosses at most three planes near molecule i when 2 rc < 3 spacings
+        if (2.0 * sqrt(a.rc2) / a.spacing < 3.0 - 1e-6) k_forces_brick_mop<true><<<grid, blk, lds, c->stream>>>(a);
+        else k_forces_brick_mop<false><<<grid, blk, lds, c->stream>>>(a);
         ktimer_end(c, KT_FORCES);
         LAUNCH_CHECK();
         return 0;
