@@ -38,11 +38,11 @@ def say(msg):
     sys.stdout.flush()
 
 
-def couette(nunits, precision, lfv=True, tplot=1):
+def couette(nunits, precision, lfv=True, tplot=1, nplanes=64):
     """The Couette configuration with every molecule moved off its lattice site by a seeded
     uniform offset in [-0.05, 0.05] per component (on the perfect lattice the pair forces cancel
     to rounding, which leaves a relative force error undefined)."""
-    p, r, v, tag, rt = config.config_couette(nunits=nunits, tplot=tplot, nplanes=64, precision=precision)
+    p, r, v, tag, rt = config.config_couette(nunits=nunits, tplot=tplot, nplanes=nplanes, precision=precision)
     if lfv:
         p.integration_algorithm = FM_INTEGRATION_LFV
     r = r + np.random.default_rng(17).uniform(-0.05, 0.05, r.shape)
@@ -94,10 +94,13 @@ def lfv_steps(o, g, it0, n):
     return ro, rg
 
 
-def test_c3_fp64_leg_vs_oracle():
-    """13,824 molecules, leap-frog, tethered walls, MOP on 64 planes every step, 30 steps."""
-    p, r, v, tag, rt = couette((12, 12, 24), 0)
-    say("case: C3 FP64 leg vs oracle, 13824 molecules, 30 LFV steps")
+@pytest.mark.parametrize("nplanes", [64, 16])
+def test_c3_fp64_leg_vs_oracle(nplanes):
+    """13,824 molecules, leap-frog, tethered walls, MOP every step, 30 steps.  64 planes (spacing
+    0.64: a pair crosses up to four, the record kernel's candidate-range path) and 16 planes
+    (spacing 2.57 > 2 rc / 3: the kernel's three plane slots per molecule)."""
+    p, r, v, tag, rt = couette((12, 12, 24), 0, nplanes=nplanes)
+    say(f"case: C3 FP64 leg vs oracle, 13824 molecules, {nplanes} planes, 30 LFV steps")
     o = Oracle(p, r, v)
     o.set_tags(tag, rt)
     o.setup()
