@@ -1148,7 +1148,7 @@ static int rebuild_lists(fm_ctx* c) {
 
 int rebuild_lists_internal(fm_ctx* c) { return rebuild_lists(c); }
 
-static int rebuild_lists_queued(fm_ctx* c);
+static int rebuild_lists_queued(fm_ctx* c, bool resets, bool keep_acc);
 
 // The rebuild of a fused loop whose list one force call reads, at the positions it is built from
 // (the next step rebuilds before its force, and that step belongs to this loop): the pairs that
@@ -1160,7 +1160,7 @@ static int rebuild_lists_queued(fm_ctx* c);
 static int rebuild_lists_for(fm_ctx* c, bool read_once, bool queued) {
     const bool skinless = read_once && c->p.potential_flag == 0 && c->p.mie_potential == 0;
     c->list_ra2 = skinless ? c->rc2 * (1.0 + 1e-12) : 0.0;
-    const int rc = queued ? rebuild_lists_queued(c) : rebuild_lists(c);
+    const int rc = queued ? rebuild_lists_queued(c, true, false) : rebuild_lists(c);
     c->list_ra2 = 0.0;
     return rc;
 }
@@ -1176,18 +1176,21 @@ static bool next_rebuild_certain(const fm_ctx* c, int it) {
 // region and list capacities of the last measured build, and a region or list beyond them raises
 // FLAG_NEED and FLAG_HALT, so the conditional launches queued after it do nothing until
 // redo_lists() has grown the capacities at the next synchronisation.
-static int rebuild_lists_queued(fm_ctx* c) {
+// resets: clear FLAG_REBUILD, FLAG_HALT, FLAG_NEED and the measured maxima first (a window with
+// one rebuild at its start); without, a halt raised by an earlier build of the window stays.
+// keep_acc: permute the accelerations too (no force call follows before the state is read).
+static int rebuild_lists_queued(fm_ctx* c, bool resets = true, bool keep_acc = false) {
     static const int reset[] = {FLAG_REBUILD, FLAG_HALT, FLAG_NEED, FLAG_MAX_REGION, FLAG_MAX_NB};
     if (c->maxreg <= 0) {  // no measured capacity yet: the synchronous build
         TRY(launch_reset_flags(c, reset, 2));
         return rebuild_lists(c);
     }
     c->cells_valid = c->nlist_valid = false;
-    TRY(launch_sendmols(c, reset, 5));
+    TRY(launch_sendmols(c, reset, resets ? 5 : 0));
     TRY(fm_sort_mols(c));
     // every launch queued after this rebuild (the force launch, or nothing until the lists are
     // rebuilt) recomputes all accelerations before reading them
-    TRY(launch_bin(c, false));
+    TRY(launch_bin(c, keep_acc));
     c->moved_since_bin = false;
     c->cells_valid = true;
     TRY(launch_nlist(c));
@@ -1195,7 +1198,7 @@ static int rebuild_lists_queued(fm_ctx* c) {
     return 0;
 }
 
-int rebuild_lists_queued_internal(fm_ctx* c) { return rebuild_lists_queued(c); }
+int rebuild_lists_queued_internal(fm_ctx* c) { return rebuild_lists_queued(c, true, false); }
 
 // A synchronisation found FLAG_NEED set by the queued list build: grow the region and list
 // capacities to what it met and build the lists again (the cells are unchanged, and every launch
@@ -1432,6 +1435,74 @@ extern "C" int fm_run_vv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* ma
 }
 
 // ---- fused md_advance_lfv loop (src/md.f90:109-139): forces, record, move, rebuild check ------
+// md_advance_lfv without a host synchronisation per step, for domains whose rebuilds are all
+// known in advance (TPLOT 1, where every step rebuilds for its record step, or the fixed interval)
+// and have no bonds, sort_mols or velocity criterion.  Steps are queued in windows of up to
+// FM_RING_SLOTS steps: force (conditional), the record sums into a ring slot, the move
+// (conditional) and the rebuild, queued without host waits.  A list build beyond the capacities
+// raises FLAG_NEED and FLAG_HALT; the force and move launches after it do nothing (the first
+// halted force launch stores its step in FLAG_HALT_STEP), while the later rebuilds only reorder
+// the unchanged state.  At the window's synchronisation the steps before the halted one are
+// complete; the list is built again with grown capacities and the next window starts at the
+// halted step.  Every completed step has the arithmetic of the step-by-step loop.
+static bool lfv_rebuild_known(const fm_ctx* c, int it) {
+    return ((it + 1) % c->p.tplot) == 0 || (c->p.rebuild_criteria == FM_REBUILD_FIXED && it % c->p.fixed_rebuild == 0);
+}
+
+static int run_lfv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, int n_macros, int* nrb, int* nm) {
+    const int end = iter0 + nsteps, W = FM_RING_SLOTS;
+    static const int reset[] = {FLAG_REBUILD, FLAG_HALT, FLAG_NEED};
+    TRY(launch_reset_flags(c, reset, 3));
+    int iter = iter0;
+    while (iter < end) {
+        const int wend = std::min(end, iter + W);
+        bool ringed[W], rebuilt[W];
+        int want = macros ? n_macros - *nm : 0;
+        for (int k = iter; k < wend; ++k) {
+            c->kt_step = k;
+            TRY(launch_forces(c, k, false, true));
+            ringed[k - iter] = want > 0 && (k % c->p.tplot) == 0;
+            if (ringed[k - iter]) {
+                TRY(launch_macros(c, -1, c->d_ring + 8 * (k - iter)));
+                --want;
+            }
+            c->moved_since_bin = true;
+            TRY(launch_move_lfv(c, false, true));
+            rebuilt[k - iter] = lfv_rebuild_known(c, k);
+            if (rebuilt[k - iter]) {
+                c->rdisp_valid = false;
+                // the loop's last list serves whatever runs next: full radius, accelerations kept
+                const bool last = k + 1 >= end;
+                const bool skinless = !last && lfv_rebuild_known(c, k + 1) && c->p.potential_flag == 0 &&
+                                      c->p.mie_potential == 0;
+                c->list_ra2 = skinless ? c->rc2 * (1.0 + 1e-12) : 0.0;
+                const int rc = rebuild_lists_queued(c, false, last);
+                c->list_ra2 = 0.0;
+                TRY(rc);
+            }
+        }
+        FM_HIP(hipMemcpyAsync(c->h_ring, c->d_ring, 8 * (size_t)(wend - iter) * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        TRY(sync_flags(c));
+        int done = wend;
+        if (c->h_flags[FLAG_HALT]) {
+            const int s = c->h_flags[FLAG_HALT_STEP];
+            if (!c->h_flags[FLAG_NEED] || s <= iter || s >= wend)
+                return fm_set_error(FM_ESTATE, "queued leap-frog halted at step %d outside %d..%d", s, iter + 1, wend - 1);
+            // steps iter .. s-1 are complete; the cells are those of the last rebuild, the state
+            // unchanged since step s-1
+            TRY(redo_lists(c));
+            done = s;
+        }
+        for (int k = iter; k < done; ++k) {
+            if (rebuilt[k - iter]) ++*nrb;
+            TRY(finish_step(c, k, macros, n_macros, nm, ringed[k - iter] ? c->h_ring + 8 * (k - iter) : nullptr));
+        }
+        iter = done;
+    }
+    return 0;
+}
+
 extern "C" int fm_run_lfv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* macros, int32_t n_macros,
                           int32_t* n_rebuilds) {
     CHECK_CTX(c);
@@ -1443,6 +1514,14 @@ extern "C" int fm_run_lfv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* m
     if (c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && !c->has_tags) return fm_set_error(FM_ESTATE, "tag_move needs tags");
     const bool sync_each_force = c->p.potential_flag == 1;
     int nrb = 0, nm = 0;
+    const bool known = c->p.tplot == 1 || c->p.rebuild_criteria == FM_REBUILD_FIXED;
+    if (known && !sync_each_force && c->p.sort_flag == 0 && c->p.rebuild_criteria != FM_REBUILD_VMAX &&
+        c->maxreg > 0 && nsteps > 0) {
+        TRY(run_lfv_queued(c, iter0, nsteps, macros, n_macros, &nrb, &nm));
+        TRY(sync_flags(c));
+        if (n_rebuilds) *n_rebuilds = nrb;
+        return 0;
+    }
     for (int iter = iter0; iter < iter0 + nsteps; ++iter) {
         TRY(launch_forces(c, iter, false));
         const bool rec = (iter % c->p.tplot) == 0;

@@ -1602,7 +1602,10 @@ __device__ inline bool halted(int* f, int step) {
     const bool h = f[FLAG_HALT] != 0 ||
                    (__atomic_load_n(&f[FLAG_REBUILD], __ATOMIC_RELAXED) != 0 &&
                     __atomic_load_n(&f[FLAG_RB_STEP], __ATOMIC_RELAXED) <= step);
-    if (h && blockIdx.x == 0 && threadIdx.x == 0) f[FLAG_HALT] = 1;
+    if (h && blockIdx.x == 0 && threadIdx.x == 0) {
+        f[FLAG_HALT] = 1;
+        atomicMin(&f[FLAG_HALT_STEP], step);
+    }
     return h;
 }
 

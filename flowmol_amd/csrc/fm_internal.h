@@ -42,7 +42,8 @@ enum {
     FLAG_RB_STEP = 9,     // step whose displacement check set FLAG_REBUILD
     FLAG_SORT_MAX = 10,   // largest sort block (sort_mols)
     FLAG_NEED = 11,       // capacity requests of queued list builds: NEED_REGION, NEED_LIST bits
-    FLAG_COUNT = 12
+    FLAG_HALT_STEP = 12,  // the first step whose conditional force launch found a halt (INT_MAX: none)
+    FLAG_COUNT = 13
 };
 enum { NEED_REGION = 1, NEED_LIST = 2 };
 
@@ -201,7 +202,7 @@ int launch_nlist(fm_ctx* c);
 // flips); *fused reports whether that happened.
 int launch_forces(fm_ctx* c, int iter, bool kick = false, bool cond = false, int next = -1, bool* fused = nullptr);
 int launch_pass1_check_disp(fm_ctx* c, bool cond = false, int step = 0);
-int launch_move_lfv(fm_ctx* c, bool check_disp);
+int launch_move_lfv(fm_ctx* c, bool check_disp, bool cond = false);  // cond: nothing while FLAG_HALT is set
 // record-step sums over state buffer st[buf] (buf < 0: the current one) and the per-molecule
 // tallies into out[8] (null: d_macros)
 int launch_macros(fm_ctx* c, int buf = -1, double* out = nullptr);

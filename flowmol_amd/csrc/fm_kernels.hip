@@ -96,7 +96,8 @@ __global__ __launch_bounds__(FM_BLOCK) void k_pass2(St s, int np, double hdt) {
 __global__ __launch_bounds__(FM_BLOCK) void k_move_lfv(St s, const int* tag, double* teth, const int* ext, int np,
                                                         int cap, double dt, double k2, double k4, double k6,
                                                         double* rd, double thresh, int disp, int fresh,
-                                                        int* flags) {
+                                                        int* flags, int cond) {
+    if (cond && flags[FLAG_HALT] != 0) return;  // whole launch: a queued rebuild is pending
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool hit = false;
     if (i < np) {
@@ -268,6 +269,7 @@ struct FlagList {
 __device__ inline void reset_flag(int* flags, int w) {
     flags[w] = 0;
     if (w == FLAG_REBUILD) flags[FLAG_RB_STEP] = 0x7fffffff;
+    if (w == FLAG_HALT) flags[FLAG_HALT_STEP] = 0x7fffffff;
 }
 
 // 1) molecules outside the box, in molecule-number order: workgroup b writes its leavers e
@@ -959,14 +961,14 @@ int launch_sort_mols(fm_ctx* c, int* overflow_out) {
 }
 
 // simulation_move_particles_lfv; check_disp: fused displacement update of rebuild criterion 2
-int launch_move_lfv(fm_ctx* c, bool check_disp) {
+int launch_move_lfv(fm_ctx* c, bool check_disp, bool cond) {
     if (c->np == 0) return 0;
     const bool tm = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE;
     ktimer_begin(c, KT_PASS1);
     k_move_lfv<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(
         state_of(c->x(), c->cap), tm ? c->tag[c->cur] : nullptr, tm ? c->teth[c->cur] : nullptr, c->ext[c->cur],
         c->np, c->cap, c->p.delta_t, c->p.teth_k2, c->p.teth_k4, c->p.teth_k6, c->rdisp, 0.5 * c->dmin,
-        check_disp ? 1 : 0, c->rdisp_valid ? 0 : 1, c->d_flags);
+        check_disp ? 1 : 0, c->rdisp_valid ? 0 : 1, c->d_flags, cond ? 1 : 0);
     ktimer_end(c, KT_PASS1);
     LAUNCH_CHECK();
     if (check_disp) c->rdisp_valid = true;

@@ -484,6 +484,42 @@ def test_queued_rebuild_grows_capacities(monkeypatch):
     g.close()
 
 
+def test_queued_leapfrog_grows_capacities(monkeypatch):
+    """The leap-frog loop at TPLOT 1 queues a rebuild every step without waiting on the device.  A
+    converging flow packs the molecules until a staged region exceeds the region capacity; the
+    launches queued after that build do nothing, the loop grows the capacity and goes on from the
+    halted step, and the trajectory, the rebuild count and every record step's sums stay those of
+    the step-by-step sequence."""
+    from flowmol_amd._abi import FM_INTEGRATION_LFV
+
+    p, r, v = config.config_fcc((6, 6, 6), 0.8, 2.5, seed=43, tplot=1)
+    p.integration_algorithm = FM_INTEGRATION_LFV
+    r = r + np.random.default_rng(43).uniform(-1e-3, 1e-3, r.shape)
+    v = v - 0.3 * r
+    say("case: queued leap-frog rebuilds under a converging flow, 864 LJ, 60 steps")
+    g = MDContext(p, r, v)
+    g.setup(0)
+    monkeypatch.setenv("FLOWMOL_TIGHT_REGIONS", "1")
+    f = MDContext(p, r, v)
+    monkeypatch.delenv("FLOWMOL_TIGHT_REGIONS")
+    f.setup(0)
+    _, reg0 = f.capacity()
+    mg, rg = [], []
+    for it in range(0, 60):
+        rb, m = g.md_advance_lfv(it, macros=True)
+        mg.append(m)
+        rg.append(rb)
+    mf, nrb = f.run_lfv(0, 60, n_macros=60)
+    _, reg1 = f.capacity()
+    assert reg1 > reg0, (reg0, reg1)
+    assert nrb == sum(bool(x) for x in rg) == 60
+    for x, y in zip(f.state(), g.state()):
+        assert np.array_equal(x, y)
+    assert mf == mg
+    f.close()
+    g.close()
+
+
 def test_polymer_random_walk_start():
     """The C4 initial state (seeded random-walk chains) on a small box: neighbour set, bond lists,
     forces and a 20-step drop-in trajectory against the oracle."""
