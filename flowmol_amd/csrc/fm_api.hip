@@ -1239,8 +1239,8 @@ static int finish_step(fm_ctx* c, int iter, fm_macros* macros, int n_macros, int
     return 0;
 }
 
-// md_advance_vv without a host synchronisation per step (rebuild criteria 1 and 2, no bonds or
-// tethers).  Steps are queued in windows that end before the next step whose rebuild the host
+// md_advance_vv without a host synchronisation per step (rebuild criteria 1 and 2; bonds and
+// tethers too, whose force launches are then not fused with the next pass 1).  Steps are queued in windows that end before the next step whose rebuild the host
 // knows in advance (forced at mod(iter+1,tplot)==0, or the fixed interval), or after
 // FM_RING_SLOTS steps; the host synchronises once per window.  The sums of a record step are
 // reduced on the device right after its force launch, from the buffer that holds the step's
@@ -1366,15 +1366,14 @@ extern "C" int fm_run_vv(fm_ctx* c, int32_t iter0, int32_t nsteps, fm_macros* ma
     if (c->p.mie_potential == 1 && !c->has_moltype) return fm_set_error(FM_ESTATE, "Mie potential needs moltype");
     if (c->p.potential_flag == 1 && !c->has_mono) return fm_set_error(FM_ESTATE, "potential_flag 1 needs monomer data");
     if (c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && !c->has_tags) return fm_set_error(FM_ESTATE, "tag_move needs tags");
-    const bool sync_each_force = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE || c->p.potential_flag == 1;
     int nrb = 0, nm = 0;
-    if (!sync_each_force &&
-        (c->p.rebuild_criteria == FM_REBUILD_DISPLACEMENT || c->p.rebuild_criteria == FM_REBUILD_FIXED)) {
+    if (c->p.rebuild_criteria == FM_REBUILD_DISPLACEMENT || c->p.rebuild_criteria == FM_REBUILD_FIXED) {
         TRY(run_vv_queued(c, iter0, nsteps, macros, n_macros, &nrb, &nm));
         TRY(sync_flags(c));
         if (n_rebuilds) *n_rebuilds = nrb;
         return 0;
     }
+    const bool sync_each_force = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE || c->p.potential_flag == 1;
     for (int iter = iter0; iter < iter0 + nsteps; ++iter) {
         const int forced = ((iter + 1) % c->p.tplot) == 0;
         int rb = forced;
