@@ -696,20 +696,34 @@ k_nlist_quant(NlqArgs a) {
             // go nearest first: by the distance g^2 from the molecule to the run's y and z faces
             // (its top 28 bits), then by run number.  That order does not depend on the radius,
             // so a list of a smaller radius is the larger one's with the farther entries left out.
-            uint64_t w[9];
+            // (the order keys only, g2:28 | k:4; each run's window is searched when its turn comes)
+            uint32_t w[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
+                const int dz = k / 3 - 1, dy = k % 3 - 1;
+                const double gy = dy < 0 ? gyl : (dy > 0 ? gyh : 0.0);
+                const double gz = dz < 0 ? gzl : (dz > 0 ? gzh : 0.0);
+                w[k] = ((__float_as_uint((float)(gy * gy + gz * gz)) >> 4) << 4) | (uint32_t)k;
+            }
+#pragma unroll
+            for (int sr = 0; sr < 9; ++sr)  // odd-even transposition sort, ascending
+#pragma unroll
+                for (int k = sr & 1; k + 1 < 9; k += 2) {
+                    const uint32_t x = w[k], y = w[k + 1];
+                    w[k] = x < y ? x : y;
+                    w[k + 1] = x < y ? y : x;
+                }
+            uint64_t word = 0;
+            for (int kk = 0; kk < 9; ++kk) {
+                const int k = (int)(w[kk] & 15u);
                 const int dz = k / 3 - 1, dy = k % 3 - 1;
                 const int c0 = (rx - 1) + R0 * ((ry + dy) + R1 * (rz + dz));
                 int jb = r.off[c0], je = r.off[c0 + 3];
                 const double gy = dy < 0 ? gyl : (dy > 0 ? gyh : 0.0);
                 const double gz = dz < 0 ? gzl : (dz > 0 ? gzh : 0.0);
-                const double g2 = gy * gy + gz * gz;
-                const double w2 = a.ra2 - g2;
-                const uint64_t order = ((uint64_t)(__float_as_uint((float)g2) >> 4) << 36) | ((uint64_t)k << 32);
-                if (!(w2 > 0.0)) {
-                    je = jb;
-                } else {
+                const double w2 = a.ra2 - (gy * gy + gz * gz);
+                if (!(w2 > 0.0)) continue;
+                {
                     const double xw = (dy == 0 && dz == 0) ? a.xwin : sqrt(w2) * (1.0 + 1e-9) + 1e-12;
                     const int xl = (int)floor((xi - xw - ox) * a.inv_h) - 1;
                     const int xh = (int)ceil((xi + xw - ox) * a.inv_h) + 1;
@@ -728,20 +742,6 @@ k_nlist_quant(NlqArgs a) {
                     }
                     je = lo;
                 }
-                w[k] = order | ((uint64_t)jb << 16) | (uint64_t)je;
-            }
-#pragma unroll
-            for (int sr = 0; sr < 9; ++sr)  // odd-even transposition sort, ascending
-#pragma unroll
-                for (int k = sr & 1; k + 1 < 9; k += 2) {
-                    const uint64_t x = w[k], y = w[k + 1];
-                    w[k] = x < y ? x : y;
-                    w[k + 1] = x < y ? y : x;
-                }
-            uint64_t word = 0;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                const int jb = (int)((w[k] >> 16) & 0xffffu), je = (int)(w[k] & 0xffffu);
                 for (int j0 = jb; j0 < je; j0 += 32) {
                     const int nn = min(32, je - j0);
                     // each test shifts its two bits in from the bottom (the first candidate ends
