@@ -865,6 +865,7 @@ struct FbArgs {
     int* halt;             // flag words of a conditional launch (null: unconditional)
     int step;              // step whose forces this launch computes (the halt rule compares it)
     int active;            // packed launches: 0 = this domain has no step in this launch
+    float band32;          // mixed, FP32 staging: |r^2 - rc^2| below which FP64 decides the cutoff
     double* cv;            // faces 1..3 of the control-volume tally, or null
     int cvnbo[3], cvnhb[3];
     double cvbs[3], cvhd[3];
@@ -1589,6 +1590,244 @@ __device__ __forceinline__ void forces_body_mop(const FbArgs& a, int brick, char
         if (spxy[t] != 0.0) atomicAdd(&a.pxy[t], spxy[t]);
 }
 
+// ---- mixed precision staged in FP32 --------------------------------------------------------------
+// The mixed mode's plain steps (no tallies, bonds or tethers) stage each molecule as a float4:
+// x y z relative to the region's lower corner in FP32, and the molecule's slot in w.  That is 16
+// B instead of 24 B (one LDS read per entry instead of two), so three workgroups share a CU.  The
+// pair law and the per-molecule sum are FP32 as in the FP64-staged mixed loop.  Whether a pair
+// counts is decided on the FP32 r^2, except within band32 of rc^2, where the FP64 r^2 of the two
+// molecules' positions in HBM decides, in the FP64 path's operation order: band32 bounds the
+// difference between the FP32 and FP64 r^2 (the rounding of the staged offsets, of their
+// differences and of the sum), so every pair counts exactly when the FP64 path counts it.
+__device__ inline void carve32(Region& r, float4*& p4, char* smem, const BG& g, int maxreg) {
+    p4 = (float4*)smem;
+    int* ip = (int*)(p4 + maxreg);
+    r.pos = nullptr;
+    r.lslot = nullptr;
+    r.ltype = nullptr;
+    r.rq = (uint16_t*)ip;
+    ip += (maxreg + 1) / 2;
+    r.off = ip;
+    ip += g.nreg + 1;
+    r.gc = ip;
+    ip += g.nreg;
+    r.sh = ip;
+    ip += g.nreg;
+    r.cst = ip;
+    ip += g.nreg;
+    const int nbmax = g.B[0] * g.B[1] * g.B[2];
+    r.ic_off = ip;
+    ip += nbmax + 1;
+    r.ic_rc = ip;
+}
+
+// staged offsets and slots of the region of `brick`; o: the region's lower corner
+__device__ bool stage_region32(Region& r, float4* p4, const BG& g, int brick, const int* start, const St& s,
+                               int capm, int* flags, const double o[3]) {
+    if (!stage_tables(r, g, brick, start, capm, flags)) return false;
+    constexpr int SU = 6;
+    for (int q0 = threadIdx.x; q0 < r.total; q0 += SU * blockDim.x) {
+        int sl[SU], sc[SU];
+        double px[SU], py[SU], pz[SU];
+#pragma unroll
+        for (int k = 0; k < SU; ++k) {
+            const int qc = min(q0 + k * (int)blockDim.x, r.total - 1);
+            const int rc = r.rq[qc];
+            sl[k] = r.cst[rc] + (qc - r.off[rc]);
+            sc[k] = r.sh[rc];
+            px[k] = s.x[sl[k]];
+            py[k] = s.y[sl[k]];
+            pz[k] = s.z[sl[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < SU; ++k) {
+            const int q = min(q0 + k * (int)blockDim.x, r.total - 1);
+            int sx, sy, sz;
+            decode_shift(sc[k], sx, sy, sz);
+            p4[q] = make_float4((float)(img(px[k], sx, g.L[0]) - o[0]), (float)(img(py[k], sy, g.L[1]) - o[1]),
+                                (float)(img(pz[k], sz, g.L[2]) - o[2]), __int_as_float(sl[k]));
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+struct P4F {
+    float x[4], y[4], z[4];
+    int s[4];
+};
+__device__ __forceinline__ void lj_stage32(P4F& p, const float4* p4, uint64_t w) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 v = p4[(int)((w >> (16 * q)) & 0xffffu)];
+        p.x[q] = v.x;
+        p.y[q] = v.y;
+        p.z[q] = v.z;
+        p.s[q] = __float_as_int(v.w);
+    }
+}
+
+// the FP64 path's cutoff test for molecule i (interior: its stored position is its staged one)
+// and region entry j (slot sj), from their positions in HBM
+__device__ __forceinline__ bool in_rc_fp64(const FbArgs& a, const Region& r, int i, int j, int sj) {
+    int sx, sy, sz;
+    decode_shift(r.sh[r.rq[j]], sx, sy, sz);
+    const double dx = a.s.x[i] - img(a.s.x[sj], sx, a.g.L[0]), dy = a.s.y[i] - img(a.s.y[sj], sy, a.g.L[1]),
+                 dz = a.s.z[i] - img(a.s.z[sj], sz, a.g.L[2]);
+    return ((dx * dx + dy * dy) + dz * dz) < a.rc2;
+}
+
+// One entry's FP32 pair term (the same arithmetic in the list loop and for a band entry).
+__device__ __forceinline__ void pair32(float dx, float dy, float dz, bool in, float& fx, float& fy, float& fz) {
+#pragma clang fp contract(fast)
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    float x = __builtin_amdgcn_rcpf(r2);
+    x = in ? x : 0.0f;
+    const float x2 = x * x;
+    const float acc = (x2 * x2) * (2.0f * (x2 * x) - 1.0f);
+    fx = fx + acc * dx;
+    fy = fy + acc * dy;
+    fz = fz + acc * dz;
+}
+__device__ __forceinline__ float r2_32(float dx, float dy, float dz) {
+#pragma clang fp contract(fast)
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// A band entry is left out of the loop's sum and its list position (12 bits) kept: up to four
+// per molecule in band (count in nb); the loop itself has no branch.
+__device__ __forceinline__ void lj_word32(const P4F& p, int kw, int left, float xi, float yi, float zi, float rc2f,
+                                          float band, float& fx, float& fy, float& fz, uint32_t& bl0,
+                                          uint32_t& bl1, int& nb) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float dx = xi - p.x[q], dy = yi - p.y[q], dz = zi - p.z[q];
+        const float r2 = r2_32(dx, dy, dz);
+        const bool inb = fabsf(r2 - rc2f) < band && q < left;
+        const bool in = r2 < rc2f && q < left && !inb;
+        pair32(dx, dy, dz, in, fx, fy, fz);
+        const uint32_t e = (uint32_t)(kw + q);
+        bl1 = inb ? (bl1 << 12) | (bl0 >> 20) : bl1;
+        bl0 = inb ? (bl0 << 12) | e : bl0;
+        nb += inb ? 1 : 0;
+    }
+}
+
+// the list loop of lj_plain (two words in flight) over the FP32 staging, then the band entries
+__device__ __forceinline__ void lj_plain32(const FbArgs& a, const Region& r, const float4* p4, int i,
+                                           const uint64_t* row, int n, size_t cap, float xi, float yi, float zi,
+                                           double& fx, double& fy, double& fz) {
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    const float rc2f = (float)a.rc2, band = a.band32;
+    uint32_t bl0 = 0, bl1 = 0;
+    int nb = 0;
+    P4F pa, pb;
+    const int lastw = n > 0 ? (n - 1) >> 2 : 0;
+    uint64_t w0 = row[0];
+    uint64_t wa = n > 0 ? w0 : 0;
+    lj_stage32(pa, p4, wa);
+    uint64_t w1 = row[(size_t)min(1, lastw) * cap];
+    uint64_t w2 = row[(size_t)min(2, lastw) * cap];
+    for (int kw = 0; kw < n; kw += 8) {
+        const int k = kw >> 2;
+        const uint64_t wb = kw + 4 < n ? w1 : 0;
+        lj_stage32(pb, p4, wb);
+        const uint64_t wk2 = w2;
+        w1 = row[(size_t)min(k + 3, lastw) * cap];
+        w2 = row[(size_t)min(k + 4, lastw) * cap];
+        __builtin_amdgcn_sched_barrier(0);
+        lj_word32(pa, kw, n - kw, xi, yi, zi, rc2f, band, sx, sy, sz, bl0, bl1, nb);
+        wa = kw + 8 < n ? wk2 : 0;
+        lj_stage32(pa, p4, wa);
+        __builtin_amdgcn_sched_barrier(0);
+        if (__builtin_amdgcn_ballot_w64(kw + 4 < n) != 0ull)
+            lj_word32(pb, kw + 4, n - kw - 4, xi, yi, zi, rc2f, band, sx, sy, sz, bl0, bl1, nb);
+    }
+    if (nb > 0) {
+        // the band entries, each decided on the FP64 r^2 (all of the list again in the improbable
+        // case of more than four: every entry in band re-decided, in list order)
+        const bool all = nb > 4 || n > 4096;  // (positions are kept in 12 bits)
+        const int cnt = all ? n : nb;
+        for (int b = 0; b < cnt; ++b) {
+            int e;
+            if (all) {
+                e = b;
+            } else {
+                const uint64_t packed = ((uint64_t)bl1 << 32) | bl0;
+                e = (int)((packed >> (12 * b)) & 0xfffu);
+            }
+            const int j = (int)((row[(size_t)(e >> 2) * cap] >> (16 * (e & 3))) & 0xffffu);
+            const float4 v = p4[j];
+            const float dx = xi - v.x, dy = yi - v.y, dz = zi - v.z;
+            if (!(fabsf(r2_32(dx, dy, dz) - rc2f) < band)) continue;
+            pair32(dx, dy, dz, in_rc_fp64(a, r, i, j, __float_as_int(v.w)), sx, sy, sz);
+        }
+    }
+    fx = 24.0 * (double)sx;
+    fy = 24.0 * (double)sy;
+    fz = 24.0 * (double)sz;
+}
+
+// P1: VV pass 2 of this step, pass 1 of the next one and its displacement update follow the force,
+// written to the other state buffer (the arithmetic of forces_body's P1 branch)
+template <bool P1>
+__device__ __forceinline__ void forces_body32(const FbArgs& a, int brick, char* smem) {
+    Region r;
+    float4* p4;
+    carve32(r, p4, smem, a.g, a.maxreg);
+    const int bxi = brick % a.g.nb[0], byi = (brick / a.g.nb[0]) % a.g.nb[1], bzi = brick / (a.g.nb[0] * a.g.nb[1]);
+    const double o[3] = {-0.5 * a.g.L[0] + (bxi * a.g.B[0] - 1) * (a.g.L[0] / a.g.nc[0]),
+                         -0.5 * a.g.L[1] + (byi * a.g.B[1] - 1) * (a.g.L[1] / a.g.nc[1]),
+                         -0.5 * a.g.L[2] + (bzi * a.g.B[2] - 1) * (a.g.L[2] / a.g.nc[2])};
+    if (!stage_region32(r, p4, a.g, brick, a.start, a.s, a.maxreg, a.flags, o)) return;
+    const size_t cap = a.cap;
+    for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
+        int rci, li, i;
+        imol(r, a.start, q, rci, li, i);
+        const float4 me = p4[li];
+        double fx, fy, fz;
+        lj_plain32(a, r, p4, i, a.nbl + i, a.nbn[i], cap, me.x, me.y, me.z, fx, fy, fz);
+        if constexpr (P1) {
+            const double ux = a.s.vx[i] + a.hdt * fx, uy = a.s.vy[i] + a.hdt * fy, uz = a.s.vz[i] + a.hdt * fz;
+            a.s2.x[i] = a.s.x[i] + a.dt * ux + a.hdt2 * fx;
+            a.s2.y[i] = a.s.y[i] + a.dt * uy + a.hdt2 * fy;
+            a.s2.z[i] = a.s.z[i] + a.dt * uz + a.hdt2 * fz;
+            const double wx = ux + a.hdt * fx, wy = uy + a.hdt * fy, wz = uz + a.hdt * fz;
+            a.s2.vx[i] = wx;
+            a.s2.vy[i] = wy;
+            a.s2.vz[i] = wz;
+            a.s2.ax[i] = fx;
+            a.s2.ay[i] = fy;
+            a.s2.az[i] = fz;
+            const size_t cp = a.cap;
+            double d0 = a.fresh ? 0.0 : a.rd[i], d1 = a.fresh ? 0.0 : a.rd[i + cp],
+                   d2 = a.fresh ? 0.0 : a.rd[i + 2 * cp];
+            d0 = d0 + wx * a.dt;
+            d1 = d1 + wy * a.dt;
+            d2 = d2 + wz * a.dt;
+            a.rd[i] = d0;
+            a.rd[i + cp] = d1;
+            a.rd[i + 2 * cp] = d2;
+            const bool hit = fabs(d0) > a.thresh || fabs(d1) > a.thresh || fabs(d2) > a.thresh;
+            const unsigned long long m = __ballot(hit);
+            if (m != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1 &&
+                __atomic_load_n(&a.flags[FLAG_REBUILD], __ATOMIC_RELAXED) == 0) {
+                atomicOr(&a.flags[FLAG_REBUILD], 1);
+                atomicExch(&a.flags[FLAG_RB_STEP], a.p1step);
+            }
+            continue;
+        }
+        a.s.ax[i] = fx;
+        a.s.ay[i] = fy;
+        a.s.az[i] = fz;
+        if (a.kick) {
+            a.s.vx[i] = a.s.vx[i] + a.hdt * fx;
+            a.s.vy[i] = a.s.vy[i] + a.hdt * fy;
+            a.s.vz[i] = a.s.vz[i] + a.hdt * fz;
+        }
+    }
+}
+
 // A conditional launch does nothing when a rebuild is pending for its own step or an earlier one,
 // and passes the halt on to the launches queued after it.  A force launch that carries the next
 // step's pass 1 may itself raise FLAG_REBUILD for that next step while its later workgroups are
@@ -1656,6 +1895,16 @@ __device__ inline bool skip_domain(const FbArgs& a) {
     const int* f = a.halt;
     return f && (f[FLAG_HALT] != 0 || (__atomic_load_n(&f[FLAG_REBUILD], __ATOMIC_RELAXED) != 0 &&
                                        __atomic_load_n(&f[FLAG_RB_STEP], __ATOMIC_RELAXED) <= a.step));
+}
+
+// the mixed mode's plain steps over the FP32 staging: registers held to what lets three workgroups
+// share a CU (the LDS of three regions fits)
+template <bool P1>
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(3 * FB_BLOCK / 256, 3 * FB_BLOCK / 256))) void
+k_forces_brick_plain32(FbArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    if (halted(a.halt, a.step)) return;
+    forces_body32<P1>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // packed launch over many domains: block b works on brick table[b].y of domain table[b].x
@@ -1798,6 +2047,7 @@ __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_mixed_multi(const FbA
     if (skip_domain(a)) return;
     if (a.rec) forces_body_mixed<true, EXTRA>(a, t.y, smem);
     else if (a.ev) forces_body_mixed<false, EXTRA, true>(a, t.y, smem);
+    else if constexpr (!EXTRA) forces_body32<false>(a, t.y, smem);
     else forces_body_mixed<false, EXTRA>(a, t.y, smem);
 }
 
@@ -1982,6 +2232,28 @@ int launch_nlist(fm_ctx* c) {
 
 size_t fargs_bytes() { return sizeof(FbArgs); }
 
+// LDS of the FP32-staged mixed launch
+static size_t lds32(const fm_ctx* c) {
+    const BG g = make_bg(c);
+    const int nbmax = g.B[0] * g.B[1] * g.B[2];
+    const size_t b = (size_t)c->maxreg * sizeof(float4) +
+                     (size_t)((c->maxreg + 1) / 2 + 4 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int);
+    return (b + 15) & ~(size_t)15;
+}
+
+// The band of the FP32-staged cutoff test: with u = 2^-24, a staged offset is within u*ext of the
+// exact one (ext bounds the offsets: the region's extent plus a cell of drift), a difference
+// within e = 2 u ext + u rc, and near the cutoff (|d| <= rc per component) the FP32 r^2 within
+// e (2 sqrt(3) rc + 3 e) + 6 u rc^2 of the FP64 one; the band is four times that.
+static float band32_of(const fm_ctx* c) {
+    const BG g = make_bg(c);
+    double ext = 0.0;
+    for (int d = 0; d < 3; ++d) ext = fmax(ext, (g.B[d] + 3) * c->cs[d]);
+    const double u = 1.0 / 16777216.0, rc = sqrt(c->rc2);
+    const double e = 2.0 * u * ext + u * rc;
+    return (float)(4.0 * (e * (2.0 * sqrt(3.0) * rc + 3.0 * e) + 6.0 * u * c->rc2));
+}
+
 // a domain's force arguments in a queued packed launch: halted by its own flag words, or idle
 void fargs_set_queued(void* fargs, int* flags, bool active) {
     FbArgs* a = (FbArgs*)fargs;
@@ -2045,6 +2317,7 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     a.order = c->brick_order;
     a.halt = nullptr;
     a.active = 1;
+    a.band32 = band32_of(c);
     a.cv = (a.rec && c->cvface) ? c->cvface : nullptr;
     for (int d = 0; d < 3; ++d) {
         a.cvnbo[d] = c->cv_nbo[d];
@@ -2087,7 +2360,7 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         a.p1step = next;
         ktimer_begin(c, KT_FORCES_P1);
         if (simple_rec) k_forces_brick_plain<true, false, true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
-        else if (mixed) k_forces_brick_plain<true, true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
+        else if (mixed) k_forces_brick_plain32<true><<<brick_count(c), FB_BLOCK, lds32(c), c->stream>>>(a);
         else k_forces_brick_plain<true, false><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
         ktimer_end(c, KT_FORCES_P1);
         LAUNCH_CHECK();
@@ -2120,7 +2393,7 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         if (extra) { if (a.rec) FM(true, true); else FM(false, true); }
         else if (a.rec) FM(true, false);
         else if (a.ev) FM(false, false);
-        else k_forces_brick_plain<false, true><<<grid, blk, lds, c->stream>>>(a);
+        else k_forces_brick_plain32<false><<<grid, blk, lds32(c), c->stream>>>(a);
     } else if (mie) {
         if (extra) { if (a.rec) FB(true, true, true); else FB(true, false, true); }
         else { if (a.rec) FB(true, true, false); else FB(true, false, false); }

@@ -349,15 +349,45 @@ def test_mixed_precision_error_vs_fp64():
     g32.setup(0)
     a64, a32 = g64.state()[2], g32.state()[2]
     rms = np.sqrt(((a32 - a64) ** 2).sum() / (a64 ** 2).sum())
+    say(f"  record-step force RMS relative error {rms:.2e}")
     assert rms < 1e-5
     pe64, _ = g64.molecular()
     pe32, _ = g32.molecular()
     assert abs(pe32.sum() - pe64.sum()) < 1e-5 * abs(pe64.sum())
-    # a step without tallies takes the branch-free FP32 loop
+    # a step without tallies takes the branch-free FP32 loop over the FP32 staging
     g64.simulation_compute_forces(3)
     g32.simulation_compute_forces(3)
     a64, a32 = g64.state()[2], g32.state()[2]
-    assert np.sqrt(((a32 - a64) ** 2).sum() / (a64 ** 2).sum()) < 1e-5
+    rms = np.sqrt(((a32 - a64) ** 2).sum() / (a64 ** 2).sum())
+    say(f"  plain-step force RMS relative error {rms:.2e}")
+    assert rms < 1e-5
+    g64.close()
+    g32.close()
+
+
+def test_mixed_precision_cutoff_band():
+    """The mixed mode's FP32-staged plain step decides a pair near rc on the FP64 r^2: on an FCC
+    lattice whose second shell lies exactly at rc (lattice constant rc; 6 such pairs per molecule)
+    jittered by 1e-7, the FP64 path counts some of those pairs and not others; a pair that the
+    mixed launch classified otherwise would move its molecule's force by |F(rc)| = 0.039, far
+    above the FP32 rounding of the rest (forces below 1e-4 here)."""
+    rc = 2.5
+    L = 6
+    p = config.make_params((L * rc, L * rc, L * rc), 4.0 / rc ** 3, rc, tplot=10)
+    pm = config.make_params((L * rc, L * rc, L * rc), 4.0 / rc ** 3, rc, tplot=10, precision=FM_PREC_MIXED)
+    r = config.fcc_lattice((L, L, L), 4.0 / rc ** 3)
+    r = r + np.random.default_rng(77).uniform(-1e-7, 1e-7, r.shape)
+    v = np.zeros_like(r)
+    say("case: mixed precision, pairs at rc within 1e-7, 864 molecules")
+    g64 = MDContext(p, r, v)
+    g64.setup(0)
+    g32 = MDContext(pm, r, v)
+    g32.setup(0)
+    g64.simulation_compute_forces(3)   # plain steps: the FP32-staged launch in the mixed context
+    g32.simulation_compute_forces(3)
+    a64, a32 = g64.state()[2], g32.state()[2]
+    assert np.abs(a64).max() > 1e-3     # the counted and uncounted shell pairs leave net forces
+    assert np.abs(a32 - a64).max() < 1e-4
     g64.close()
     g32.close()
 
