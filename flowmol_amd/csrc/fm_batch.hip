@@ -32,8 +32,9 @@
 struct Member {
     fm_ctx* c;
     int iter;                 // iteration number of the member's next step
-    hipStream_t own_stream;   // restored when the member leaves
+    hipStream_t own_stream;   // restored when the member leaves; the member's rebuilds run on it
     int fired;                // step whose displacement check fired (its pass 1 has run), or -1
+    hipEvent_t done = nullptr;  // the member's rebuild, for the batch stream to wait on
 };
 
 struct WindowGraph {          // the launches of a window of W steps after its rebuilds
@@ -63,6 +64,7 @@ struct fm_batch {
     WindowGraph wg[BATCH_WINDOW + 1];
     int64_t rebuilds = 0;
     int64_t windows = 0;
+    hipEvent_t p1done = nullptr;   // pass 1 of a window's first step, for the members' rebuilds
 };
 
 
@@ -191,7 +193,9 @@ extern "C" int fm_batch_create(fm_batch** out) {
     if (!out) return fm_set_error(FM_EINVAL, "null argument");
     fm_batch* b = new fm_batch();
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->p1done, hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (b->stream) hipStreamDestroy(b->stream);
         delete b;
         return fm_set_error(FM_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
@@ -202,8 +206,12 @@ extern "C" int fm_batch_create(fm_batch** out) {
 extern "C" void fm_batch_destroy(fm_batch* b) {
     if (!b) return;
     hipStreamSynchronize(b->stream);
-    for (auto& mm : b->m) mm.c->stream = mm.own_stream;
+    for (auto& mm : b->m) {
+        mm.c->stream = mm.own_stream;
+        if (mm.done) hipEventDestroy(mm.done);
+    }
     free_buffers(b);
+    if (b->p1done) hipEventDestroy(b->p1done);
     hipStreamDestroy(b->stream);
     delete b;
 }
@@ -237,7 +245,9 @@ extern "C" int fm_batch_add(fm_batch* b, fm_ctx* c, int32_t iter0) {
     if (int rc = launch_reset_flags(c, reset, 3)) return rc;
     FM_HIP(hipStreamSynchronize(c->stream));
     b->kind = kind;
-    b->m.push_back({c, iter0, c->stream, -1});
+    hipEvent_t ev = nullptr;
+    FM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    b->m.push_back({c, iter0, c->stream, -1, ev});
     c->stream = b->stream;
     b->dirty = true;
     return 0;
@@ -249,6 +259,7 @@ extern "C" int fm_batch_remove(fm_batch* b, fm_ctx* c) {
         if (b->m[k].c == c) {
             FM_HIP(hipStreamSynchronize(b->stream));
             c->stream = b->m[k].own_stream;
+            if (b->m[k].done) hipEventDestroy(b->m[k].done);
             b->m.erase(b->m.begin() + k);
             b->dirty = true;
             if (b->m.empty()) b->kind = -1;
@@ -347,12 +358,24 @@ extern "C" int fm_batch_run(fm_batch* b, int32_t nsteps, int32_t* n_rebuilds) {
                 // without host waits (they move members' state to their other buffer)
                 FM_HIP(hipMemcpyAsync(b->d_p1, b->h_p1, n_ * sizeof(P1Desc), hipMemcpyHostToDevice, b->stream));
                 TRY(launch_pass1_multi(b->stream, b->d_p1, b->d_t1, b->nblk1));
+                // the rebuilds run side by side, each on its member's own stream after the pass 1
+                // launch; the batch stream waits for all of them before the forces
+                bool any = false;
+                for (size_t k = 0; k < n_; ++k) any = any || (len[k] > 0 && rb0[k]);
+                if (any) FM_HIP(hipEventRecord(b->p1done, b->stream));
                 for (size_t k = 0; k < n_; ++k) {
                     if (len[k] == 0 || !rb0[k]) continue;
-                    fm_ctx* c = b->m[k].c;
+                    Member& mm = b->m[k];
+                    fm_ctx* c = mm.c;
                     c->rdisp_valid = false;
                     const int old_bricks = brick_count(c);
-                    TRY(rebuild_lists_queued_internal(c));
+                    FM_HIP(hipStreamWaitEvent(mm.own_stream, b->p1done, 0));
+                    c->stream = mm.own_stream;
+                    const int rc = rebuild_lists_queued_internal(c);
+                    c->stream = b->stream;
+                    TRY(rc);
+                    FM_HIP(hipEventRecord(mm.done, mm.own_stream));
+                    FM_HIP(hipStreamWaitEvent(b->stream, mm.done, 0));
                     if (brick_count(c) != old_bricks) b->dirty = true;
                     ++nrb;
                 }
