@@ -7,7 +7,7 @@ name and argument meaning (paths relative to the reference's root):
   assign_to_neighbourlist      src/linklist.f90:2710-2737 (FORCE_LIST 3)
   linklist_deallocateall       src/linklist.f90:2872-2963
   sendmols                     src/messenger.MPI.f90:1383-1421 (one periodic rank)
-  sort_mols                    src/linklist.f90:2744-2865 (SORT_FLAG 0)
+  sort_mols                    src/linklist.f90:2744-2865 (SORT_FLAG 0, 1, 2)
   messenger_updateborders      src/messenger.MPI.f90:594-628
   simulation_checkrebuild      src/simulation_checkrebuild.f90:17-174
   simulation_compute_forces    src/simulation_compute_forces.f90:182-275

@@ -207,7 +207,8 @@ int  fm_checkrebuild(fm_ctx* ctx, int32_t iter, int32_t* rebuild);
 int  fm_linklist_deallocateall(fm_ctx* ctx);
 /* sendmols on one periodic rank, src/messenger.MPI.f90:1383-1421,1460-1904: wrap and renumber */
 int  fm_sendmols(fm_ctx* ctx);
-/* sort_mols with SORT_FLAG 0 (no reordering visible to the host), src/linklist.f90:2744-2865 */
+/* sort_mols (src/linklist.f90:2744-2865): SORT_FLAG 1 or 2 renumbers the molecules by the block
+   curve every SORT_FREQ rebuilds (fm_params.sort_flag, sort_freq, sortblocksize); 0 does nothing */
 int  fm_sort_mols(fm_ctx* ctx);
 /* assign_to_cell, src/linklist.f90:2645-2676 */
 int  fm_assign_to_cell(fm_ctx* ctx);
