@@ -2,6 +2,8 @@
 // per-routine entry points that replace the reference's force / neighbour / integration bodies.
 #include "fm_internal.h"
 
+#include <climits>
+
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -1485,13 +1487,15 @@ static int run_lfv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, i
         TRY(sync_flags(c));
         int done = wend;
         if (c->h_flags[FLAG_HALT]) {
+            // a list build met a region or list beyond the capacities: the first force launch
+            // after it halted (none did when it was the window's last rebuild)
             const int s = c->h_flags[FLAG_HALT_STEP];
-            if (!c->h_flags[FLAG_NEED] || s <= iter || s >= wend)
+            if (!c->h_flags[FLAG_NEED] || (s != INT_MAX && (s <= iter || s >= wend)))
                 return fm_set_error(FM_ESTATE, "queued leap-frog halted at step %d outside %d..%d", s, iter + 1, wend - 1);
             // steps iter .. s-1 are complete; the cells are those of the last rebuild, the state
             // unchanged since step s-1
             TRY(redo_lists(c));
-            done = s;
+            if (s != INT_MAX) done = s;
         }
         for (int k = iter; k < done; ++k) {
             if (rebuilt[k - iter]) ++*nrb;
