@@ -1677,17 +1677,30 @@ __device__ __forceinline__ bool in_rc_fp64(const FbArgs& a, const Region& r, int
     return ((dx * dx + dy * dy) + dz * dz) < a.rc2;
 }
 
-// One entry's FP32 pair term (the same arithmetic in the list loop and for a band entry).
-__device__ __forceinline__ void pair32(float dx, float dy, float dz, bool in, float& fx, float& fy, float& fz) {
+// One entry's FP32 pair term (the same arithmetic in the list loop and for a band entry).  REC:
+// also the PE term 4 (x^6 - x^3) - potshift in FP64 from the FP32 x, and the virial term
+// x^4 (2 x^3 - 1) r^2 (scaled by 24 with the force), summed in FP64; an entry that does not
+// count adds exactly zero.
+template <bool REC = false>
+__device__ __forceinline__ void pair32(float dx, float dy, float dz, bool in, float& fx, float& fy, float& fz,
+                                       double potshift = 0.0, double* pe = nullptr, double* vt = nullptr) {
+    float acc, r2, x;
+    {
 #pragma clang fp contract(fast)
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    float x = __builtin_amdgcn_rcpf(r2);
-    x = in ? x : 0.0f;
-    const float x2 = x * x;
-    const float acc = (x2 * x2) * (2.0f * (x2 * x) - 1.0f);
-    fx = fx + acc * dx;
-    fy = fy + acc * dy;
-    fz = fz + acc * dz;
+        r2 = dx * dx + dy * dy + dz * dz;
+        x = __builtin_amdgcn_rcpf(r2);
+        x = in ? x : 0.0f;
+        const float x2 = x * x;
+        acc = (x2 * x2) * (2.0f * (x2 * x) - 1.0f);
+        fx = fx + acc * dx;
+        fy = fy + acc * dy;
+        fz = fz + acc * dz;
+    }
+    if constexpr (REC) {
+        const double xd = (double)x;
+        *pe = *pe + (in ? 4.0 * (p6(xd) - p3(xd)) - potshift : 0.0);
+        *vt = *vt + (double)acc * (double)r2;
+    }
 }
 __device__ __forceinline__ float r2_32(float dx, float dy, float dz) {
 #pragma clang fp contract(fast)
@@ -1696,16 +1709,17 @@ __device__ __forceinline__ float r2_32(float dx, float dy, float dz) {
 
 // A band entry is left out of the loop's sum and its list position (12 bits) kept: up to four
 // per molecule in band (count in nb); the loop itself has no branch.
+template <bool REC>
 __device__ __forceinline__ void lj_word32(const P4F& p, int kw, int left, float xi, float yi, float zi, float rc2f,
                                           float band, float& fx, float& fy, float& fz, uint32_t& bl0,
-                                          uint32_t& bl1, int& nb) {
+                                          uint32_t& bl1, int& nb, double potshift, double* pe, double* vt) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const float dx = xi - p.x[q], dy = yi - p.y[q], dz = zi - p.z[q];
         const float r2 = r2_32(dx, dy, dz);
         const bool inb = fabsf(r2 - rc2f) < band && q < left;
         const bool in = r2 < rc2f && q < left && !inb;
-        pair32(dx, dy, dz, in, fx, fy, fz);
+        pair32<REC>(dx, dy, dz, in, fx, fy, fz, potshift, pe, vt);
         const uint32_t e = (uint32_t)(kw + q);
         bl1 = inb ? (bl1 << 12) | (bl0 >> 20) : bl1;
         bl0 = inb ? (bl0 << 12) | e : bl0;
@@ -1714,10 +1728,14 @@ __device__ __forceinline__ void lj_word32(const P4F& p, int kw, int left, float 
 }
 
 // the list loop of lj_plain (two words in flight) over the FP32 staging, then the band entries
+// (REC: with the PE and virial sums)
+template <bool REC>
 __device__ __forceinline__ void lj_plain32(const FbArgs& a, const Region& r, const float4* p4, int i,
                                            const uint64_t* row, int n, size_t cap, float xi, float yi, float zi,
-                                           double& fx, double& fy, double& fz) {
+                                           double& fx, double& fy, double& fz, double& pe, double& vir) {
     float sx = 0.f, sy = 0.f, sz = 0.f;
+    double vt = 0.0;
+    pe = 0.0;
     const float rc2f = (float)a.rc2, band = a.band32;
     uint32_t bl0 = 0, bl1 = 0;
     int nb = 0;
@@ -1736,12 +1754,13 @@ __device__ __forceinline__ void lj_plain32(const FbArgs& a, const Region& r, con
         w1 = row[(size_t)min(k + 3, lastw) * cap];
         w2 = row[(size_t)min(k + 4, lastw) * cap];
         __builtin_amdgcn_sched_barrier(0);
-        lj_word32(pa, kw, n - kw, xi, yi, zi, rc2f, band, sx, sy, sz, bl0, bl1, nb);
+        lj_word32<REC>(pa, kw, n - kw, xi, yi, zi, rc2f, band, sx, sy, sz, bl0, bl1, nb, a.potshift, &pe, &vt);
         wa = kw + 8 < n ? wk2 : 0;
         lj_stage32(pa, p4, wa);
         __builtin_amdgcn_sched_barrier(0);
         if (__builtin_amdgcn_ballot_w64(kw + 4 < n) != 0ull)
-            lj_word32(pb, kw + 4, n - kw - 4, xi, yi, zi, rc2f, band, sx, sy, sz, bl0, bl1, nb);
+            lj_word32<REC>(pb, kw + 4, n - kw - 4, xi, yi, zi, rc2f, band, sx, sy, sz, bl0, bl1, nb, a.potshift, &pe,
+                           &vt);
     }
     if (nb > 0) {
         // the band entries, each decided on the FP64 r^2 (all of the list again in the improbable
@@ -1760,17 +1779,19 @@ __device__ __forceinline__ void lj_plain32(const FbArgs& a, const Region& r, con
             const float4 v = p4[j];
             const float dx = xi - v.x, dy = yi - v.y, dz = zi - v.z;
             if (!(fabsf(r2_32(dx, dy, dz) - rc2f) < band)) continue;
-            pair32(dx, dy, dz, in_rc_fp64(a, r, i, j, __float_as_int(v.w)), sx, sy, sz);
+            pair32<REC>(dx, dy, dz, in_rc_fp64(a, r, i, j, __float_as_int(v.w)), sx, sy, sz, a.potshift, &pe, &vt);
         }
     }
     fx = 24.0 * (double)sx;
     fy = 24.0 * (double)sy;
     fz = 24.0 * (double)sz;
+    vir = 24.0 * vt;
 }
 
 // P1: VV pass 2 of this step, pass 1 of the next one and its displacement update follow the force,
-// written to the other state buffer (the arithmetic of forces_body's P1 branch)
-template <bool P1>
+// written to the other state buffer (the arithmetic of forces_body's P1 branch).  REC: a record
+// step tallying PE and virial only.
+template <bool P1, bool REC = false>
 __device__ __forceinline__ void forces_body32(const FbArgs& a, int brick, char* smem) {
     Region r;
     float4* p4;
@@ -1785,8 +1806,12 @@ __device__ __forceinline__ void forces_body32(const FbArgs& a, int brick, char* 
         int rci, li, i;
         imol(r, a.start, q, rci, li, i);
         const float4 me = p4[li];
-        double fx, fy, fz;
-        lj_plain32(a, r, p4, i, a.nbl + i, a.nbn[i], cap, me.x, me.y, me.z, fx, fy, fz);
+        double fx, fy, fz, pe, vir;
+        lj_plain32<REC>(a, r, p4, i, a.nbl + i, a.nbn[i], cap, me.x, me.y, me.z, fx, fy, fz, pe, vir);
+        if constexpr (REC) {
+            a.vir[i] = vir;
+            a.pe_lj[i] = pe;
+        }
         if constexpr (P1) {
             const double ux = a.s.vx[i] + a.hdt * fx, uy = a.s.vy[i] + a.hdt * fy, uz = a.s.vz[i] + a.hdt * fz;
             a.s2.x[i] = a.s.x[i] + a.dt * ux + a.hdt2 * fx;
@@ -1814,6 +1839,13 @@ __device__ __forceinline__ void forces_body32(const FbArgs& a, int brick, char* 
                 __atomic_load_n(&a.flags[FLAG_REBUILD], __ATOMIC_RELAXED) == 0) {
                 atomicOr(&a.flags[FLAG_REBUILD], 1);
                 atomicExch(&a.flags[FLAG_RB_STEP], a.p1step);
+            }
+            if constexpr (REC) {
+                // v at the end of this step stays in this buffer for the record-step sums (no
+                // other thread of this launch reads this molecule's velocity)
+                a.s.vx[i] = ux;
+                a.s.vy[i] = uy;
+                a.s.vz[i] = uz;
             }
             continue;
         }
@@ -1899,12 +1931,12 @@ __device__ inline bool skip_domain(const FbArgs& a) {
 
 // the mixed mode's plain steps over the FP32 staging: registers held to what lets three workgroups
 // share a CU (the LDS of three regions fits)
-template <bool P1>
+template <bool P1, bool REC = false>
 __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(3 * FB_BLOCK / 256, 3 * FB_BLOCK / 256))) void
 k_forces_brick_plain32(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt, a.step)) return;
-    forces_body32<P1>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+    forces_body32<P1, REC>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // packed launch over many domains: block b works on brick table[b].y of domain table[b].x
@@ -2045,6 +2077,13 @@ __global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_mixed_multi(const FbA
     const int2 t = table[blockIdx.x];
     const FbArgs& a = descs[t.x];
     if (skip_domain(a)) return;
+    const bool simple_rec = a.rec && a.pxy == nullptr && a.cv == nullptr && a.rf == nullptr;
+    if constexpr (!EXTRA) {
+        if (simple_rec) {
+            forces_body32<false, true>(a, t.y, smem);
+            return;
+        }
+    }
     if (a.rec) forces_body_mixed<true, EXTRA>(a, t.y, smem);
     else if (a.ev) forces_body_mixed<false, EXTRA, true>(a, t.y, smem);
     else if constexpr (!EXTRA) forces_body32<false>(a, t.y, smem);
@@ -2348,8 +2387,9 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
     if (int rc = make_fargs(c, iter, kick, &a, &lds, &kind)) return rc;
     if (cond) a.halt = c->d_flags;
     const bool mixed = kind & 1, mie = kind & 2, extra = kind & 4;
-    // a record step tallying PE and virial only takes the branch-free loop too (FP64)
-    const bool simple_rec = kind == 0 && a.rec && a.pxy == nullptr && a.cv == nullptr && a.rf == nullptr;
+    // a record step tallying PE and virial only takes the branch-free loop too (FP64, and the
+    // FP32-staged one in mixed precision)
+    const bool simple_rec = (kind == 0 || kind == 1) && a.rec && a.pxy == nullptr && a.cv == nullptr && a.rf == nullptr;
     if (next >= 0 && (((kind == 0 || kind == 1) && !a.rec && !a.ev) || simple_rec) && kick && fused) {
         // this step's force and kick, then pass 1 of step `next` into the other state buffer
         a.s2 = state_of(c->st[1 - c->scur], c->cap);
@@ -2359,7 +2399,8 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         a.fresh = c->rdisp_valid ? 0 : 1;
         a.p1step = next;
         ktimer_begin(c, KT_FORCES_P1);
-        if (simple_rec) k_forces_brick_plain<true, false, true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
+        if (simple_rec && mixed) k_forces_brick_plain32<true, true><<<brick_count(c), FB_BLOCK, lds32(c), c->stream>>>(a);
+        else if (simple_rec) k_forces_brick_plain<true, false, true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
         else if (mixed) k_forces_brick_plain32<true><<<brick_count(c), FB_BLOCK, lds32(c), c->stream>>>(a);
         else k_forces_brick_plain<true, false><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
         ktimer_end(c, KT_FORCES_P1);
@@ -2391,6 +2432,7 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
                : k_forces_brick_mixed<false, E, false><<<grid, blk, lds, c->stream>>>(a)))
     if (mixed) {
         if (extra) { if (a.rec) FM(true, true); else FM(false, true); }
+        else if (simple_rec) k_forces_brick_plain32<false, true><<<grid, blk, lds32(c), c->stream>>>(a);
         else if (a.rec) FM(true, false);
         else if (a.ev) FM(false, false);
         else k_forces_brick_plain32<false><<<grid, blk, lds32(c), c->stream>>>(a);
