@@ -41,6 +41,7 @@ struct WindowGraph {          // the launches of a window of W steps after its r
     hipGraphExec_t g = nullptr;
     int nblk1 = -1, nblk2 = -1;
     size_t lds = 0;
+    uint32_t plain = 0;       // bit j: step j's force launch has no record or virial step
 };
 
 struct fm_batch {
@@ -62,6 +63,7 @@ struct fm_batch {
     // graphs
     bool use_graph = true;
     WindowGraph wg[BATCH_WINDOW + 1];
+    uint32_t plain = 0;       // the current window's plain force launches (bit j)
     int64_t rebuilds = 0;
     int64_t windows = 0;
     hipEvent_t p1done = nullptr;   // pass 1 of a window's first step, for the members' rebuilds
@@ -154,7 +156,8 @@ static char* slot_fargs(fm_batch* b, char* base, int j) { return base + (size_t)
 static int enqueue_window(fm_batch* b, int W, size_t lds) {
     for (int j = 0; j < W; ++j) {
         if (j > 0) TRY(launch_pass1_multi(b->stream, slot_p1(b, b->d_p1, j), b->d_t1, b->nblk1));
-        TRY(launch_forces_multi(b->stream, b->kind, slot_fargs(b, b->d_fargs, j), b->d_t2, b->nblk2, lds));
+        TRY(launch_forces_multi(b->stream, b->kind, slot_fargs(b, b->d_fargs, j), b->d_t2, b->nblk2, lds,
+                                (b->plain >> j) & 1u));
     }
     return 0;
 }
@@ -163,7 +166,7 @@ static int launch_window(fm_batch* b, int W, size_t lds) {
     if (!b->use_graph) return enqueue_window(b, W, lds);
     WindowGraph& w = b->wg[W];
     // a graph's LDS size is fixed: keep it until a member needs more, with headroom
-    if (!w.g || w.nblk1 != b->nblk1 || w.nblk2 != b->nblk2 || lds > w.lds) {
+    if (!w.g || w.nblk1 != b->nblk1 || w.nblk2 != b->nblk2 || lds > w.lds || w.plain != b->plain) {
         if (w.g) hipGraphExecDestroy(w.g);
         w.g = nullptr;
         size_t glds = std::max(lds, lds + lds / 8);
@@ -184,6 +187,7 @@ static int launch_window(fm_batch* b, int W, size_t lds) {
         w.nblk1 = b->nblk1;
         w.nblk2 = b->nblk2;
         w.lds = glds;
+        w.plain = b->plain;
     }
     FM_HIP(hipGraphLaunch(w.g, b->stream));
     return 0;
@@ -328,6 +332,7 @@ extern "C" int fm_batch_run(fm_batch* b, int32_t nsteps, int32_t* n_rebuilds) {
         if (W == 0) break;
         const size_t n_ = n;
         size_t lds = 0;
+        b->plain = 0;
         for (int j = 0; j < W; ++j) {
             // ---- VV pass 1 and the displacement check of every member's step j
             for (size_t k = 0; k < n_; ++k) {
@@ -382,14 +387,17 @@ extern "C" int fm_batch_run(fm_batch* b, int32_t nsteps, int32_t* n_rebuilds) {
                 if (b->dirty) TRY(upload_tables(b));
             }
             // ---- forces with the fused VV pass 2 of every member's step j
+            bool plain = true;
             for (size_t k = 0; k < n_; ++k) {
                 size_t l = 0;
                 int kind = 0;
                 char* fa = slot_fargs(b, b->h_fargs, j) + k * fsz;
                 TRY(make_fargs(b->m[k].c, it0[k] + j, true, fa, &l, &kind));
                 fargs_set_queued(fa, b->m[k].c->d_flags, j < len[k]);
+                if (j < len[k] && !fargs_plain(fa)) plain = false;
                 lds = std::max(lds, l);
             }
+            if (plain) b->plain |= 1u << j;
         }
         // the window's other descriptors in two copies, then its launches
         if (W > 1)

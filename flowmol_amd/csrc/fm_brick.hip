@@ -1880,8 +1880,11 @@ __device__ inline bool halted(int* f, int step) {
     return h;
 }
 
+// (the LJ steps without tallies, with bonds or tethers: registers held to what lets two
+// workgroups share a CU)
 template <bool MIE, bool RECORD, bool EXTRA, bool VIR>
-__global__ __launch_bounds__(FB_BLOCK) void k_forces_brick(FbArgs a) {
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(!MIE && !RECORD ? FB_BLOCK / 128 : 1))) void
+k_forces_brick(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt, a.step)) return;
     forces_body<MIE, RECORD, EXTRA, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
@@ -1939,13 +1942,20 @@ k_forces_brick_plain32(FbArgs a) {
     forces_body32<P1, REC>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
-// packed launch over many domains: block b works on brick table[b].y of domain table[b].x
-template <bool MIE, bool EXTRA>
-__global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_multi(const FbArgs* descs, const int2* table) {
+// packed launch over many domains: block b works on brick table[b].y of domain table[b].x.
+// PLAIN: no domain of the launch is on a record or virial step (only the plain path is compiled,
+// and its registers are held to what lets two workgroups share a CU, as for the one-domain launch)
+template <bool MIE, bool EXTRA, bool PLAIN>
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(PLAIN && !MIE ? FB_BLOCK / 128 : 1))) void
+k_forces_brick_multi(const FbArgs* descs, const int2* table) {
     extern __shared__ __align__(16) char smem[];
     const int2 t = table[blockIdx.x];
     const FbArgs& a = descs[t.x];
     if (skip_domain(a)) return;
+    if constexpr (PLAIN) {
+        forces_body<MIE, false, EXTRA>(a, t.y, smem);
+        return;
+    }
     if (a.rec) forces_body<MIE, true, EXTRA>(a, t.y, smem);
     else if (a.ev) forces_body<MIE, false, EXTRA, true>(a, t.y, smem);
     else forces_body<MIE, false, EXTRA>(a, t.y, smem);
@@ -2071,12 +2081,18 @@ k_forces_brick_mixed(FbArgs a) {
     forces_body_mixed<RECORD, EXTRA, VIR>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
-template <bool EXTRA>
-__global__ __launch_bounds__(FB_BLOCK) void k_forces_brick_mixed_multi(const FbArgs* descs, const int2* table) {
+template <bool EXTRA, bool PLAIN>
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(PLAIN ? FB_BLOCK / 128 : 1))) void
+k_forces_brick_mixed_multi(const FbArgs* descs, const int2* table) {
     extern __shared__ __align__(16) char smem[];
     const int2 t = table[blockIdx.x];
     const FbArgs& a = descs[t.x];
     if (skip_domain(a)) return;
+    if constexpr (PLAIN) {
+        if constexpr (!EXTRA) forces_body32<false>(a, t.y, smem);
+        else forces_body_mixed<false, EXTRA>(a, t.y, smem);
+        return;
+    }
     const bool simple_rec = a.rec && a.pxy == nullptr && a.cv == nullptr && a.rf == nullptr;
     if constexpr (!EXTRA) {
         if (simple_rec) {
@@ -2293,6 +2309,12 @@ static float band32_of(const fm_ctx* c) {
     return (float)(4.0 * (e * (2.0 * sqrt(3.0) * rc + 3.0 * e) + 6.0 * u * c->rc2));
 }
 
+// a step without record or virial tallies
+bool fargs_plain(const void* fargs) {
+    const FbArgs* a = (const FbArgs*)fargs;
+    return !a->rec && !a->ev;
+}
+
 // a domain's force arguments in a queued packed launch: halted by its own flag words, or idle
 void fargs_set_queued(void* fargs, int* flags, bool active) {
     FbArgs* a = (FbArgs*)fargs;
@@ -2453,20 +2475,25 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
     return 0;
 }
 
-int launch_forces_multi(hipStream_t st, int kind, const void* d_descs, const int2* d_table, int nblk, size_t lds) {
+int launch_forces_multi(hipStream_t st, int kind, const void* d_descs, const int2* d_table, int nblk, size_t lds,
+                        bool plain) {
     if (nblk == 0) return 0;
     const FbArgs* d = (const FbArgs*)d_descs;
     const bool mixed = kind & 1, mie = kind & 2, extra = kind & 4;
+#define MULTI(K, ...) \
+    (plain ? K<__VA_ARGS__, true><<<nblk, FB_BLOCK, lds, st>>>(d, d_table) \
+           : K<__VA_ARGS__, false><<<nblk, FB_BLOCK, lds, st>>>(d, d_table))
     if (mixed) {
-        if (extra) k_forces_brick_mixed_multi<true><<<nblk, FB_BLOCK, lds, st>>>(d, d_table);
-        else k_forces_brick_mixed_multi<false><<<nblk, FB_BLOCK, lds, st>>>(d, d_table);
+        if (extra) MULTI(k_forces_brick_mixed_multi, true);
+        else MULTI(k_forces_brick_mixed_multi, false);
     } else if (mie) {
-        if (extra) k_forces_brick_multi<true, true><<<nblk, FB_BLOCK, lds, st>>>(d, d_table);
-        else k_forces_brick_multi<true, false><<<nblk, FB_BLOCK, lds, st>>>(d, d_table);
+        if (extra) MULTI(k_forces_brick_multi, true, true);
+        else MULTI(k_forces_brick_multi, true, false);
     } else {
-        if (extra) k_forces_brick_multi<false, true><<<nblk, FB_BLOCK, lds, st>>>(d, d_table);
-        else k_forces_brick_multi<false, false><<<nblk, FB_BLOCK, lds, st>>>(d, d_table);
+        if (extra) MULTI(k_forces_brick_multi, false, true);
+        else MULTI(k_forces_brick_multi, false, false);
     }
+#undef MULTI
     LAUNCH_CHECK();
     return 0;
 }

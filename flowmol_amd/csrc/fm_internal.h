@@ -228,7 +228,10 @@ size_t fargs_bytes();
 void fargs_set_queued(void* fargs, int* flags, bool active);
 // force-kernel arguments of one domain for step iter; kind: bit0 mixed, bit1 Mie, bit2 bonds/tethers
 int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds, int* kind);
-int launch_forces_multi(hipStream_t st, int kind, const void* d_descs, const int2* d_table, int nblk, size_t lds);
+// plain: no domain of the launch is on a record or virial step
+int launch_forces_multi(hipStream_t st, int kind, const void* d_descs, const int2* d_table, int nblk, size_t lds,
+                        bool plain);
+bool fargs_plain(const void* fargs);
 int rebuild_lists_internal(fm_ctx* c);
 int rebuild_lists_queued_internal(fm_ctx* c);  // the rebuild without host waits (capacity requests in FLAG_NEED)
 int redo_lists_internal(fm_ctx* c);            // after FLAG_NEED: grow the capacities, build the list again
