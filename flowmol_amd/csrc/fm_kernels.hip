@@ -340,10 +340,82 @@ __global__ __launch_bounds__(1024) void k_sendmols_passes(St s, double* teth, co
     }
     const int n = chunk_base;
     __syncthreads();
-    for (int q = tid; q < n; q += blockDim.x) sl[q] = slot_of[vals[q]];
-    __syncthreads();
     const double L[3] = {Lx, Ly, Lz};
     double* pos[3] = {s.x, s.y, s.z};
+    if (n <= (int)blockDim.x) {
+        // One leaver per thread, held in LDS by list position: its slot and position.  The same
+        // passes as below, without a global read in them; a pass without leavers ends at its
+        // count.  The renumbering moves slots and positions between list positions together.
+        __shared__ int cs[1024], cl[1024], ct[1024];
+        __shared__ double cp[3][1024], cq[3][1024];
+        if (tid < n) {
+            const int slot = slot_of[vals[tid]];
+            cs[tid] = slot;
+            cp[0][tid] = pos[0][slot];
+            cp[1][tid] = pos[1][slot];
+            cp[2][tid] = pos[2][slot];
+        }
+        __syncthreads();
+        for (int it = 0; it < 3; ++it)
+            for (int ax = 0; ax < 3; ++ax)
+                for (int dir = -1; dir <= 1; dir += 2) {
+                    const double h = 0.5 * L[ax];
+                    const bool lv = tid < n && (dir == -1 ? (cp[ax][tid] < -h) : (cp[ax][tid] >= h));
+                    const unsigned long long m = __ballot(lv);
+                    if (lane == 0) wsum[wid] = __popcll(m);
+                    __syncthreads();
+                    int off = 0, k = 0;
+                    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+                        if (w < wid) off += wsum[w];
+                        k += wsum[w];
+                    }
+                    __syncthreads();  // (wsum is written again by the next pass)
+                    if (k == 0) continue;
+                    if (lv) cl[off + __popcll(m & ((1ull << lane) - 1ull))] = tid;
+                    __syncthreads();
+                    // wrap the leavers: r(ixyz) - dir*domain(ixyz)
+                    if (tid < k) {
+                        const int q = cl[tid], slot = cs[q];
+                        cp[ax][q] = cp[ax][q] - (double)dir * L[ax];
+                        pos[ax][slot] = cp[ax][q];
+                        if (tag_move && (tag[slot] == FM_TAG_TETH || tag[slot] == FM_TAG_TETH_SLIDE))
+                            teth[slot + (size_t)ax * cap] = teth[slot + (size_t)ax * cap] - (double)dir * L[ax];
+                    }
+                    __syncthreads();
+                    if (tid < k) {
+                        const int q = cl[k - 1 - tid];
+                        ct[tid] = cs[q];
+                        cq[0][tid] = cp[0][q];
+                        cq[1][tid] = cp[1][q];
+                        cq[2][tid] = cp[2][q];
+                    }
+                    __syncthreads();
+                    if (tid < k) {
+                        const int q = cl[tid], slot = ct[tid];
+                        cs[q] = slot;
+                        cp[0][q] = cq[0][tid];
+                        cp[1][q] = cq[1][tid];
+                        cp[2][q] = cq[2][tid];
+                        if (tag_move) {  // read_tag on the renumbered molecule
+                            const int tg = tag[slot];
+                            const double sg = tg == FM_TAG_TETH_SLIDE   ? copysign(1.0, cq[1][tid])
+                                              : tg == FM_TAG_FIXED_SLIDE ? 1.0
+                                                                         : 0.0;
+                            teth[slot + 3 * (size_t)cap] = sg != 0.0 ? wsx * sg : 0.0;
+                            teth[slot + 4 * (size_t)cap] = sg != 0.0 ? wsy * sg : 0.0;
+                            teth[slot + 5 * (size_t)cap] = sg != 0.0 ? wsz * sg : 0.0;
+                        }
+                    }
+                    __syncthreads();
+                }
+        if (tid < n) {
+            ext[cs[tid]] = vals[tid];
+            slot_of[vals[tid]] = cs[tid];
+        }
+        return;
+    }
+    for (int q = tid; q < n; q += blockDim.x) sl[q] = slot_of[vals[q]];
+    __syncthreads();
     for (int it = 0; it < 3; ++it)
         for (int ax = 0; ax < 3; ++ax)
             for (int dir = -1; dir <= 1; dir += 2) {
