@@ -52,6 +52,7 @@ def parse():
                     help="serve: replay each window's packed launches (after its rebuilds) as one HIP graph")
     ap.add_argument("--serve-tune", choices=["on", "off"], default="on",
                     help="serve: brick shape per replica grid from the brick tuner, before the stream starts")
+    ap.add_argument("--brick", default="", help="c3: brick shape BX,BY,BZ (cells) instead of the default")
     ap.add_argument("--extras", choices=["on", "off"], default="on",
                     help="c2 (FP64): also the mixed-precision run and the replica stream, in the same JSON line")
     ap.add_argument("--c4-tune", choices=["on", "off"], default="on",
@@ -257,6 +258,8 @@ def c3_main(a):
     for name, prec in (("fp64", FM_PREC_FP64), ("mixed", FM_PREC_MIXED)):
         p, r, v, tag, rt = config.config_couette(nunits=(n_u, n_u, n_u), nplanes=64, tplot=1, precision=prec)
         p.integration_algorithm = FM_INTEGRATION_LFV
+        if a.brick:
+            p.brick[0], p.brick[1], p.brick[2] = (int(x) for x in a.brick.split(","))
         N = r.shape[0]
         g = MDContext(p, r, v)
         g.set_tags(tag, rt)
