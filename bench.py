@@ -52,7 +52,7 @@ def parse():
                     help="serve: replay each window's packed launches (after its rebuilds) as one HIP graph")
     ap.add_argument("--serve-tune", choices=["on", "off"], default="on",
                     help="serve: brick shape per replica grid from the brick tuner, before the stream starts")
-    ap.add_argument("--brick", default="", help="c3: brick shape BX,BY,BZ (cells) instead of the default")
+    ap.add_argument("--brick", default="", help="c2, c3: brick shape BX,BY,BZ (cells) instead of the default")
     ap.add_argument("--extras", choices=["on", "off"], default="on",
                     help="c2 (FP64): also the mixed-precision run and the replica stream, in the same JSON line")
     ap.add_argument("--c4-tune", choices=["on", "off"], default="on",
@@ -455,6 +455,9 @@ def main():
     p, r, v = config.config_fcc((n_u, n_u, n_u), 0.8, 2.5, seed=2, tplot=10, precision=prec)
     N = r.shape[0]
     brick = None
+    if a.brick and not a.autotune:
+        p.brick[0], p.brick[1], p.brick[2] = (int(x) for x in a.brick.split(","))
+        brick = tuple(p.brick)
     if a.autotune:
         from flowmol_amd.md import tune_brick
 
@@ -541,7 +544,7 @@ def main():
                                   "totenergy_first": logged[0]["totenergy"] if logged else None,
                                   "totenergy_last": logged[-1]["totenergy"] if logged else None,
                                   "pressure_last": logged[-1]["pressure"] if logged else None},
-                   "brick": "autotuned %dx%dx%d" % brick if a.autotune else "default",
+                   "brick": ("autotuned %dx%dx%d" % brick) if a.autotune else ("%dx%dx%d" % brick if brick else "default"),
                    "precision": a.precision},
         "kernels_ms": {k: round(vv[0], 5) for k, vv in kt.items() if vv[1] > 0},
         "roofline": roof,
