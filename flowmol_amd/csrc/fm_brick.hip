@@ -1039,16 +1039,17 @@ __device__ __forceinline__ void lj_word_mixed(const P4& p, int left, double xi, 
         const double dx = xi - p.x[q], dy = yi - p.y[q], dz = zi - p.z[q];
         const bool in = ((dx * dx + dy * dy) + dz * dz) < rc2 && q < left;  // FP64, no contraction
         {
-#pragma clang fp contract(fast)
+            // the FP32 operations spelled out (explicit fused multiply-adds), so that every kernel
+            // that inlines this loop rounds alike
             const float rx = (float)dx, ry = (float)dy, rz = (float)dz;
-            const float r2 = rx * rx + ry * ry + rz * rz;
+            const float r2 = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
             float x = __builtin_amdgcn_rcpf(r2);
             x = in ? x : 0.0f;
             const float x2 = x * x;
-            const float acc = (x2 * x2) * (2.0f * (x2 * x) - 1.0f);
-            fx = fx + acc * rx;
-            fy = fy + acc * ry;
-            fz = fz + acc * rz;
+            const float acc = (x2 * x2) * fmaf(2.0f, x2 * x, -1.0f);
+            fx = fmaf(acc, rx, fx);
+            fy = fmaf(acc, ry, fy);
+            fz = fmaf(acc, rz, fz);
         }
     }
 }
@@ -1684,28 +1685,23 @@ __device__ __forceinline__ bool in_rc_fp64(const FbArgs& a, const Region& r, int
 template <bool REC = false>
 __device__ __forceinline__ void pair32(float dx, float dy, float dz, bool in, float& fx, float& fy, float& fz,
                                        double potshift = 0.0, double* pe = nullptr, double* vt = nullptr) {
-    float acc, r2, x;
-    {
-#pragma clang fp contract(fast)
-        r2 = dx * dx + dy * dy + dz * dz;
-        x = __builtin_amdgcn_rcpf(r2);
-        x = in ? x : 0.0f;
-        const float x2 = x * x;
-        acc = (x2 * x2) * (2.0f * (x2 * x) - 1.0f);
-        fx = fx + acc * dx;
-        fy = fy + acc * dy;
-        fz = fz + acc * dz;
-    }
+    // the FP32 operations spelled out (explicit fused multiply-adds), so that the one-domain and
+    // the packed launches, and the loop and the band entries, round alike
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    float x = __builtin_amdgcn_rcpf(r2);
+    x = in ? x : 0.0f;
+    const float x2 = x * x;
+    const float acc = (x2 * x2) * fmaf(2.0f, x2 * x, -1.0f);
+    fx = fmaf(acc, dx, fx);
+    fy = fmaf(acc, dy, fy);
+    fz = fmaf(acc, dz, fz);
     if constexpr (REC) {
         const double xd = (double)x;
         *pe = *pe + (in ? 4.0 * (p6(xd) - p3(xd)) - potshift : 0.0);
         *vt = *vt + (double)acc * (double)r2;
     }
 }
-__device__ __forceinline__ float r2_32(float dx, float dy, float dz) {
-#pragma clang fp contract(fast)
-    return dx * dx + dy * dy + dz * dz;
-}
+__device__ __forceinline__ float r2_32(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }
 
 // A band entry is left out of the loop's sum and its list position (12 bits) kept: up to four
 // per molecule in band (count in nb); the loop itself has no branch.
@@ -2013,14 +2009,14 @@ __device__ __forceinline__ void forces_body_mixed(const FbArgs& a, int brick, ch
             const float rx = (float)rxd, ry = (float)ryd, rz = (float)rzd;
             float acc, inv;
             {
-#pragma clang fp contract(fast)
-                const float r2 = rx * rx + ry * ry + rz * rz;
+                // explicit fused multiply-adds: the one-domain and packed launches round alike
+                const float r2 = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
                 inv = __builtin_amdgcn_rcpf(r2);
                 const float i2 = inv * inv, i4 = i2 * i2, i3 = i2 * inv;
-                acc = 48.f * (i4 * i3 - 0.5f * i4);
-                fx += acc * rx;
-                fy += acc * ry;
-                fz += acc * rz;
+                acc = 48.f * fmaf(i4, i3, -0.5f * i4);
+                fx = fmaf(acc, rx, fx);
+                fy = fmaf(acc, ry, fy);
+                fz = fmaf(acc, rz, fz);
             }
             if (VIR) vir = vir + (double)acc * ((rxd * rxd + ryd * ryd) + rzd * rzd);
             if (RECORD) {

@@ -153,3 +153,43 @@ def test_server_abort_and_resume():
     assert res2[2]["macros"] == m
     assert np.abs(res2[2]["mop"] - c.mop()).max() <= 1e-12 * max(1.0, np.abs(c.mop()).max())
     c.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lj_batch_members_match_alone(precision):
+    """Plain LJ replicas (no walls or planes) of different sizes in one batch: the packed plain
+    launches (only the plain path compiled), the packed record steps of PE and virial, and in
+    mixed precision the FP32-staged launch.  Positions, velocities and accelerations, and the
+    record sums, are bit-identical to each replica advanced alone by the fused loop."""
+    from flowmol_amd._abi import FM_PREC_MIXED
+
+    prec = FM_PREC_MIXED if precision == "mixed" else 0
+    cases = []
+    for k, n in enumerate((6, 8, 7)):
+        p, r, v = config.config_fcc((n, n, n), 0.8, 2.5, seed=60 + k, tplot=10, precision=prec)
+        r = r + np.random.default_rng(60 + k).uniform(-0.03, 0.03, r.shape)
+        cases.append((p, r, v))
+    say(f"case: batch of 3 LJ replicas, {precision}, 45 steps")
+    ref = []
+    for p, r, v in cases:
+        c = MDContext(p, r, v)
+        c.setup(0)
+        c.run_vv(1, 45)
+        ref.append(c.state() + (c.evaluate_macroscopic_properties(),))
+        c.close()
+    b = Batch(graphs=False)
+    ctx = []
+    for p, r, v in cases:
+        c = MDContext(p, r, v)
+        c.setup(0)
+        ctx.append(c)
+        b.add(c, 1)
+    b.run(20)
+    b.run(25)
+    for k, c in enumerate(ctx):
+        for x, y in zip(c.state(), ref[k][:3]):
+            assert np.array_equal(x, y), k
+        assert c.evaluate_macroscopic_properties() == ref[k][3], k
+    b.close()
+    for c in ctx:
+        c.close()
