@@ -1028,46 +1028,12 @@ __device__ __forceinline__ void lj_word_rec(const P4& p, int left, double xi, do
     }
 }
 
-// Mixed precision: the FP64 coordinate difference converted to FP32, then the same law in FP32
-// (rcp without refinement); an entry that does not count selects 1/r^2 = 0.  Whether a pair
-// counts is decided on the FP64 r^2, as the FP64 path decides it: an FP32 test would move pairs
-// within rounding of rc across the cutoff, a discrete error of the whole pair force.
-__device__ __forceinline__ void lj_word_mixed(const P4& p, int left, double xi, double yi, double zi, double rc2,
-                                              float& fx, float& fy, float& fz) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const double dx = xi - p.x[q], dy = yi - p.y[q], dz = zi - p.z[q];
-        const bool in = ((dx * dx + dy * dy) + dz * dz) < rc2 && q < left;  // FP64, no contraction
-        {
-            // the FP32 operations spelled out (explicit fused multiply-adds), so that every kernel
-            // that inlines this loop rounds alike
-            const float rx = (float)dx, ry = (float)dy, rz = (float)dz;
-            const float r2 = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
-            float x = __builtin_amdgcn_rcpf(r2);
-            x = in ? x : 0.0f;
-            const float x2 = x * x;
-            const float acc = (x2 * x2) * fmaf(2.0f, x2 * x, -1.0f);
-            fx = fmaf(acc, rx, fx);
-            fy = fmaf(acc, ry, fy);
-            fz = fmaf(acc, rz, fz);
-        }
-    }
-}
-template <typename T>
-__device__ __forceinline__ void lj_word_t(const P4& p, int left, double xi, double yi, double zi, double rc2, T& fx,
-                                          T& fy, T& fz) {
-    if constexpr (sizeof(T) == 8) lj_word(p, left, xi, yi, zi, rc2, fx, fy, fz);
-    else lj_word_mixed(p, left, xi, yi, zi, rc2, fx, fy, fz);
-}
-
-// MIXED: the pair law and the per-molecule sum in FP32 (lj_word_mixed), the result in FP64.
-// REC (FP64 only): also the molecule's PE and virial (lj_word_rec).
-template <bool MIXED = false, bool REC = false>
+// REC: also the molecule's PE and virial (lj_word_rec).
+template <bool REC = false>
 __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row, int n, size_t cap, double xi,
                                          double yi, double zi, double rc2, double& fx, double& fy, double& fz,
                                          double potshift = 0.0, double* pe_out = nullptr, double* vir_out = nullptr) {
-    using T = typename std::conditional<MIXED, float, double>::type;
-    T sx = 0, sy = 0, sz = 0;
+    double sx = 0, sy = 0, sz = 0;
     double pe = 0.0, vt = 0.0;
     P4 pa, pb;
     // At the top of each pass pa holds the positions of word k = kw/4, w1 is word k+1 and w2
@@ -1090,18 +1056,18 @@ __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row,
         w2 = row[(size_t)min(k + 4, lastw) * cap];
         __builtin_amdgcn_sched_barrier(0);  // keep the reads above the evaluation
         if constexpr (REC) lj_word_rec(pa, n - kw, xi, yi, zi, rc2, potshift, sx, sy, sz, pe, vt);
-        else lj_word_t(pa, n - kw, xi, yi, zi, rc2, sx, sy, sz);
+        else lj_word(pa, n - kw, xi, yi, zi, rc2, sx, sy, sz);
         lj_stage(pa, pos, kw + 8 < n ? wk2 : 0);
         __builtin_amdgcn_sched_barrier(0);
         // the second word only when a lane of the wavefront has entries in it
         if (__builtin_amdgcn_ballot_w64(kw + 4 < n) != 0ull) {
             if constexpr (REC) lj_word_rec(pb, n - kw - 4, xi, yi, zi, rc2, potshift, sx, sy, sz, pe, vt);
-            else lj_word_t(pb, n - kw - 4, xi, yi, zi, rc2, sx, sy, sz);
+            else lj_word(pb, n - kw - 4, xi, yi, zi, rc2, sx, sy, sz);
         }
     }
-    fx = 24.0 * (double)sx;
-    fy = 24.0 * (double)sy;
-    fz = 24.0 * (double)sz;
+    fx = 24.0 * sx;
+    fy = 24.0 * sy;
+    fz = 24.0 * sz;
     if constexpr (REC) {
         *pe_out = pe;
         *vir_out = 24.0 * vt;
@@ -1111,7 +1077,7 @@ __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row,
 // VIR: virial on a non-record step (teval); record steps always tally it.  P1: VV pass 2 of this
 // step, pass 1 of the next one and its displacement update follow the force (same arithmetic as
 // the separate kernels), written to the other state buffer.
-template <bool MIE, bool RECORD, bool EXTRA, bool VIR = RECORD, bool P1 = false, bool MIXED = false>
+template <bool MIE, bool RECORD, bool EXTRA, bool VIR = RECORD, bool P1 = false>
 __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* smem) {
     Region r;
     const bool mop = RECORD && a.pxy != nullptr;
@@ -1164,11 +1130,11 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         const uint64_t* row = a.nbl + i;
         bool looped = false;  // the pairs went through the branch-free loop
         if constexpr (!MIE && !RECORD && !VIR) {
-            lj_plain<MIXED>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz);
+            lj_plain(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz);
             looped = true;
-        } else if constexpr (!MIE && RECORD && VIR && !EXTRA && !MIXED) {
+        } else if constexpr (!MIE && RECORD && VIR && !EXTRA) {
             if (simple_rec) {  // PE and virial only
-                lj_plain<false, true>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz, a.potshift, &pe, &vir);
+                lj_plain<true>(r.pos, row, n, cap, xi, yi, zi, a.rc2, fx, fy, fz, a.potshift, &pe, &vir);
                 looped = true;
             }
         }
@@ -1908,14 +1874,13 @@ k_forces_brick_mop(FbArgs a) {
 
 // LJ steps without tallies, bonds or tethers (the common step): registers held to what lets two
 // workgroups share a CU (the LDS of two regions fits its 160 KB)
-// (P1: with pass 1 of the next step; MIXED: the FP32 pair law; REC: a record step tallying PE and
-// virial only)
-template <bool P1, bool MIXED, bool REC = false>
+// (P1: with pass 1 of the next step; REC: a record step tallying PE and virial only)
+template <bool P1, bool REC = false>
 __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128, FB_BLOCK / 128))) void
 k_forces_brick_plain(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt, a.step)) return;
-    forces_body<false, REC, false, REC, P1, MIXED>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+    forces_body<false, REC, false, REC, P1>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // A domain's step in a packed launch does nothing when the domain has no step in this launch, or
@@ -2418,9 +2383,9 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         a.p1step = next;
         ktimer_begin(c, KT_FORCES_P1);
         if (simple_rec && mixed) k_forces_brick_plain32<true, true><<<brick_count(c), FB_BLOCK, lds32(c), c->stream>>>(a);
-        else if (simple_rec) k_forces_brick_plain<true, false, true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
+        else if (simple_rec) k_forces_brick_plain<true, true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
         else if (mixed) k_forces_brick_plain32<true><<<brick_count(c), FB_BLOCK, lds32(c), c->stream>>>(a);
-        else k_forces_brick_plain<true, false><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
+        else k_forces_brick_plain<true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
         ktimer_end(c, KT_FORCES_P1);
         LAUNCH_CHECK();
         c->scur = 1 - c->scur;
@@ -2459,10 +2424,10 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         else { if (a.rec) FB(true, true, false); else FB(true, false, false); }
     } else {
         if (extra) { if (a.rec) FB(false, true, true); else FB(false, false, true); }
-        else if (simple_rec) k_forces_brick_plain<false, false, true><<<grid, blk, lds, c->stream>>>(a);
+        else if (simple_rec) k_forces_brick_plain<false, true><<<grid, blk, lds, c->stream>>>(a);
         else if (a.rec) k_forces_brick_lj<true, true><<<grid, blk, lds, c->stream>>>(a);
         else if (a.ev) k_forces_brick_lj<false, true><<<grid, blk, lds, c->stream>>>(a);
-        else k_forces_brick_plain<false, false><<<grid, blk, lds, c->stream>>>(a);
+        else k_forces_brick_plain<false><<<grid, blk, lds, c->stream>>>(a);
     }
 #undef FB
 #undef FM
