@@ -946,6 +946,20 @@ static int size_regions(fm_ctx* c, int measured) {
         if (region_bytes(c, mid) <= budget) lo = mid;
         else hi = mid - 1;
     }
+    // the branch-free Mie launch stages fewer bytes per molecule: its workgroups per CU (two at
+    // most, as its registers allow) stay what its LDS at the measured size gives
+    const size_t mb = brick_lds_bytes(c, m0, 3);
+    if (mb > 0) {
+        const size_t w = std::min<size_t>(2, std::max<size_t>(c->lds_cu / mb, 1));
+        const size_t bm = std::min(c->lds_limit, (c->lds_cu / w) & ~(size_t)1023);
+        int l2 = m0, h2 = lo;
+        while (l2 < h2) {
+            const int mid = l2 + (h2 - l2 + 1) / 2;
+            if (brick_lds_bytes(c, mid, 3) <= bm) l2 = mid;
+            else h2 = mid - 1;
+        }
+        lo = l2;
+    }
     c->maxreg = lo;
     return 0;
 }

@@ -105,6 +105,7 @@ struct Region {
     double* pos;   // staged positions, x y z interleaved (one address, three reads)
     int* lslot;    // optional: slot of each staged molecule (the force kernels: its molecule number)
     int* ltype;    // optional: molecule type (Mie tables) of each staged molecule
+    uint8_t* ltype8 = nullptr;  // optional: the same in one byte (the branch-free Mie launch)
     int nic;
     int total;     // staged molecules
     int ni;        // i-molecules
@@ -261,7 +262,7 @@ __device__ bool stage_region(Region& r, const BG& g, int brick, const int* start
                              const int* mono_src, int cap, int* lchain, int* lsub, int capm, int* flags,
                              const int* type_src = nullptr, const int* slot_map = nullptr) {
     if (!stage_tables(r, g, brick, start, capm, flags)) return false;
-    if (stage_pos || r.lslot || lchain) {
+    if (stage_pos || r.lslot || r.ltype8 || lchain) {
         // up to SU molecules per thread and pass, all their loads issued before the first store
         // (out-of-range indices are clamped to the last molecule and not stored)
         constexpr int SU = 6;
@@ -294,6 +295,7 @@ __device__ bool stage_region(Region& r, const BG& g, int brick, const int* start
                 }
                 if (r.lslot) r.lslot[q] = slot_map ? slot_map[sl[k]] : sl[k];
                 if (r.ltype) r.ltype[q] = type_src[sl[k]];
+                if (r.ltype8) r.ltype8[q] = (uint8_t)type_src[sl[k]];
                 if (lchain) {
                     lchain[q] = mono_src[sl[k]];
                     lsub[q] = mono_src[sl[k] + (size_t)cap];
@@ -1074,6 +1076,65 @@ __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row,
     }
 }
 
+// FENE bonds (simulation_compute_forces_poly) and tethers (tether_force) of molecule i, added to
+// its force (fx, fy, fz already per unit mass); on record steps the bond energy and tallies too.
+template <bool RECORD, bool VIR>
+__device__ __forceinline__ void bond_tether_forces(const FbArgs& a, int i, double xi, double yi, double zi,
+                                                   double inv_mi, bool do_rf, double* rf, double& fx, double& fy,
+                                                   double& fz, double& vir) {
+    const size_t cap = a.cap;
+    if (a.bondn) {  // FENE, simulation_compute_forces_poly
+        const int nb = a.bondn[i];
+        double pp = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            const uint32_t w = a.bond[(size_t)b * cap + i];
+            const int j = (int)(w >> 5);
+            int sx, sy, sz;
+            decode_shift((int)(w & 31u), sx, sy, sz);
+            const double xj = img(a.s.x[j], sx, a.g.L[0]), yj = img(a.s.y[j], sy, a.g.L[1]),
+                         zj = img(a.s.z[j], sz, a.g.L[2]);
+            const double rx = xi - xj, ry = yi - yj, rz = zi - zj;
+            const double r2 = (rx * rx + ry * ry) + rz * rz;
+            if (r2 >= a.R02) {
+                set_err(a.flags, FM_EBOND, a.ext[i]);
+                continue;
+            }
+            const double acc = -a.k_c / (1.0 - (r2 / a.R02));
+            fx = fx + acc * rx * inv_mi;
+            fy = fy + acc * ry * inv_mi;
+            fz = fz + acc * rz * inv_mi;
+            if (VIR) vir = vir + acc * r2;
+            if (RECORD) {
+                if (a.cv) {
+                    const double ri[3] = {xi, yi, zi}, rj[3] = {xj, yj, zj};
+                    cv_entry(a, ri, rj, acc, a.ext[i]);
+                }
+                pp = pp + 0.5 * a.k_c * a.R0 * a.R0 * log(1.0 - (r2 / a.R02));
+                if (do_rf) {
+                    const double rr[3] = {rx, ry, rz};
+                    for (int p = 0; p < 3; ++p)
+                        for (int qq = 0; qq < 3; ++qq) rf[3 * p + qq] = rf[3 * p + qq] + acc * rr[p] * rr[qq];
+                }
+            }
+        }
+        if (RECORD) a.pe_poly[i] = pp;
+    }
+    if (a.tag) {  // tether_force, src/molecular_properties.f90:833-915
+        const int tg = a.tag[i];
+        if (tg == FM_TAG_TETH || tg == FM_TAG_TETH_SLIDE) {
+            double* T = a.teth;
+            const double rio[3] = {xi - T[i], yi - T[i + cap], zi - T[i + 2 * cap]};
+            const double mag = sqrt((rio[0] * rio[0] + rio[1] * rio[1]) + rio[2] * rio[2]);
+            const double am = -2.0 * a.teth_k2 * mag - 4.0 * a.teth_k4 * (mag * mag) -
+                              6.0 * a.teth_k6 * pow(mag, 4.0);
+            fx = fx + am * rio[0];
+            fy = fy + am * rio[1];
+            fz = fz + am * rio[2];
+            for (int d = 0; d < 3; ++d) T[i + d * cap] = T[i + d * cap] + T[i + (3 + d) * cap] * a.dt;
+        }
+    }
+}
+
 // VIR: virial on a non-record step (teval); record steps always tally it.  P1: VV pass 2 of this
 // step, pass 1 of the next one and its displacement update follow the force (same arithmetic as
 // the separate kernels), written to the other state buffer.
@@ -1210,56 +1271,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
             }
         }
         }
-        if (EXTRA && a.bondn) {  // FENE, simulation_compute_forces_poly
-            const int nb = a.bondn[i];
-            double pp = 0.0;
-            for (int b = 0; b < nb; ++b) {
-                const uint32_t w = a.bond[(size_t)b * cap + i];
-                const int j = (int)(w >> 5);
-                int sx, sy, sz;
-                decode_shift((int)(w & 31u), sx, sy, sz);
-                const double xj = img(a.s.x[j], sx, a.g.L[0]), yj = img(a.s.y[j], sy, a.g.L[1]),
-                             zj = img(a.s.z[j], sz, a.g.L[2]);
-                const double rx = xi - xj, ry = yi - yj, rz = zi - zj;
-                const double r2 = (rx * rx + ry * ry) + rz * rz;
-                if (r2 >= a.R02) {
-                    set_err(a.flags, FM_EBOND, a.ext[i]);
-                    continue;
-                }
-                const double acc = -a.k_c / (1.0 - (r2 / a.R02));
-                fx = fx + acc * rx * inv_mi;
-                fy = fy + acc * ry * inv_mi;
-                fz = fz + acc * rz * inv_mi;
-                if (VIR) vir = vir + acc * r2;
-                if (RECORD) {
-                    if (a.cv) {
-                        const double ri[3] = {xi, yi, zi}, rj[3] = {xj, yj, zj};
-                        cv_entry(a, ri, rj, acc, a.ext[i]);
-                    }
-                    pp = pp + 0.5 * a.k_c * a.R0 * a.R0 * log(1.0 - (r2 / a.R02));
-                    if (do_rf) {
-                        const double rr[3] = {rx, ry, rz};
-                        for (int p = 0; p < 3; ++p)
-                            for (int qq = 0; qq < 3; ++qq) rf[3 * p + qq] = rf[3 * p + qq] + acc * rr[p] * rr[qq];
-                    }
-                }
-            }
-            if (RECORD) a.pe_poly[i] = pp;
-        }
-        if (EXTRA && a.tag) {  // tether_force, src/molecular_properties.f90:833-915
-            const int tg = a.tag[i];
-            if (tg == FM_TAG_TETH || tg == FM_TAG_TETH_SLIDE) {
-                double* T = a.teth;
-                const double rio[3] = {xi - T[i], yi - T[i + cap], zi - T[i + 2 * cap]};
-                const double mag = sqrt((rio[0] * rio[0] + rio[1] * rio[1]) + rio[2] * rio[2]);
-                const double am = -2.0 * a.teth_k2 * mag - 4.0 * a.teth_k4 * (mag * mag) -
-                                  6.0 * a.teth_k6 * pow(mag, 4.0);
-                fx = fx + am * rio[0];
-                fy = fy + am * rio[1];
-                fz = fz + am * rio[2];
-                for (int d = 0; d < 3; ++d) T[i + d * cap] = T[i + d * cap] + T[i + (3 + d) * cap] * a.dt;
-            }
-        }
+        if constexpr (EXTRA) bond_tether_forces<RECORD, VIR>(a, i, xi, yi, zi, inv_mi, do_rf, rf, fx, fy, fz, vir);
         if constexpr (P1) {
             // simulation_move_particles_vv(2) of this step, then (1) of the next step and the
             // rdisp update of rebuild criterion 2, as k_pass2 and k_pass1_disp compute them
@@ -1320,6 +1332,216 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         __syncthreads();
         for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x)
             if (spxy[t] != 0.0) atomicAdd(&a.pxy[t], spxy[t]);
+    }
+}
+
+// ---- Mie steps without tallies ------------------------------------------------------------------
+// The Mie pair law of forces_body's general loop over every list entry without branches, when each
+// type pair's exponents lr/2 + 1 and la/2 + 1 are whole numbers below 2^NB.  Molecule i's row of
+// the type tables (NT partner types) sits in registers: the pair cutoff, A = C eps lr and
+// B = C eps alpha la, and the exponents' bits; an entry picks them by its partner's type.  The
+// powers x^n of x = 1/r^2 (one Newton step) come from the binary expansion of n, with the
+// multiplications of mie_pow; the magnitude is A x^nr - B x^na, and an entry beyond its pair's
+// cutoff or past the end of the list gets zero.  The pair forces are summed per unit mass and
+// divided by the mass at the end.
+template <int NT, int NB>
+struct MieRow {
+    double rc2[NT], A[NT], B[NT];
+    int e[NT];  // lr/2 + 1 | (la/2 + 1) << 8
+};
+
+template <int NT, int NB>
+__device__ __forceinline__ void mie_row(const FbArgs& a, int ti, MieRow<NT, NB>& m) {
+    const int T2 = FM_MAX_TYPES * FM_MAX_TYPES;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int k = ti * FM_MAX_TYPES + t;
+        const double eps = a.mie[k], lr = a.mie[T2 + k], la = a.mie[2 * T2 + k], C = a.mie[3 * T2 + k],
+                     al = a.mie[4 * T2 + k], c = a.mie[6 * T2 + k];
+        m.rc2[t] = c > 0.0 ? c : a.rc2;
+        m.A[t] = (C * eps) * lr;
+        m.B[t] = (C * eps) * (al * la);
+        m.e[t] = (int)(0.5 * lr + 1.0) | (int)(0.5 * la + 1.0) << 8;
+    }
+}
+
+// the row entry of partner type tj (types are below NT)
+template <int NT, int NB>
+__device__ __forceinline__ void mie_pick(const MieRow<NT, NB>& m, int tj, double& rc2, double& A, double& B,
+                                         int& e) {
+    static_assert(NT == 2 || NT == 4, "two or four partner types");
+    if constexpr (NT == 2) {
+        const bool s = tj != 0;
+        rc2 = s ? m.rc2[1] : m.rc2[0];
+        A = s ? m.A[1] : m.A[0];
+        B = s ? m.B[1] : m.B[0];
+        e = s ? m.e[1] : m.e[0];
+    } else {
+        const bool s0 = (tj & 1) != 0, s1 = (tj & 2) != 0;
+        rc2 = s1 ? (s0 ? m.rc2[3] : m.rc2[2]) : (s0 ? m.rc2[1] : m.rc2[0]);
+        A = s1 ? (s0 ? m.A[3] : m.A[2]) : (s0 ? m.A[1] : m.A[0]);
+        B = s1 ? (s0 ? m.B[3] : m.B[2]) : (s0 ? m.B[1] : m.B[0]);
+        e = s1 ? (s0 ? m.e[3] : m.e[2]) : (s0 ? m.e[1] : m.e[0]);
+    }
+}
+
+// x^n from pw[k] = x^(2^k) and the bits of n (mie_pow's products, in its order)
+template <int NB>
+__device__ __forceinline__ double mie_bits_pow(const double (&pw)[NB], int n) {
+    double r = (n & 1) ? pw[0] : 1.0;
+#pragma unroll
+    for (int k = 1; k < NB; ++k) r = ((n >> k) & 1) ? r * pw[k] : r;
+    return r;
+}
+
+struct P4M {
+    double x[4], y[4], z[4];
+    int t[4];
+};
+__device__ __forceinline__ void mie_stage(P4M& p, const double* pos, const uint8_t* lt, uint64_t w) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int j = (int)((w >> (16 * q)) & 0xffffu);
+        p.x[q] = pos[3 * j];
+        p.y[q] = pos[3 * j + 1];
+        p.z[q] = pos[3 * j + 2];
+        p.t[q] = lt[j];
+    }
+}
+template <int NT, int NB>
+__device__ __forceinline__ void mie_word(const P4M& p, int left, double xi, double yi, double zi,
+                                         const MieRow<NT, NB>& m, double& fx, double& fy, double& fz) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double rx = xi - p.x[q], ry = yi - p.y[q], rz = zi - p.z[q];
+        const double r2 = (rx * rx + ry * ry) + rz * rz;
+        double rc2, A, B;
+        int e;
+        mie_pick<NT, NB>(m, p.t[q], rc2, A, B, e);
+        double pw[NB];
+        pw[0] = recip1(r2);
+#pragma unroll
+        for (int k = 1; k < NB; ++k) pw[k] = pw[k - 1] * pw[k - 1];
+        const double acc = fmaf64(A, mie_bits_pow<NB>(pw, e), -(B * mie_bits_pow<NB>(pw, e >> 8)));
+        // a select, not a product: a padding entry may sit at r = 0 (an infinite or NaN acc)
+        const double g = (r2 < rc2 && q < left) ? acc : 0.0;
+        fx = fmaf64(g, rx, fx);
+        fy = fmaf64(g, ry, fy);
+        fz = fmaf64(g, rz, fz);
+    }
+}
+
+// The list loop of lj_plain (two words in flight, the word after them loading) over mie_word.
+template <int NT, int NB>
+__device__ __forceinline__ void mie_plain(const double* pos, const uint8_t* lt, const uint64_t* row, int n,
+                                          size_t cap, double xi, double yi, double zi, const MieRow<NT, NB>& m,
+                                          double& fx, double& fy, double& fz) {
+    double sx = 0, sy = 0, sz = 0;
+    P4M pa, pb;
+    const int lastw = n > 0 ? (n - 1) >> 2 : 0;
+    const uint64_t w0 = row[0];
+    mie_stage(pa, pos, lt, n > 0 ? w0 : 0);
+    uint64_t w1 = row[(size_t)min(1, lastw) * cap];
+    uint64_t w2 = row[(size_t)min(2, lastw) * cap];
+    for (int kw = 0; kw < n; kw += 8) {
+        const int k = kw >> 2;
+        mie_stage(pb, pos, lt, kw + 4 < n ? w1 : 0);
+        const uint64_t wk2 = w2;
+        w1 = row[(size_t)min(k + 3, lastw) * cap];
+        w2 = row[(size_t)min(k + 4, lastw) * cap];
+        __builtin_amdgcn_sched_barrier(0);
+        mie_word<NT, NB>(pa, n - kw, xi, yi, zi, m, sx, sy, sz);
+        mie_stage(pa, pos, lt, kw + 8 < n ? wk2 : 0);
+        __builtin_amdgcn_sched_barrier(0);
+        if (__builtin_amdgcn_ballot_w64(kw + 4 < n) != 0ull) mie_word<NT, NB>(pb, n - kw - 4, xi, yi, zi, m, sx, sy, sz);
+    }
+    fx = sx;
+    fy = sy;
+    fz = sz;
+}
+
+// LDS of the branch-free Mie launch: positions, region cell and one-byte type of each staged
+// molecule, then the region tables (no slots; the type rows are read from HBM once per molecule)
+__device__ inline void carve_mie(Region& r, char* smem, const BG& g, int maxreg) {
+    r.pos = (double*)smem;
+    int* ip = (int*)(r.pos + 3 * (size_t)maxreg);
+    r.lslot = nullptr;
+    r.ltype = nullptr;
+    r.rq = (uint16_t*)ip;
+    ip += (maxreg + 1) / 2;
+    r.ltype8 = (uint8_t*)ip;
+    ip += (maxreg + 3) / 4;
+    r.off = ip;
+    ip += g.nreg + 1;
+    r.gc = ip;
+    ip += g.nreg;
+    r.sh = ip;
+    ip += g.nreg;
+    r.cst = ip;
+    ip += g.nreg;
+    const int nbmax = g.B[0] * g.B[1] * g.B[2];
+    r.ic_off = ip;
+    ip += nbmax + 1;
+    r.ic_rc = ip;
+}
+__host__ __device__ inline size_t mie_lds_bytes(const BG& g, int maxreg) {
+    const int nbmax = g.B[0] * g.B[1] * g.B[2];
+    const size_t b = (size_t)maxreg * 3 * sizeof(double) +
+                     (size_t)((maxreg + 1) / 2 + (maxreg + 3) / 4 + 4 * g.nreg + 1 + 2 * nbmax + 1) * sizeof(int);
+    return (b + 15) & ~(size_t)15;
+}
+
+// Forces of a Mie step without tallies (bonds and tethers too when EXTRA, added to the pair sum
+// after it); the kick of a fused simulation_move_particles_vv(2) as forces_body does it.
+template <int NT, int NB, bool EXTRA>
+__device__ __forceinline__ void forces_body_mie(const FbArgs& a, int brick, char* smem) {
+    Region r;
+    carve_mie(r, smem, a.g, a.maxreg);
+    if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags,
+                      a.moltype, nullptr))
+        return;
+    const int T2 = FM_MAX_TYPES * FM_MAX_TYPES;
+    if constexpr (EXTRA) {
+        // bonds and tethers in a pass of their own (each thread its own molecules), their force
+        // parked in a: what they read is then not held through the pair loops
+        for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
+            int rci, li, i;
+            imol(r, a.start, q, rci, li, i);
+            const double inv_mi = 1.0 / a.mie[7 * T2 + a.moltype[i]];
+            double bx = 0.0, by = 0.0, bz = 0.0, vir = 0.0;
+            bond_tether_forces<false, false>(a, i, r.pos[3 * li], r.pos[3 * li + 1], r.pos[3 * li + 2], inv_mi, false,
+                                             nullptr, bx, by, bz, vir);
+            a.s.ax[i] = bx;
+            a.s.ay[i] = by;
+            a.s.az[i] = bz;
+        }
+    }
+    for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
+        int rci, li, i;
+        imol(r, a.start, q, rci, li, i);
+        const double xi = r.pos[3 * li], yi = r.pos[3 * li + 1], zi = r.pos[3 * li + 2];
+        const int ti = a.moltype[i];
+        MieRow<NT, NB> m;
+        mie_row<NT, NB>(a, ti, m);
+        const double inv_mi = 1.0 / a.mie[7 * T2 + ti];
+        double fx, fy, fz;
+        mie_plain<NT, NB>(r.pos, r.ltype8, a.nbl + i, a.nbn[i], a.cap, xi, yi, zi, m, fx, fy, fz);
+        fx = fx * inv_mi;
+        fy = fy * inv_mi;
+        fz = fz * inv_mi;
+        if constexpr (EXTRA) {
+            fx = fx + a.s.ax[i];
+            fy = fy + a.s.ay[i];
+            fz = fz + a.s.az[i];
+        }
+        a.s.ax[i] = fx;
+        a.s.ay[i] = fy;
+        a.s.az[i] = fz;
+        if (a.kick) {
+            a.s.vx[i] = a.s.vx[i] + a.hdt * fx;
+            a.s.vy[i] = a.s.vy[i] + a.hdt * fy;
+            a.s.vz[i] = a.s.vz[i] + a.hdt * fz;
+        }
     }
 }
 
@@ -1883,6 +2105,16 @@ k_forces_brick_plain(FbArgs a) {
     forces_body<false, REC, false, REC, P1>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
+// Mie steps without tallies (forces_body_mie): registers held to what lets two workgroups share a
+// CU (the LDS of two regions with one-byte types fits its 160 KB)
+template <int NT, int NB, bool EXTRA>
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128, FB_BLOCK / 128))) void
+k_forces_brick_mie(FbArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    if (halted(a.halt, a.step)) return;
+    forces_body_mie<NT, NB, EXTRA>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+}
+
 // A domain's step in a packed launch does nothing when the domain has no step in this launch, or
 // when a rebuild is pending for this step or an earlier one (the rule of halted(), read only:
 // a packed launch's first workgroup belongs to one domain, so none raises FLAG_HALT).
@@ -2147,14 +2379,39 @@ int brick_count(const fm_ctx* c) {
     return g.nb[0] * g.nb[1] * g.nb[2];
 }
 
+// Partner types of the branch-free Mie launch (2 or 4; bits of the exponents in *nb), or 0 when
+// some type pair's exponent lr/2 + 1 or la/2 + 1 is not a whole number below 32 (the general loop
+// then runs)
+static int mie_plain_types(const fm_ctx* c, int* nb) {
+    const fm_params& p = c->p;
+    if (p.mie_potential != 1) return 0;
+    int emax = 0;
+    for (int ti = 0; ti < p.ntypes; ++ti)
+        for (int tj = 0; tj < p.ntypes; ++tj) {
+            const int k = ti * FM_MAX_TYPES + tj;
+            for (double lam : {p.lambdar_lookup[k], p.lambdaa_lookup[k]}) {
+                const double e = 0.5 * lam + 1.0;
+                if (!(e >= 0.0 && e < 32.0) || e != floor(e)) return 0;
+                emax = std::max(emax, (int)e);
+            }
+        }
+    *nb = emax < 16 ? 4 : 5;
+    return p.ntypes <= 2 ? 2 : 4;
+}
+
 size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind) {
     BG g = make_bg(c);
-    // kind 0: list build (+ slots and chain ids with bonds); 1: forces; 2: forces with MOP
+    // kind 0: list build (+ slots and chain ids with bonds); 1: forces; 2: forces with MOP;
+    // 3: the branch-free Mie launch (0 when it does not run)
     if (kind == 0) {
         bool poly = c->p.potential_flag == 1;
         return region_lds_bytes(g, maxreg, poly) + (poly ? 2 * (size_t)maxreg * sizeof(int) : 0);
     }
     const bool mie = c->p.mie_potential == 1;
+    if (kind == 3) {  // the branch-free Mie launch, or 0 when it does not run
+        int nb;
+        return mie_plain_types(c, &nb) > 0 ? mie_lds_bytes(g, maxreg) : 0;
+    }
     const size_t tab = mie ? FM_MIE_WORDS * sizeof(double) : 0;  // Mie tables
     if (kind == 1) return region_lds_bytes(g, maxreg, mie || c->p.vflux_outflag == 4, mie) + tab;
     // kind 2: the plane tallies, and for the MOP record kernel the plane positions
@@ -2420,7 +2677,17 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         else if (a.ev) FM(false, false);
         else k_forces_brick_plain32<false><<<grid, blk, lds32(c), c->stream>>>(a);
     } else if (mie) {
-        if (extra) { if (a.rec) FB(true, true, true); else FB(true, false, true); }
+        int nb = 0;
+        const int nt = (!a.rec && !a.ev) ? mie_plain_types(c, &nb) : 0;
+        if (nt > 0) {  // a step without tallies: the branch-free loop
+            const size_t lm = brick_lds_bytes(c, c->maxreg, 3);
+#define FMIE(NT, NB)                                                               \
+    (extra ? k_forces_brick_mie<NT, NB, true><<<grid, blk, lm, c->stream>>>(a) \
+           : k_forces_brick_mie<NT, NB, false><<<grid, blk, lm, c->stream>>>(a))
+            if (nt == 2) { if (nb == 4) FMIE(2, 4); else FMIE(2, 5); }
+            else { if (nb == 4) FMIE(4, 4); else FMIE(4, 5); }
+#undef FMIE
+        } else if (extra) { if (a.rec) FB(true, true, true); else FB(true, false, true); }
         else { if (a.rec) FB(true, true, false); else FB(true, false, false); }
     } else {
         if (extra) { if (a.rec) FB(false, true, true); else FB(false, false, true); }

@@ -239,7 +239,7 @@ int check_flags_internal(fm_ctx* c);           // report a device-side error in 
 
 // LDS-brick kernels (fm_brick.hip)
 int brick_count(const fm_ctx* c);
-size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind);
+size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind);  // kind 0 list build, 1 forces, 2 MOP, 3 Mie plain
 int launch_region_max(fm_ctx* c);
 int launch_decode_nlist(fm_ctx* c, uint32_t* out);
 

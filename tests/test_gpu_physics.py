@@ -576,3 +576,40 @@ def test_polymer_random_walk_start():
     assert ro == rg
     assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-9
     g.close()
+
+
+@pytest.mark.parametrize("case", ["whole_exponents", "half_exponent"])
+def test_mie_plain_step_forces(case):
+    """Forces of a step without tallies against the oracle, then a 20-step drop-in run, for a
+    fluid of four Mie types with per-pair cutoffs (2^(1/6) to 2.5) and masses 0.5 to 2.  With
+    every exponent lr/2+1, la/2+1 whole (up to x^18: five bits) the step takes the branch-free
+    loop; with one pair at lr = 13 (x^7.5) the general loop."""
+    p, r, v = config.config_fcc((6, 6, 6), 0.8, 2.5, seed=71, tplot=10)
+    lr11 = 34.0 if case == "whole_exponents" else 13.0
+    config.set_mie_types(p, {(0, 0): dict(lr=12.0, la=6.0), (0, 1): dict(lr=14.0, la=6.0, eps=0.8, rc=2.0),
+                             (0, 2): dict(lr=20.0, la=6.0, rc=1.8), (0, 3): dict(lr=12.0, la=6.0, rc=2 ** (1 / 6)),
+                             (1, 1): dict(lr=lr11, la=8.0, eps=1.2, alpha=0.9), (1, 2): dict(lr=16.0, la=6.0, rc=2.2),
+                             (1, 3): dict(lr=12.0, la=6.0, rc=1.5), (2, 2): dict(lr=14.0, la=6.0),
+                             (2, 3): dict(lr=12.0, la=6.0, eps=0.5), (3, 3): dict(lr=24.0, la=12.0)})
+    for t, m in enumerate((1.0, 2.0, 0.5, 1.5)):
+        p.mass_lookup[t] = m
+    rng = np.random.default_rng(71)
+    r = r + rng.uniform(-0.05, 0.05, r.shape)
+    mt = rng.integers(0, 4, r.shape[0]).astype(np.int32)
+    say("case: Mie plain step, four types, %s, 864 molecules" % case)
+    o = Oracle(p, r, v)
+    o.set_moltype(mt)
+    g = MDContext(p, r, v)
+    g.set_moltype(mt)
+    o.setup()
+    o.compute_forces(0)
+    g.setup(0)
+    o.compute_forces(1)   # a step without tallies (TPLOT 10)
+    g.simulation_compute_forces(1)
+    ao, ag = o.state()[2], g.state()[2]
+    assert np.abs(ao).max() > 1.0
+    assert rel_force_err(ag, ao, o.force_scale()) < TOL
+    ro, rg = drop_in_steps(o, g, 1, 20)
+    assert ro == rg
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-9
+    g.close()
