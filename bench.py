@@ -52,7 +52,7 @@ def parse():
                     help="serve: replay each window's packed launches (after its rebuilds) as one HIP graph")
     ap.add_argument("--serve-tune", choices=["on", "off"], default="on",
                     help="serve: brick shape per replica grid from the brick tuner, before the stream starts")
-    ap.add_argument("--brick", default="", help="c2, c3: brick shape BX,BY,BZ (cells) instead of the default")
+    ap.add_argument("--brick", default="", help="c2, c3, c4: brick shape BX,BY,BZ (cells) instead of the default (c4: instead of the tuner)")
     ap.add_argument("--extras", choices=["on", "off"], default="on",
                     help="c2 (FP64): also the mixed-precision run and the replica stream, in the same JSON line")
     ap.add_argument("--c4-tune", choices=["on", "off"], default="on",
@@ -211,7 +211,10 @@ def c4_main(a):
         ctx.set_monomers(*mono)
 
     brick = "default"
-    if a.autotune or a.c4_tune == "on":
+    if a.brick:
+        p.brick[0], p.brick[1], p.brick[2] = (int(x) for x in a.brick.split(","))
+        brick = "given %s" % a.brick
+    elif a.autotune or a.c4_tune == "on":
         from flowmol_amd.md import tune_brick
         best, tuned = tune_brick(p, r, v, prepare=prepare, max_candidates=6)
         p.brick[0], p.brick[1], p.brick[2] = best

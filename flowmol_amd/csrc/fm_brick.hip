@@ -1076,6 +1076,46 @@ __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row,
     }
 }
 
+// simulation_move_particles_vv(2) of this step, then (1) of the next step and the rdisp update of
+// rebuild criterion 2 (the arithmetic of k_pass2 and k_pass1_disp), the state after them written
+// to s2; the first lane of the wavefront that sees a displacement beyond the threshold raises the
+// rebuild flag with the next step's number.  KEEP_V (record steps): v at the end of this step stays
+// in s for the record-step sums (no other thread of the launch reads this molecule's velocity).
+template <bool KEEP_V>
+__device__ __forceinline__ void vv_pass2_pass1(const FbArgs& a, int i, double fx, double fy, double fz) {
+    const double ux = a.s.vx[i] + a.hdt * fx, uy = a.s.vy[i] + a.hdt * fy, uz = a.s.vz[i] + a.hdt * fz;
+    a.s2.x[i] = a.s.x[i] + a.dt * ux + a.hdt2 * fx;
+    a.s2.y[i] = a.s.y[i] + a.dt * uy + a.hdt2 * fy;
+    a.s2.z[i] = a.s.z[i] + a.dt * uz + a.hdt2 * fz;
+    const double wx = ux + a.hdt * fx, wy = uy + a.hdt * fy, wz = uz + a.hdt * fz;
+    a.s2.vx[i] = wx;
+    a.s2.vy[i] = wy;
+    a.s2.vz[i] = wz;
+    a.s2.ax[i] = fx;
+    a.s2.ay[i] = fy;
+    a.s2.az[i] = fz;
+    const size_t cp = a.cap;
+    double d0 = a.fresh ? 0.0 : a.rd[i], d1 = a.fresh ? 0.0 : a.rd[i + cp], d2 = a.fresh ? 0.0 : a.rd[i + 2 * cp];
+    d0 = d0 + wx * a.dt;
+    d1 = d1 + wy * a.dt;
+    d2 = d2 + wz * a.dt;
+    a.rd[i] = d0;
+    a.rd[i + cp] = d1;
+    a.rd[i + 2 * cp] = d2;
+    const bool hit = fabs(d0) > a.thresh || fabs(d1) > a.thresh || fabs(d2) > a.thresh;
+    const unsigned long long m = __ballot(hit);
+    if (m != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1 &&
+        __atomic_load_n(&a.flags[FLAG_REBUILD], __ATOMIC_RELAXED) == 0) {
+        atomicOr(&a.flags[FLAG_REBUILD], 1);
+        atomicExch(&a.flags[FLAG_RB_STEP], a.p1step);
+    }
+    if constexpr (KEEP_V) {
+        a.s.vx[i] = ux;
+        a.s.vy[i] = uy;
+        a.s.vz[i] = uz;
+    }
+}
+
 // FENE bonds (simulation_compute_forces_poly) and tethers (tether_force) of molecule i, added to
 // its force (fx, fy, fz already per unit mass); on record steps the bond energy and tallies too.
 template <bool RECORD, bool VIR>
@@ -1273,44 +1313,9 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         }
         if constexpr (EXTRA) bond_tether_forces<RECORD, VIR>(a, i, xi, yi, zi, inv_mi, do_rf, rf, fx, fy, fz, vir);
         if constexpr (P1) {
-            // simulation_move_particles_vv(2) of this step, then (1) of the next step and the
-            // rdisp update of rebuild criterion 2, as k_pass2 and k_pass1_disp compute them
-            const double ux = a.s.vx[i] + a.hdt * fx, uy = a.s.vy[i] + a.hdt * fy, uz = a.s.vz[i] + a.hdt * fz;
-            a.s2.x[i] = a.s.x[i] + a.dt * ux + a.hdt2 * fx;
-            a.s2.y[i] = a.s.y[i] + a.dt * uy + a.hdt2 * fy;
-            a.s2.z[i] = a.s.z[i] + a.dt * uz + a.hdt2 * fz;
-            const double wx = ux + a.hdt * fx, wy = uy + a.hdt * fy, wz = uz + a.hdt * fz;
-            a.s2.vx[i] = wx;
-            a.s2.vy[i] = wy;
-            a.s2.vz[i] = wz;
-            a.s2.ax[i] = fx;
-            a.s2.ay[i] = fy;
-            a.s2.az[i] = fz;
-            const size_t cp = a.cap;
-            double d0 = a.fresh ? 0.0 : a.rd[i], d1 = a.fresh ? 0.0 : a.rd[i + cp],
-                   d2 = a.fresh ? 0.0 : a.rd[i + 2 * cp];
-            d0 = d0 + wx * a.dt;
-            d1 = d1 + wy * a.dt;
-            d2 = d2 + wz * a.dt;
-            a.rd[i] = d0;
-            a.rd[i + cp] = d1;
-            a.rd[i + 2 * cp] = d2;
-            const bool hit = fabs(d0) > a.thresh || fabs(d1) > a.thresh || fabs(d2) > a.thresh;
-            const unsigned long long m = __ballot(hit);
-            if (m != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1 &&
-                __atomic_load_n(&a.flags[FLAG_REBUILD], __ATOMIC_RELAXED) == 0) {
-                atomicOr(&a.flags[FLAG_REBUILD], 1);
-                atomicExch(&a.flags[FLAG_RB_STEP], a.p1step);
-            }
+            vv_pass2_pass1<RECORD>(a, i, fx, fy, fz);
             if (VIR) a.vir[i] = vir;
-            if (RECORD) {
-                a.pe_lj[i] = pe;
-                // v at the end of this step stays in this buffer for the record-step sums (no
-                // other thread of this launch reads this molecule's velocity)
-                a.s.vx[i] = ux;
-                a.s.vy[i] = uy;
-                a.s.vz[i] = uz;
-            }
+            if (RECORD) a.pe_lj[i] = pe;
             continue;
         }
         a.s.ax[i] = fx;
@@ -1335,23 +1340,27 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
     }
 }
 
-// ---- Mie steps without tallies ------------------------------------------------------------------
+// ---- Mie steps without tallies, and record steps of PE and virial -------------------------------
 // The Mie pair law of forces_body's general loop over every list entry without branches, when each
 // type pair's exponents lr/2 + 1 and la/2 + 1 are whole numbers below 2^NB.  Molecule i's row of
 // the type tables (NT partner types) sits in registers: the pair cutoff, A = C eps lr and
-// B = C eps alpha la, and the exponents' bits; an entry picks them by its partner's type.  The
-// powers x^n of x = 1/r^2 (one Newton step) come from the binary expansion of n, with the
-// multiplications of mie_pow; the magnitude is A x^nr - B x^na, and an entry beyond its pair's
-// cutoff or past the end of the list gets zero.  The pair forces are summed per unit mass and
-// divided by the mass at the end.
-template <int NT, int NB>
+// B = C eps alpha la, and the exponents; an entry picks them by its partner's type.  The powers
+// x^n of x = 1/r^2 come from the binary expansion of n, with the multiplications of mie_pow; the
+// magnitude is A x^nr - B x^na, and an entry beyond its pair's cutoff or past the end of the list
+// gets zero.  The pair forces are summed per unit mass and divided by the mass at the end.  REC
+// (record steps tallying PE and virial only): x from two Newton steps (the IEEE quotient, as the
+// general loop), the PE term C eps (x^(lr/2) - alpha x^(la/2)) - potshift of the pair and the
+// virial magnitude times r^2, both zero for an entry that does not count.  Plain steps take x
+// from one Newton step.
+template <int NT, int REC>
 struct MieRow {
     double rc2[NT], A[NT], B[NT];
-    int e[NT];  // lr/2 + 1 | (la/2 + 1) << 8
+    int e[NT];  // lr/2 + 1 | (la/2 + 1) << 8 | (lr/2) << 16 | (la/2) << 24
+    double ce[REC ? NT : 1], al[REC ? NT : 1], sh[REC ? NT : 1];  // C eps, alpha, potshift
 };
 
-template <int NT, int NB>
-__device__ __forceinline__ void mie_row(const FbArgs& a, int ti, MieRow<NT, NB>& m) {
+template <int NT, int REC>
+__device__ __forceinline__ void mie_row(const FbArgs& a, int ti, MieRow<NT, REC>& m) {
     const int T2 = FM_MAX_TYPES * FM_MAX_TYPES;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -1361,28 +1370,27 @@ __device__ __forceinline__ void mie_row(const FbArgs& a, int ti, MieRow<NT, NB>&
         m.rc2[t] = c > 0.0 ? c : a.rc2;
         m.A[t] = (C * eps) * lr;
         m.B[t] = (C * eps) * (al * la);
-        m.e[t] = (int)(0.5 * lr + 1.0) | (int)(0.5 * la + 1.0) << 8;
+        const int nr = (int)(0.5 * lr + 1.0), na = (int)(0.5 * la + 1.0);
+        m.e[t] = nr | na << 8 | (nr - 1) << 16 | (na - 1) << 24;
+        if constexpr (REC) {
+            m.ce[t] = C * eps;
+            m.al[t] = al;
+            m.sh[t] = a.mie[5 * T2 + k];
+        }
     }
 }
 
-// the row entry of partner type tj (types are below NT)
-template <int NT, int NB>
-__device__ __forceinline__ void mie_pick(const MieRow<NT, NB>& m, int tj, double& rc2, double& A, double& B,
-                                         int& e) {
+// v[tj] of a row array (types are below NT)
+template <int NT, typename T>
+__device__ __forceinline__ T mie_sel(const T (&v)[NT], int tj) {
     static_assert(NT == 2 || NT == 4, "two or four partner types");
-    if constexpr (NT == 2) {
-        const bool s = tj != 0;
-        rc2 = s ? m.rc2[1] : m.rc2[0];
-        A = s ? m.A[1] : m.A[0];
-        B = s ? m.B[1] : m.B[0];
-        e = s ? m.e[1] : m.e[0];
-    } else {
-        const bool s0 = (tj & 1) != 0, s1 = (tj & 2) != 0;
-        rc2 = s1 ? (s0 ? m.rc2[3] : m.rc2[2]) : (s0 ? m.rc2[1] : m.rc2[0]);
-        A = s1 ? (s0 ? m.A[3] : m.A[2]) : (s0 ? m.A[1] : m.A[0]);
-        B = s1 ? (s0 ? m.B[3] : m.B[2]) : (s0 ? m.B[1] : m.B[0]);
-        e = s1 ? (s0 ? m.e[3] : m.e[2]) : (s0 ? m.e[1] : m.e[0]);
-    }
+    if constexpr (NT == 2) return tj != 0 ? v[1] : v[0];
+    const bool s0 = (tj & 1) != 0, s1 = (tj & 2) != 0;
+    // the values pass an empty asm, so that the selects are not turned into an indexed load (which
+    // would keep the row in scratch memory)
+    T v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
+    asm("" : "+v"(v0), "+v"(v1), "+v"(v2), "+v"(v3));
+    return s1 ? (s0 ? v3 : v2) : (s0 ? v1 : v0);
 }
 
 // x^n from pw[k] = x^(2^k) and the bits of n (mie_pow's products, in its order)
@@ -1408,35 +1416,44 @@ __device__ __forceinline__ void mie_stage(P4M& p, const double* pos, const uint8
         p.t[q] = lt[j];
     }
 }
-template <int NT, int NB>
+template <int NT, int NB, bool REC>
 __device__ __forceinline__ void mie_word(const P4M& p, int left, double xi, double yi, double zi,
-                                         const MieRow<NT, NB>& m, double& fx, double& fy, double& fz) {
+                                         const MieRow<NT, REC>& m, double& fx, double& fy, double& fz, double& pe,
+                                         double& vt) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const double rx = xi - p.x[q], ry = yi - p.y[q], rz = zi - p.z[q];
         const double r2 = (rx * rx + ry * ry) + rz * rz;
-        double rc2, A, B;
-        int e;
-        mie_pick<NT, NB>(m, p.t[q], rc2, A, B, e);
+        const int tj = p.t[q];
+        const int e = mie_sel<NT>(m.e, tj);
         double pw[NB];
-        pw[0] = recip1(r2);
+        pw[0] = REC ? recip(r2) : recip1(r2);
 #pragma unroll
         for (int k = 1; k < NB; ++k) pw[k] = pw[k - 1] * pw[k - 1];
-        const double acc = fmaf64(A, mie_bits_pow<NB>(pw, e), -(B * mie_bits_pow<NB>(pw, e >> 8)));
+        const double acc =
+            fmaf64(mie_sel<NT>(m.A, tj), mie_bits_pow<NB>(pw, e), -(mie_sel<NT>(m.B, tj) * mie_bits_pow<NB>(pw, e >> 8)));
         // a select, not a product: a padding entry may sit at r = 0 (an infinite or NaN acc)
-        const double g = (r2 < rc2 && q < left) ? acc : 0.0;
+        const bool in = r2 < mie_sel<NT>(m.rc2, tj) && q < left;
+        const double g = in ? acc : 0.0;
         fx = fmaf64(g, rx, fx);
         fy = fmaf64(g, ry, fy);
         fz = fmaf64(g, rz, fz);
+        if constexpr (REC) {
+            vt = fmaf64(g, r2, vt);
+            const double u = mie_sel<NT>(m.ce, tj) *
+                                 fmaf64(-mie_sel<NT>(m.al, tj), mie_bits_pow<NB>(pw, e >> 24), mie_bits_pow<NB>(pw, e >> 16)) -
+                             mie_sel<NT>(m.sh, tj);
+            pe = pe + (in ? u : 0.0);
+        }
     }
 }
 
 // The list loop of lj_plain (two words in flight, the word after them loading) over mie_word.
-template <int NT, int NB>
+template <int NT, int NB, bool REC>
 __device__ __forceinline__ void mie_plain(const double* pos, const uint8_t* lt, const uint64_t* row, int n,
-                                          size_t cap, double xi, double yi, double zi, const MieRow<NT, NB>& m,
-                                          double& fx, double& fy, double& fz) {
-    double sx = 0, sy = 0, sz = 0;
+                                          size_t cap, double xi, double yi, double zi, const MieRow<NT, REC>& m,
+                                          double& fx, double& fy, double& fz, double& pe, double& vt) {
+    double sx = 0, sy = 0, sz = 0, spe = 0, svt = 0;
     P4M pa, pb;
     const int lastw = n > 0 ? (n - 1) >> 2 : 0;
     const uint64_t w0 = row[0];
@@ -1450,14 +1467,17 @@ __device__ __forceinline__ void mie_plain(const double* pos, const uint8_t* lt, 
         w1 = row[(size_t)min(k + 3, lastw) * cap];
         w2 = row[(size_t)min(k + 4, lastw) * cap];
         __builtin_amdgcn_sched_barrier(0);
-        mie_word<NT, NB>(pa, n - kw, xi, yi, zi, m, sx, sy, sz);
+        mie_word<NT, NB, REC>(pa, n - kw, xi, yi, zi, m, sx, sy, sz, spe, svt);
         mie_stage(pa, pos, lt, kw + 8 < n ? wk2 : 0);
         __builtin_amdgcn_sched_barrier(0);
-        if (__builtin_amdgcn_ballot_w64(kw + 4 < n) != 0ull) mie_word<NT, NB>(pb, n - kw - 4, xi, yi, zi, m, sx, sy, sz);
+        if (__builtin_amdgcn_ballot_w64(kw + 4 < n) != 0ull)
+            mie_word<NT, NB, REC>(pb, n - kw - 4, xi, yi, zi, m, sx, sy, sz, spe, svt);
     }
     fx = sx;
     fy = sy;
     fz = sz;
+    pe = spe;
+    vt = svt;
 }
 
 // LDS of the branch-free Mie launch: positions, region cell and one-byte type of each staged
@@ -1491,9 +1511,10 @@ __host__ __device__ inline size_t mie_lds_bytes(const BG& g, int maxreg) {
     return (b + 15) & ~(size_t)15;
 }
 
-// Forces of a Mie step without tallies (bonds and tethers too when EXTRA, added to the pair sum
-// after it); the kick of a fused simulation_move_particles_vv(2) as forces_body does it.
-template <int NT, int NB, bool EXTRA>
+// Forces of a Mie step without tallies (REC: a record step tallying PE and virial only), bonds and
+// tethers added to the pair sum after it; the kick of a fused simulation_move_particles_vv(2) as
+// forces_body does it, or with P1 pass 2 of this step and pass 1 of the next (vv_pass2_pass1).
+template <int NT, int NB, bool EXTRA, bool REC, bool P1>
 __device__ __forceinline__ void forces_body_mie(const FbArgs& a, int brick, char* smem) {
     Region r;
     carve_mie(r, smem, a.g, a.maxreg);
@@ -1503,17 +1524,19 @@ __device__ __forceinline__ void forces_body_mie(const FbArgs& a, int brick, char
     const int T2 = FM_MAX_TYPES * FM_MAX_TYPES;
     if constexpr (EXTRA) {
         // bonds and tethers in a pass of their own (each thread its own molecules), their force
-        // parked in a: what they read is then not held through the pair loops
+        // (and bond virial) parked in this launch's output: what they read is then not held
+        // through the pair loops
         for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
             int rci, li, i;
             imol(r, a.start, q, rci, li, i);
             const double inv_mi = 1.0 / a.mie[7 * T2 + a.moltype[i]];
             double bx = 0.0, by = 0.0, bz = 0.0, vir = 0.0;
-            bond_tether_forces<false, false>(a, i, r.pos[3 * li], r.pos[3 * li + 1], r.pos[3 * li + 2], inv_mi, false,
-                                             nullptr, bx, by, bz, vir);
-            a.s.ax[i] = bx;
-            a.s.ay[i] = by;
-            a.s.az[i] = bz;
+            bond_tether_forces<REC, REC>(a, i, r.pos[3 * li], r.pos[3 * li + 1], r.pos[3 * li + 2], inv_mi, false,
+                                         nullptr, bx, by, bz, vir);
+            (P1 ? a.s2.ax : a.s.ax)[i] = bx;  // where this launch leaves the force
+            (P1 ? a.s2.ay : a.s.ay)[i] = by;
+            (P1 ? a.s2.az : a.s.az)[i] = bz;
+            if constexpr (REC) a.vir[i] = vir;
         }
     }
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
@@ -1521,26 +1544,42 @@ __device__ __forceinline__ void forces_body_mie(const FbArgs& a, int brick, char
         imol(r, a.start, q, rci, li, i);
         const double xi = r.pos[3 * li], yi = r.pos[3 * li + 1], zi = r.pos[3 * li + 2];
         const int ti = a.moltype[i];
-        MieRow<NT, NB> m;
-        mie_row<NT, NB>(a, ti, m);
+        MieRow<NT, REC> m;
+        mie_row<NT, REC>(a, ti, m);
         const double inv_mi = 1.0 / a.mie[7 * T2 + ti];
-        double fx, fy, fz;
-        mie_plain<NT, NB>(r.pos, r.ltype8, a.nbl + i, a.nbn[i], a.cap, xi, yi, zi, m, fx, fy, fz);
+        double fx, fy, fz, pe, vir;
+        mie_plain<NT, NB, REC>(r.pos, r.ltype8, a.nbl + i, a.nbn[i], a.cap, xi, yi, zi, m, fx, fy, fz, pe, vir);
+        // the arguments below are read again from the kernel's argument segment on each pass (the
+        // launch's only argument, at its start; the empty asm hides that the pointer is the same),
+        // so that they are not held in scalar registers through the pair loop
+        const __attribute__((address_space(4))) char* kp =
+            (const __attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr();
+        asm volatile("" : "+s"(kp));
+        const FbArgs& b = *(const FbArgs*)kp;
         fx = fx * inv_mi;
         fy = fy * inv_mi;
         fz = fz * inv_mi;
         if constexpr (EXTRA) {
-            fx = fx + a.s.ax[i];
-            fy = fy + a.s.ay[i];
-            fz = fz + a.s.az[i];
+            fx = fx + (P1 ? b.s2.ax : b.s.ax)[i];
+            fy = fy + (P1 ? b.s2.ay : b.s.ay)[i];
+            fz = fz + (P1 ? b.s2.az : b.s.az)[i];
+            if constexpr (REC) vir = vir + b.vir[i];
         }
-        a.s.ax[i] = fx;
-        a.s.ay[i] = fy;
-        a.s.az[i] = fz;
-        if (a.kick) {
-            a.s.vx[i] = a.s.vx[i] + a.hdt * fx;
-            a.s.vy[i] = a.s.vy[i] + a.hdt * fy;
-            a.s.vz[i] = a.s.vz[i] + a.hdt * fz;
+        if constexpr (REC) {
+            b.vir[i] = vir;
+            b.pe_lj[i] = pe;
+        }
+        if constexpr (P1) {
+            vv_pass2_pass1<REC>(b, i, fx, fy, fz);
+            continue;
+        }
+        b.s.ax[i] = fx;
+        b.s.ay[i] = fy;
+        b.s.az[i] = fz;
+        if (b.kick) {
+            b.s.vx[i] = b.s.vx[i] + b.hdt * fx;
+            b.s.vy[i] = b.s.vy[i] + b.hdt * fy;
+            b.s.vz[i] = b.s.vz[i] + b.hdt * fz;
         }
     }
 }
@@ -1997,40 +2036,7 @@ __device__ __forceinline__ void forces_body32(const FbArgs& a, int brick, char* 
             a.pe_lj[i] = pe;
         }
         if constexpr (P1) {
-            const double ux = a.s.vx[i] + a.hdt * fx, uy = a.s.vy[i] + a.hdt * fy, uz = a.s.vz[i] + a.hdt * fz;
-            a.s2.x[i] = a.s.x[i] + a.dt * ux + a.hdt2 * fx;
-            a.s2.y[i] = a.s.y[i] + a.dt * uy + a.hdt2 * fy;
-            a.s2.z[i] = a.s.z[i] + a.dt * uz + a.hdt2 * fz;
-            const double wx = ux + a.hdt * fx, wy = uy + a.hdt * fy, wz = uz + a.hdt * fz;
-            a.s2.vx[i] = wx;
-            a.s2.vy[i] = wy;
-            a.s2.vz[i] = wz;
-            a.s2.ax[i] = fx;
-            a.s2.ay[i] = fy;
-            a.s2.az[i] = fz;
-            const size_t cp = a.cap;
-            double d0 = a.fresh ? 0.0 : a.rd[i], d1 = a.fresh ? 0.0 : a.rd[i + cp],
-                   d2 = a.fresh ? 0.0 : a.rd[i + 2 * cp];
-            d0 = d0 + wx * a.dt;
-            d1 = d1 + wy * a.dt;
-            d2 = d2 + wz * a.dt;
-            a.rd[i] = d0;
-            a.rd[i + cp] = d1;
-            a.rd[i + 2 * cp] = d2;
-            const bool hit = fabs(d0) > a.thresh || fabs(d1) > a.thresh || fabs(d2) > a.thresh;
-            const unsigned long long m = __ballot(hit);
-            if (m != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1 &&
-                __atomic_load_n(&a.flags[FLAG_REBUILD], __ATOMIC_RELAXED) == 0) {
-                atomicOr(&a.flags[FLAG_REBUILD], 1);
-                atomicExch(&a.flags[FLAG_RB_STEP], a.p1step);
-            }
-            if constexpr (REC) {
-                // v at the end of this step stays in this buffer for the record-step sums (no
-                // other thread of this launch reads this molecule's velocity)
-                a.s.vx[i] = ux;
-                a.s.vy[i] = uy;
-                a.s.vz[i] = uz;
-            }
+            vv_pass2_pass1<REC>(a, i, fx, fy, fz);
             continue;
         }
         a.s.ax[i] = fx;
@@ -2105,14 +2111,17 @@ k_forces_brick_plain(FbArgs a) {
     forces_body<false, REC, false, REC, P1>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
-// Mie steps without tallies (forces_body_mie): registers held to what lets two workgroups share a
-// CU (the LDS of two regions with one-byte types fits its 160 KB)
-template <int NT, int NB, bool EXTRA>
-__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLOCK / 128, FB_BLOCK / 128))) void
+// Mie steps without tallies and record steps of PE and virial (forces_body_mie): the steps without
+// tallies of up to two types held to the registers that let two workgroups share a CU (the LDS of
+// two regions with one-byte types fits its 160 KB); record steps (one in TPLOT) and four types
+// (the row of four partner types) without a cap
+template <int NT, int NB, bool EXTRA, bool REC, bool P1>
+__global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(REC || NT > 2 ? 1 : FB_BLOCK / 128,
+                                                                        REC || NT > 2 ? 8 : FB_BLOCK / 128))) void
 k_forces_brick_mie(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt, a.step)) return;
-    forces_body_mie<NT, NB, EXTRA>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+    forces_body_mie<NT, NB, EXTRA, REC, P1>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // A domain's step in a packed launch does nothing when the domain has no step in this launch, or
@@ -2618,6 +2627,28 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     return 0;
 }
 
+// the branch-free Mie launch with nt partner types and nb exponent bits (mie_plain_types)
+static void launch_forces_mie(const fm_ctx* c, const FbArgs& a, int nt, int nb, bool rec, bool p1) {
+    const size_t lm = brick_lds_bytes(c, c->maxreg, 3);
+    const dim3 grid(brick_count(c)), blk(FB_BLOCK);
+    const bool extra = a.bondn != nullptr || a.tag != nullptr;
+#define FMIE_E(NT, NB, E)                                                                                \
+    (rec ? (p1 ? k_forces_brick_mie<NT, NB, E, true, true><<<grid, blk, lm, c->stream>>>(a)             \
+               : k_forces_brick_mie<NT, NB, E, true, false><<<grid, blk, lm, c->stream>>>(a))           \
+         : (p1 ? k_forces_brick_mie<NT, NB, E, false, true><<<grid, blk, lm, c->stream>>>(a)            \
+               : k_forces_brick_mie<NT, NB, E, false, false><<<grid, blk, lm, c->stream>>>(a)))
+#define FMIE(NT, NB) (extra ? FMIE_E(NT, NB, true) : FMIE_E(NT, NB, false))
+    if (nt == 2) {
+        if (nb == 4) FMIE(2, 4);
+        else FMIE(2, 5);
+    } else {
+        if (nb == 4) FMIE(4, 4);
+        else FMIE(4, 5);
+    }
+#undef FMIE
+#undef FMIE_E
+}
+
 int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fused) {
     if (fused) *fused = false;
     if (c->np == 0) return 0;
@@ -2630,7 +2661,12 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
     // a record step tallying PE and virial only takes the branch-free loop too (FP64, and the
     // FP32-staged one in mixed precision)
     const bool simple_rec = (kind == 0 || kind == 1) && a.rec && a.pxy == nullptr && a.cv == nullptr && a.rf == nullptr;
-    if (next >= 0 && (((kind == 0 || kind == 1) && !a.rec && !a.ev) || simple_rec) && kick && fused) {
+    // the branch-free Mie launch (FP64): steps without tallies, and record steps of PE and virial
+    int mnb = 0;
+    const int mnt = (mie && !mixed) ? mie_plain_types(c, &mnb) : 0;
+    const bool mie_rec = mnt > 0 && a.rec && a.pxy == nullptr && a.cv == nullptr && a.rf == nullptr;
+    const bool mie_fast = mnt > 0 && ((!a.rec && !a.ev) || mie_rec);
+    if (next >= 0 && ((((kind == 0 || kind == 1) && !a.rec && !a.ev) || simple_rec) || mie_fast) && kick && fused) {
         // this step's force and kick, then pass 1 of step `next` into the other state buffer
         a.s2 = state_of(c->st[1 - c->scur], c->cap);
         a.rd = c->rdisp;
@@ -2639,7 +2675,8 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         a.fresh = c->rdisp_valid ? 0 : 1;
         a.p1step = next;
         ktimer_begin(c, KT_FORCES_P1);
-        if (simple_rec && mixed) k_forces_brick_plain32<true, true><<<brick_count(c), FB_BLOCK, lds32(c), c->stream>>>(a);
+        if (mie_fast) launch_forces_mie(c, a, mnt, mnb, mie_rec, true);
+        else if (simple_rec && mixed) k_forces_brick_plain32<true, true><<<brick_count(c), FB_BLOCK, lds32(c), c->stream>>>(a);
         else if (simple_rec) k_forces_brick_plain<true, true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
         else if (mixed) k_forces_brick_plain32<true><<<brick_count(c), FB_BLOCK, lds32(c), c->stream>>>(a);
         else k_forces_brick_plain<true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
@@ -2677,17 +2714,8 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
         else if (a.ev) FM(false, false);
         else k_forces_brick_plain32<false><<<grid, blk, lds32(c), c->stream>>>(a);
     } else if (mie) {
-        int nb = 0;
-        const int nt = (!a.rec && !a.ev) ? mie_plain_types(c, &nb) : 0;
-        if (nt > 0) {  // a step without tallies: the branch-free loop
-            const size_t lm = brick_lds_bytes(c, c->maxreg, 3);
-#define FMIE(NT, NB)                                                               \
-    (extra ? k_forces_brick_mie<NT, NB, true><<<grid, blk, lm, c->stream>>>(a) \
-           : k_forces_brick_mie<NT, NB, false><<<grid, blk, lm, c->stream>>>(a))
-            if (nt == 2) { if (nb == 4) FMIE(2, 4); else FMIE(2, 5); }
-            else { if (nb == 4) FMIE(4, 4); else FMIE(4, 5); }
-#undef FMIE
-        } else if (extra) { if (a.rec) FB(true, true, true); else FB(true, false, true); }
+        if (mie_fast) launch_forces_mie(c, a, mnt, mnb, mie_rec, false);
+        else if (extra) { if (a.rec) FB(true, true, true); else FB(true, false, true); }
         else { if (a.rec) FB(true, true, false); else FB(true, false, false); }
     } else {
         if (extra) { if (a.rec) FB(false, true, true); else FB(false, false, true); }

@@ -613,3 +613,40 @@ def test_mie_plain_step_forces(case):
     assert ro == rg
     assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-9
     g.close()
+
+
+def test_polymer_fused_loop_equals_routine_sequence(polymer_state):
+    """The queued loop's Mie force launches with bonds (the next step's pass 1 carried by the
+    force launch, record steps of PE and virial every 10 steps) repeat the step-by-step sequence
+    bit for bit: state, rebuild steps and record sums; the record sums also match the oracle."""
+    p, r, v, a, mt, mono = polymer_state
+    say("case: polymer fused loop vs routine sequence, 1152 beads, 60 steps")
+    g = MDContext(p, r, v, a)
+    f = MDContext(p, r, v, a)
+    o = Oracle(p, r, v, a)
+    for x in (g, f, o):
+        x.set_moltype(mt)
+        x.set_monomers(*mono)
+    g.setup(0)
+    f.setup(0)
+    o.setup()
+    o.compute_forces(0)
+    mg, rg = [], []
+    for it in range(1, 61):
+        if g.md_advance_vv(it):
+            rg.append(it)
+        if it % 10 == 0:
+            mg.append(g.evaluate_macroscopic_properties())
+    mf, nrb = f.run_vv(1, 60, n_macros=6)
+    mo, _ = o.run_vv(1, 60, n_macros=6)
+    assert nrb == len(rg) and len(rg) > 2
+    assert len(mf) == len(mg) == len(mo) == 6
+    for x, y in zip(mf, mg):
+        assert x == y
+    for x, y in zip(mf, mo):
+        for k in ("potenergysum_LJ", "potenergysum_POLY", "virial", "kinenergy"):
+            assert abs(x[k] - y[k]) <= 1e-9 * max(1.0, abs(y[k])), k
+    for x, y in zip(f.state(), g.state()):
+        assert np.array_equal(x, y)
+    f.close()
+    g.close()
