@@ -1076,6 +1076,17 @@ __device__ __forceinline__ void lj_plain(const double* pos, const uint64_t* row,
     }
 }
 
+// The launch's argument block read again from the kernel's argument segment (the launch's only
+// argument, at its start): an empty asm hides that the pointer is the same on each call, so the
+// fields read through it after a pair loop are loaded there and not held in scalar registers
+// through the loop.  Only for kernels whose one argument is an FbArgs.
+__device__ __forceinline__ const FbArgs& fargs_again() {
+    const __attribute__((address_space(4))) char* kp =
+        (const __attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(kp));
+    return *(const FbArgs*)kp;
+}
+
 // simulation_move_particles_vv(2) of this step, then (1) of the next step and the rdisp update of
 // rebuild criterion 2 (the arithmetic of k_pass2 and k_pass1_disp), the state after them written
 // to s2; the first lane of the wavefront that sees a displacement beyond the threshold raises the
@@ -1312,10 +1323,11 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         }
         }
         if constexpr (EXTRA) bond_tether_forces<RECORD, VIR>(a, i, xi, yi, zi, inv_mi, do_rf, rf, fx, fy, fz, vir);
-        if constexpr (P1) {
-            vv_pass2_pass1<RECORD>(a, i, fx, fy, fz);
-            if (VIR) a.vir[i] = vir;
-            if (RECORD) a.pe_lj[i] = pe;
+        if constexpr (P1) {  // (k_forces_brick_plain only: its one argument is the FbArgs)
+            const FbArgs& b = fargs_again();
+            vv_pass2_pass1<RECORD>(b, i, fx, fy, fz);
+            if (VIR) b.vir[i] = vir;
+            if (RECORD) b.pe_lj[i] = pe;
             continue;
         }
         a.s.ax[i] = fx;
@@ -1549,13 +1561,7 @@ __device__ __forceinline__ void forces_body_mie(const FbArgs& a, int brick, char
         const double inv_mi = 1.0 / a.mie[7 * T2 + ti];
         double fx, fy, fz, pe, vir;
         mie_plain<NT, NB, REC>(r.pos, r.ltype8, a.nbl + i, a.nbn[i], a.cap, xi, yi, zi, m, fx, fy, fz, pe, vir);
-        // the arguments below are read again from the kernel's argument segment on each pass (the
-        // launch's only argument, at its start; the empty asm hides that the pointer is the same),
-        // so that they are not held in scalar registers through the pair loop
-        const __attribute__((address_space(4))) char* kp =
-            (const __attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr();
-        asm volatile("" : "+s"(kp));
-        const FbArgs& b = *(const FbArgs*)kp;
+        const FbArgs& b = fargs_again();  // the arguments below: not held through the pair loop
         fx = fx * inv_mi;
         fy = fy * inv_mi;
         fz = fz * inv_mi;
@@ -2035,8 +2041,8 @@ __device__ __forceinline__ void forces_body32(const FbArgs& a, int brick, char* 
             a.vir[i] = vir;
             a.pe_lj[i] = pe;
         }
-        if constexpr (P1) {
-            vv_pass2_pass1<REC>(a, i, fx, fy, fz);
+        if constexpr (P1) {  // (k_forces_brick_plain32 only: its one argument is the FbArgs)
+            vv_pass2_pass1<REC>(fargs_again(), i, fx, fy, fz);
             continue;
         }
         a.s.ax[i] = fx;
