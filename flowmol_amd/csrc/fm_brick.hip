@@ -1363,15 +1363,17 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
 // (record steps tallying PE and virial only): x from two Newton steps (the IEEE quotient, as the
 // general loop), the PE term C eps (x^(lr/2) - alpha x^(la/2)) - potshift of the pair and the
 // virial magnitude times r^2, both zero for an entry that does not count.  Plain steps take x
-// from one Newton step.
+// from one Newton step.  NB = 0: the tables' usual class, la = 6 for every pair (x^4) and
+// lr = 12..18 (x^7..x^10), where the powers are x^4 and x^4 x^3 x^(nr-7) (and x^3, x^6 x^(nr-7)
+// for the PE): a few products instead of the bit expansions (within a few ulp of them).
 template <int NT, int REC>
 struct MieRow {
     double rc2[NT], A[NT], B[NT];
-    int e[NT];  // lr/2 + 1 | (la/2 + 1) << 8 | (lr/2) << 16 | (la/2) << 24
+    int e[NT];  // lr/2 + 1 | (la/2 + 1) << 8 | (lr/2) << 16 | (la/2) << 24 (NB = 0: lr/2 - 6)
     double ce[REC ? NT : 1], al[REC ? NT : 1], sh[REC ? NT : 1];  // C eps, alpha, potshift
 };
 
-template <int NT, int REC>
+template <int NT, int NB, int REC>
 __device__ __forceinline__ void mie_row(const FbArgs& a, int ti, MieRow<NT, REC>& m) {
     const int T2 = FM_MAX_TYPES * FM_MAX_TYPES;
 #pragma unroll
@@ -1383,7 +1385,8 @@ __device__ __forceinline__ void mie_row(const FbArgs& a, int ti, MieRow<NT, REC>
         m.A[t] = (C * eps) * lr;
         m.B[t] = (C * eps) * (al * la);
         const int nr = (int)(0.5 * lr + 1.0), na = (int)(0.5 * la + 1.0);
-        m.e[t] = nr | na << 8 | (nr - 1) << 16 | (na - 1) << 24;
+        if constexpr (NB == 0) m.e[t] = nr - 7;  // (the host checks 7 <= nr <= 10 and na = 4)
+        else m.e[t] = nr | na << 8 | (nr - 1) << 16 | (na - 1) << 24;
         if constexpr (REC) {
             m.ce[t] = C * eps;
             m.al[t] = al;
@@ -1438,12 +1441,31 @@ __device__ __forceinline__ void mie_word(const P4M& p, int left, double xi, doub
         const double r2 = (rx * rx + ry * ry) + rz * rz;
         const int tj = p.t[q];
         const int e = mie_sel<NT>(m.e, tj);
-        double pw[NB];
-        pw[0] = REC ? recip(r2) : recip1(r2);
+        const double x = REC ? recip(r2) : recip1(r2);
+        // x^nr, x^na, and on record steps x^(nr-1), x^(na-1)
+        double pr, pa, ur = 0.0, ua = 0.0;
+        if constexpr (NB == 0) {
+            const double x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
+            const double xd = ((e & 1) ? x : 1.0) * ((e & 2) ? x2 : 1.0);  // x^(nr-7)
+            pr = (x4 * x3) * xd;
+            pa = x4;
+            if constexpr (REC) {
+                ur = (x3 * x3) * xd;
+                ua = x3;
+            }
+        } else {
+            double pw[NB > 0 ? NB : 1];
+            pw[0] = x;
 #pragma unroll
-        for (int k = 1; k < NB; ++k) pw[k] = pw[k - 1] * pw[k - 1];
-        const double acc =
-            fmaf64(mie_sel<NT>(m.A, tj), mie_bits_pow<NB>(pw, e), -(mie_sel<NT>(m.B, tj) * mie_bits_pow<NB>(pw, e >> 8)));
+            for (int k = 1; k < NB; ++k) pw[k] = pw[k - 1] * pw[k - 1];
+            pr = mie_bits_pow<NB>(pw, e);
+            pa = mie_bits_pow<NB>(pw, e >> 8);
+            if constexpr (REC) {
+                ur = mie_bits_pow<NB>(pw, e >> 16);
+                ua = mie_bits_pow<NB>(pw, e >> 24);
+            }
+        }
+        const double acc = fmaf64(mie_sel<NT>(m.A, tj), pr, -(mie_sel<NT>(m.B, tj) * pa));
         // a select, not a product: a padding entry may sit at r = 0 (an infinite or NaN acc)
         const bool in = r2 < mie_sel<NT>(m.rc2, tj) && q < left;
         const double g = in ? acc : 0.0;
@@ -1452,9 +1474,7 @@ __device__ __forceinline__ void mie_word(const P4M& p, int left, double xi, doub
         fz = fmaf64(g, rz, fz);
         if constexpr (REC) {
             vt = fmaf64(g, r2, vt);
-            const double u = mie_sel<NT>(m.ce, tj) *
-                                 fmaf64(-mie_sel<NT>(m.al, tj), mie_bits_pow<NB>(pw, e >> 24), mie_bits_pow<NB>(pw, e >> 16)) -
-                             mie_sel<NT>(m.sh, tj);
+            const double u = mie_sel<NT>(m.ce, tj) * fmaf64(-mie_sel<NT>(m.al, tj), ua, ur) - mie_sel<NT>(m.sh, tj);
             pe = pe + (in ? u : 0.0);
         }
     }
@@ -1557,7 +1577,7 @@ __device__ __forceinline__ void forces_body_mie(const FbArgs& a, int brick, char
         const double xi = r.pos[3 * li], yi = r.pos[3 * li + 1], zi = r.pos[3 * li + 2];
         const int ti = a.moltype[i];
         MieRow<NT, REC> m;
-        mie_row<NT, REC>(a, ti, m);
+        mie_row<NT, NB, REC>(a, ti, m);
         const double inv_mi = 1.0 / a.mie[7 * T2 + ti];
         double fx, fy, fz, pe, vir;
         mie_plain<NT, NB, REC>(r.pos, r.ltype8, a.nbl + i, a.nbn[i], a.cap, xi, yi, zi, m, fx, fy, fz, pe, vir);
@@ -2394,13 +2414,14 @@ int brick_count(const fm_ctx* c) {
     return g.nb[0] * g.nb[1] * g.nb[2];
 }
 
-// Partner types of the branch-free Mie launch (2 or 4; bits of the exponents in *nb), or 0 when
-// some type pair's exponent lr/2 + 1 or la/2 + 1 is not a whole number below 32 (the general loop
-// then runs)
+// Partner types of the branch-free Mie launch (2 or 4; bits of the exponents in *nb, 0 for the
+// class la = 6, lr = 12..18), or 0 when some type pair's exponent lr/2 + 1 or la/2 + 1 is not a
+// whole number below 32 (the general loop then runs)
 static int mie_plain_types(const fm_ctx* c, int* nb) {
     const fm_params& p = c->p;
     if (p.mie_potential != 1) return 0;
     int emax = 0;
+    bool la6 = true;  // la = 6 for every pair and lr = 12, 14, 16 or 18 (nb = 0)
     for (int ti = 0; ti < p.ntypes; ++ti)
         for (int tj = 0; tj < p.ntypes; ++tj) {
             const int k = ti * FM_MAX_TYPES + tj;
@@ -2409,8 +2430,10 @@ static int mie_plain_types(const fm_ctx* c, int* nb) {
                 if (!(e >= 0.0 && e < 32.0) || e != floor(e)) return 0;
                 emax = std::max(emax, (int)e);
             }
+            const double lr = p.lambdar_lookup[k];
+            la6 = la6 && p.lambdaa_lookup[k] == 6.0 && (lr == 12.0 || lr == 14.0 || lr == 16.0 || lr == 18.0);
         }
-    *nb = emax < 16 ? 4 : 5;
+    *nb = la6 ? 0 : (emax < 16 ? 4 : 5);
     return p.ntypes <= 2 ? 2 : 4;
 }
 
@@ -2645,10 +2668,12 @@ static void launch_forces_mie(const fm_ctx* c, const FbArgs& a, int nt, int nb, 
                : k_forces_brick_mie<NT, NB, E, false, false><<<grid, blk, lm, c->stream>>>(a)))
 #define FMIE(NT, NB) (extra ? FMIE_E(NT, NB, true) : FMIE_E(NT, NB, false))
     if (nt == 2) {
-        if (nb == 4) FMIE(2, 4);
+        if (nb == 0) FMIE(2, 0);
+        else if (nb == 4) FMIE(2, 4);
         else FMIE(2, 5);
     } else {
-        if (nb == 4) FMIE(4, 4);
+        if (nb == 0) FMIE(4, 0);
+        else if (nb == 4) FMIE(4, 4);
         else FMIE(4, 5);
     }
 #undef FMIE

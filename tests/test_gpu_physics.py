@@ -578,25 +578,46 @@ def test_polymer_random_walk_start():
     g.close()
 
 
-@pytest.mark.parametrize("case", ["whole_exponents", "half_exponent"])
+MIE_CASES = {
+    # four types, exponents up to x^18 (five bits)
+    "lr_to_34": {(0, 0): dict(lr=12.0, la=6.0), (0, 1): dict(lr=14.0, la=6.0, eps=0.8, rc=2.0),
+                 (0, 2): dict(lr=20.0, la=6.0, rc=1.8), (0, 3): dict(lr=12.0, la=6.0, rc=2 ** (1 / 6)),
+                 (1, 1): dict(lr=34.0, la=8.0, eps=1.2, alpha=0.9), (1, 2): dict(lr=16.0, la=6.0, rc=2.2),
+                 (1, 3): dict(lr=12.0, la=6.0, rc=1.5), (2, 2): dict(lr=14.0, la=6.0),
+                 (2, 3): dict(lr=12.0, la=6.0, eps=0.5), (3, 3): dict(lr=24.0, la=12.0)},
+    # four types, la = 6 and lr = 12..18 throughout (the products class)
+    "la6_lr12_to_18": {(0, 0): dict(lr=12.0, la=6.0), (0, 1): dict(lr=14.0, la=6.0, eps=0.8, rc=2.0),
+                       (0, 2): dict(lr=16.0, la=6.0, rc=1.8), (0, 3): dict(lr=18.0, la=6.0, rc=2 ** (1 / 6)),
+                       (1, 1): dict(lr=18.0, la=6.0, eps=1.2, alpha=0.9), (1, 2): dict(lr=12.0, la=6.0, rc=2.2),
+                       (1, 3): dict(lr=16.0, la=6.0, rc=1.5), (2, 2): dict(lr=14.0, la=6.0),
+                       (2, 3): dict(lr=12.0, la=6.0, eps=0.5), (3, 3): dict(lr=18.0, la=6.0)},
+    # two types, la = 8 (four bits)
+    "two_types_la8": {(0, 0): dict(lr=12.0, la=8.0), (0, 1): dict(lr=14.0, la=8.0, eps=0.8, rc=2.0),
+                      (1, 1): dict(lr=16.0, la=8.0, eps=1.2, alpha=0.9, rc=1.6)},
+    # one pair at lr = 13 (x^7.5): the general loop
+    "half_exponent": {(0, 0): dict(lr=12.0, la=6.0), (0, 1): dict(lr=14.0, la=6.0, eps=0.8, rc=2.0),
+                      (1, 1): dict(lr=13.0, la=8.0, eps=1.2, alpha=0.9), (2, 2): dict(lr=14.0, la=6.0),
+                      (0, 2): dict(lr=12.0, la=6.0), (1, 2): dict(lr=12.0, la=6.0), (3, 3): dict(lr=12.0, la=6.0),
+                      (0, 3): dict(lr=12.0, la=6.0), (1, 3): dict(lr=12.0, la=6.0), (2, 3): dict(lr=12.0, la=6.0)},
+}
+
+
+@pytest.mark.parametrize("case", sorted(MIE_CASES))
 def test_mie_plain_step_forces(case):
-    """Forces of a step without tallies against the oracle, then a 20-step drop-in run, for a
-    fluid of four Mie types with per-pair cutoffs (2^(1/6) to 2.5) and masses 0.5 to 2.  With
-    every exponent lr/2+1, la/2+1 whole (up to x^18: five bits) the step takes the branch-free
-    loop; with one pair at lr = 13 (x^7.5) the general loop."""
+    """Forces of a step without tallies against the oracle, then a 20-step drop-in run (record
+    steps of PE and virial every 10), for Mie mixtures with per-pair cutoffs (2^(1/6) to 2.5) and
+    masses 0.5 to 2.  With every exponent lr/2+1, la/2+1 whole the steps take the branch-free
+    loop: the products class (la = 6, lr = 12..18), four and five exponent bits, two and four
+    types; with one pair at lr = 13 (x^7.5) the general loop."""
     p, r, v = config.config_fcc((6, 6, 6), 0.8, 2.5, seed=71, tplot=10)
-    lr11 = 34.0 if case == "whole_exponents" else 13.0
-    config.set_mie_types(p, {(0, 0): dict(lr=12.0, la=6.0), (0, 1): dict(lr=14.0, la=6.0, eps=0.8, rc=2.0),
-                             (0, 2): dict(lr=20.0, la=6.0, rc=1.8), (0, 3): dict(lr=12.0, la=6.0, rc=2 ** (1 / 6)),
-                             (1, 1): dict(lr=lr11, la=8.0, eps=1.2, alpha=0.9), (1, 2): dict(lr=16.0, la=6.0, rc=2.2),
-                             (1, 3): dict(lr=12.0, la=6.0, rc=1.5), (2, 2): dict(lr=14.0, la=6.0),
-                             (2, 3): dict(lr=12.0, la=6.0, eps=0.5), (3, 3): dict(lr=24.0, la=12.0)})
+    config.set_mie_types(p, MIE_CASES[case])
+    ntypes = p.ntypes
     for t, m in enumerate((1.0, 2.0, 0.5, 1.5)):
         p.mass_lookup[t] = m
     rng = np.random.default_rng(71)
     r = r + rng.uniform(-0.05, 0.05, r.shape)
-    mt = rng.integers(0, 4, r.shape[0]).astype(np.int32)
-    say("case: Mie plain step, four types, %s, 864 molecules" % case)
+    mt = rng.integers(0, ntypes, r.shape[0]).astype(np.int32)
+    say("case: Mie plain step, %d types, %s, 864 molecules" % (ntypes, case))
     o = Oracle(p, r, v)
     o.set_moltype(mt)
     g = MDContext(p, r, v)
@@ -612,6 +633,9 @@ def test_mie_plain_step_forces(case):
     ro, rg = drop_in_steps(o, g, 1, 20)
     assert ro == rg
     assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-9
+    mo, mg = o.macros(), g.evaluate_macroscopic_properties()   # step 20: a record step
+    for k in ("potenergysum_LJ", "virial"):
+        assert abs(mg[k] - mo[k]) <= 1e-10 * max(1.0, abs(mo[k])), k
     g.close()
 
 
