@@ -688,6 +688,36 @@ k_nlist_quant(NlqArgs a) {
 #pragma unroll
                 for (int g4 = 0; g4 < 4; ++g4) bf[g4] = a.mono[slot + (size_t)(3 + g4) * cap];
             }
+            // (POLY) region indices of the waiting hits of the molecule's own chain, two per word
+            uint32_t pend[4] = {0u, 0u, 0u, 0u};
+            int npend = 0;
+            // the bond test of the waiting hits, four at a time: the chain and subchain ids of
+            // their slots loaded together, then the tests and bond entries in hit order
+            auto flush_bonds = [&]() {
+                for (int k0 = 0; k0 < npend; k0 += 4) {
+                    int sj[4], cj[4], sc[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int jj = (int)((pend[(k0 + k) >> 1] >> (16 * ((k0 + k) & 1))) & 0xffffu);
+                        sj[k] = k0 + k < npend ? ls[jj] : 0;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int sl = sj[k] >> 5;
+                        cj[k] = k0 + k < npend ? a.mono[sl] : ~chain;
+                        sc[k] = k0 + k < npend ? a.mono[sl + cap] : 0;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (cj[k] == chain && bondflag_of(bf, sc[k])) {
+                            if (nb < funcy && nb < FM_MAX_FUNCY) a.bond[(size_t)nb * cap + slot] = (uint32_t)sj[k];
+                            ++nb;
+                        }
+                }
+                npend = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) pend[k] = 0u;
+            };
             // faces of the molecule's cell in y and z, as in k_nlist_brick
             const int vy = by * g.B[1] + ry - 1, vz = bz * g.B[2] + rz - 1;
             const double ylo = -a.hd[1] + vy * a.cs[1], zlo = -a.hd[2] + vz * a.cs[2];
@@ -796,13 +826,14 @@ k_nlist_quant(NlqArgs a) {
                         ++cnt;
                     }
                     if (POLY) {
+                        // hits of the molecule's own chain wait in registers (up to eight) for
+                        // the bond test after the runs, where their id loads go out together; a
+                        // ninth first has the eight tested (hit order is kept)
                         for (uint32_t h = same; h; h &= h - 1) {
+                            if (npend == 8) flush_bonds();
                             const int j = j0 + __builtin_ctz(h);
-                            const int sj = ls[j] >> 5;  // the candidate's slot: its chain and subchain ids
-                            if (a.mono[sj] == chain && bondflag_of(bf, a.mono[sj + cap])) {
-                                if (nb < funcy && nb < FM_MAX_FUNCY) a.bond[(size_t)nb * cap + slot] = (uint32_t)ls[j];
-                                ++nb;
-                            }
+                            pend[npend >> 1] |= (uint32_t)j << (16 * (npend & 1));
+                            ++npend;
                         }
                     }
                 }
@@ -810,6 +841,7 @@ k_nlist_quant(NlqArgs a) {
             if ((cnt & 3) != 0 && cnt < a.maxnb) a.nbl[(size_t)(cnt >> 2) * cap + slot] = word >> (16 * (4 - (cnt & 3)));
             a.nbn[slot] = cnt < a.maxnb ? cnt : a.maxnb;
             if (POLY) {
+                flush_bonds();
                 a.bondn[slot] = nb < funcy ? nb : funcy;
                 if (nb > funcy) set_err(a.flags, FM_EBOND, a.ext[slot]);
             }
