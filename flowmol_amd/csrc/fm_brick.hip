@@ -1221,12 +1221,12 @@ __device__ __forceinline__ void bond_tether_forces(const FbArgs& a, int i, doubl
 // VIR: virial on a non-record step (teval); record steps always tally it.  P1: VV pass 2 of this
 // step, pass 1 of the next one and its displacement update follow the force (same arithmetic as
 // the separate kernels), written to the other state buffer.
-template <bool MIE, bool RECORD, bool EXTRA, bool VIR = RECORD, bool P1 = false>
+template <bool MIE, bool RECORD, bool EXTRA, bool VIR = RECORD, bool P1 = false, bool SIMPLE = false>
 __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* smem) {
     Region r;
-    const bool mop = RECORD && a.pxy != nullptr;
+    const bool mop = !SIMPLE && RECORD && a.pxy != nullptr;
     // the stress tallies need each staged molecule's slot (molecule numbers for the owner rule)
-    const bool cvrec = RECORD && a.cv != nullptr;
+    const bool cvrec = !SIMPLE && RECORD && a.cv != nullptr;
     carve(r, smem, a.g, a.maxreg, mop || MIE || cvrec, MIE);
     double* spxy = nullptr;
     if (mop) spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true, MIE));
@@ -1251,7 +1251,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
     }
     const size_t cap = a.cap;
     // a record step that tallies PE and virial only (no MOP, CV or rfmol)
-    const bool simple_rec = RECORD && !mop && !cvrec && a.rf == nullptr;
+    const bool simple_rec = RECORD && (SIMPLE || (!mop && !cvrec && a.rf == nullptr));
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
         int rci, li, i;
         imol(r, a.start, q, rci, li, i);
@@ -1259,7 +1259,7 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         const int n = a.nbn[i];
         double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, vir = 0.0;
         double rf[9];
-        const bool do_rf = RECORD && a.rf != nullptr;
+        const bool do_rf = !SIMPLE && RECORD && a.rf != nullptr;
         if (do_rf)
             for (int k = 0; k < 9; ++k) rf[k] = 0.0;
         int ti = 0;
@@ -2166,7 +2166,7 @@ __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(FB_BLO
 k_forces_brick_plain(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt, a.step)) return;
-    forces_body<false, REC, false, REC, P1>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+    forces_body<false, REC, false, REC, P1, true>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // Mie steps without tallies and record steps of PE and virial (forces_body_mie): the steps without
