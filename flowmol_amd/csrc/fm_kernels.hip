@@ -543,7 +543,8 @@ __global__ __launch_bounds__(FM_BLOCK) void k_bin_count(St s, const int* ext, in
 // start[0..n) = exclusive prefix of count[0..n) by one workgroup, in tiles of 14336 counts (60 KB
 // of LDS): a coalesced pass into LDS (one padding word per 14, so that the threads' runs of 14
 // consecutive counts start an odd number of words apart, in distinct banks), a scan by thread,
-// wavefront and workgroup, and a coalesced pass out.
+// wavefront and workgroup, and a coalesced pass out.  The next tile's counts are loaded into
+// registers while a tile is scanned.
 // The counts are zeroed as they are read (ready for the next binning) and so is cursor[0..ncur).
 #define SCAN_PER 14
 #define SCAN_TILE (1024 * SCAN_PER)
@@ -552,11 +553,19 @@ __global__ __launch_bounds__(1024) void k_scan_cells(int* count, int* start, int
     __shared__ int wsum[16];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     int carry = 0;
+    // indices past the end re-read the last count
+    int vn[SCAN_PER];
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) vn[k] = count[min(k * 1024 + tid, n - 1)];
     for (int base = 0; base < n; base += SCAN_TILE) {
         int v[SCAN_PER], s = 0;
-        // all loads first (indices past the end re-read the last count), then the stores
 #pragma unroll
-        for (int k = 0; k < SCAN_PER; ++k) v[k] = count[min(base + k * 1024 + tid, n - 1)];
+        for (int k = 0; k < SCAN_PER; ++k) v[k] = vn[k];
+        // the next tile's loads go out before this tile's stores (other indices)
+        if (base + SCAN_TILE < n) {
+#pragma unroll
+            for (int k = 0; k < SCAN_PER; ++k) vn[k] = count[min(base + SCAN_TILE + k * 1024 + tid, n - 1)];
+        }
 #pragma unroll
         for (int k = 0; k < SCAN_PER; ++k) {
             const int j = k * 1024 + tid, i = base + j;
@@ -608,23 +617,18 @@ __global__ __launch_bounds__(FM_BLOCK) void k_bin_scatter(const int* cell_of, co
 }
 
 // order each cell by increasing x (ties: molecule number), which lets the list build window
-// its candidate runs by x: one wavefront per cell, bitonic sort in registers; cells above 64
-// molecules fall back to an insertion sort by lane 0
+// its candidate runs by x.  A wavefront takes two cells: when both hold at most 32 molecules,
+// each half ranks its molecules (each lane counts the keys of its cell below its own, the other
+// lanes' keys coming through shuffles); otherwise the whole wavefront takes each cell in turn,
+// ranking up to 64 molecules with the keys through readlane, and cells above 64 molecules fall
+// back to an insertion sort by lane 0.  Keys are distinct (ties in x are broken by the molecule
+// number), so the ranks are a permutation.
 __device__ inline bool xkey_less(double xa, int ea, double xb, int eb) { return xa < xb || (xa == xb && ea < eb); }
 
-// one wavefront per cell
-__global__ __launch_bounds__(FM_BLOCK) void k_bin_sort(const int* start, int* perm, const int* ext, const double* x,
-                                                       int ncell, int maxcell, int* flags) {
-    const int lane = threadIdx.x & 63;
-    const int c = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
-    if (c >= ncell) return;  // whole wavefront
-    const int b = start[c], e = start[c + 1], n = e - b;
-    if (n > maxcell && lane == 0) set_err(flags, FM_EOVERFLOW, c);
+__device__ inline void bin_sort_cell(int b, int e, int* perm, const int* ext, const double* x) {
+    const int lane = threadIdx.x & 63, n = e - b;
     if (n <= 1) return;
     if (n <= 64) {
-        // rank sort: each lane counts the keys below its own (keys are distinct: ties in x are
-        // broken by the molecule number); the other lanes' keys come through readlane, so the
-        // loop has no LDS round trips
         const int v = lane < n ? perm[b + lane] : -1;
         const double kx = lane < n ? x[v] : 0.0;
         const int ke = lane < n ? ext[v] : 0;
@@ -651,6 +655,37 @@ __global__ __launch_bounds__(FM_BLOCK) void k_bin_sort(const int* start, int* pe
             perm[m + 1] = v;
         }
     }
+}
+
+// cells 2w and 2w + 1 by wavefront w
+__global__ __launch_bounds__(FM_BLOCK) void k_bin_sort(const int* start, int* perm, const int* ext, const double* x,
+                                                       int ncell, int maxcell, int* flags) {
+    const int lane = threadIdx.x & 63;
+    const int c0 = 2 * (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (c0 >= ncell) return;  // whole wavefront
+    const int hl = lane & 31, c = c0 + (lane >> 5);
+    const int b = c < ncell ? start[c] : 0, n = c < ncell ? start[c + 1] - b : 0;
+    if (n > maxcell && hl == 0) set_err(flags, FM_EOVERFLOW, c);
+    if (__builtin_amdgcn_ballot_w64(n > 32) == 0ull) {
+        const bool own = hl < n;
+        const int v = own ? perm[b + hl] : -1;
+        const double kx = own ? x[v] : 0.0;
+        const int ke = own ? ext[v] : 0;
+        const long long kb = __double_as_longlong(kx);
+        const int klo = (int)kb, khi = (int)(kb >> 32);
+        const int nmax = max(n, __shfl_xor(n, 32, 64));
+        int rank = 0;
+        for (int k = 0; k < nmax; ++k) {
+            const int src = (lane & 32) + k;
+            const int olo = __shfl(klo, src, 64), ohi = __shfl(khi, src, 64), oe = __shfl(ke, src, 64);
+            const double ox = __longlong_as_double(((long long)ohi << 32) | (long long)(unsigned)olo);
+            rank += (k < n && xkey_less(ox, oe, kx, ke)) ? 1 : 0;
+        }
+        if (own) perm[b + rank] = v;
+        return;
+    }
+    bin_sort_cell(start[c0], start[c0 + 1], perm, ext, x);
+    if (c0 + 1 < ncell) bin_sort_cell(start[c0 + 1], start[c0 + 2], perm, ext, x);
 }
 
 struct PermArgs {
@@ -1097,7 +1132,7 @@ int launch_bin(fm_ctx* c, bool keep_acc) {
                                                                c->np);
     LAUNCH_CHECK();
     int maxcell = c->p.max_cell_np > 0 ? c->p.max_cell_np : (1 << 26);
-    const int nsort = nblocks((size_t)c->ncell * 64);
+    const int nsort = nblocks((size_t)((c->ncell + 1) / 2) * 64);  // two cells per wavefront
     k_bin_sort<<<nsort, FM_BLOCK, 0, c->stream>>>(c->cell_start, c->perm, c->ext[c->cur], s.x, c->ncell, maxcell,
                                                   c->d_flags);
     LAUNCH_CHECK();
