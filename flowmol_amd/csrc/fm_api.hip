@@ -160,7 +160,7 @@ static void free_ctx(fm_ctx* c) {
                     c->cell_count, c->cell_start, c->cell_cursor, c->perm, c->nbl, c->nbn, c->bond, c->bondn,
                     c->pe_lj, c->pe_poly, c->vir, c->rf, c->pxy, c->cvface, c->planes, c->out_flag, c->out_vals,
                     c->out_slot, c->out_count, c->red, c->d_macros, c->d_ring, c->d_mie, c->d_vmax, c->d_rmax,
-                    c->d_flags, c->stage, c->brick_order, c->hcurve, c->sort_key,
+                    c->d_flags, c->stage, c->brick_order, c->btab, c->hcurve, c->sort_key,
                     c->sort_val, c->sort_key2, c->sort_val2, c->sort_cnt, c->sort_tmp};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -918,6 +918,28 @@ extern "C" int fm_sort_mols(fm_ctx* c) {
     return launch_sort_mols(c, &skipped);
 }
 
+// The bricks' staging tables (fm_ctx::btab) for the current region capacity: reallocated when the
+// capacity or the brick count grew (at a synchronisation point: no launch uses the old ones), and
+// invalid until the next list build either way.
+static int size_brick_tables(fm_ctx* c) {
+    c->btab_valid = false;
+    const int nbrick = brick_count(c);
+    if (c->btab && c->maxreg <= c->btab_reg && nbrick <= c->btab_nbrick) {
+        c->btab_stride = (2 + 2 * c->maxreg + (c->maxreg + 1) / 2 + 3) & ~3;
+        return 0;
+    }
+    if (c->btab) hipFree(c->btab);
+    c->btab = nullptr;
+    c->btab_reg = c->btab_nbrick = 0;
+    const int reg = c->maxreg + c->maxreg / 8;  // headroom for later growth
+    const int stride = (2 + 2 * reg + (reg + 1) / 2 + 3) & ~3;
+    TRY(dalloc(&c->btab, (size_t)stride * nbrick));
+    c->btab_reg = reg;
+    c->btab_nbrick = nbrick;
+    c->btab_stride = (2 + 2 * c->maxreg + (c->maxreg + 1) / 2 + 3) & ~3;
+    return 0;
+}
+
 // LDS of a brick kernel staging m molecules: the largest over the kernel kinds
 static size_t region_bytes(const fm_ctx* c, int m) {
     return std::max(brick_lds_bytes(c, m, 0), brick_lds_bytes(c, m, 2));
@@ -936,7 +958,7 @@ static int size_regions(fm_ctx* c, int measured) {
                             c->lds_limit);
     if (c->tight_regions) {
         c->maxreg = m0;
-        return 0;
+        return size_brick_tables(c);
     }
     const size_t wgs = std::max<size_t>(c->lds_cu / need, 1);
     const size_t budget = std::min(c->lds_limit, (c->lds_cu / wgs) & ~(size_t)1023);
@@ -961,7 +983,7 @@ static int size_regions(fm_ctx* c, int measured) {
         lo = l2;
     }
     c->maxreg = lo;
-    return 0;
+    return size_brick_tables(c);
 }
 
 extern "C" int fm_assign_to_cell(fm_ctx* c) {

@@ -307,6 +307,42 @@ __device__ bool stage_region(Region& r, const BG& g, int brick, const int* start
     return true;
 }
 
+// Staged positions from the brick's table, which the list build wrote since the last binning: the
+// slot and image shift of each staged molecule in the region layout (total and interior counts
+// first), then the slot and LDS index of each interior molecule.  No region tables are formed.
+// Ends with __syncthreads().
+__device__ bool stage_from_table(Region& r, const BG& g, const St& s, const int* T, int capm) {
+    r.total = T[0];
+    r.ni = T[1];
+    if (r.total > capm) return false;  // (a build that met a larger region halted its loop)
+    const int* stg = T + 2;
+    constexpr int SU = 6;
+    for (int q0 = threadIdx.x; q0 < r.total; q0 += SU * blockDim.x) {
+        int w[SU];
+        double px[SU], py[SU], pz[SU];
+#pragma unroll
+        for (int k = 0; k < SU; ++k) w[k] = stg[min(q0 + k * (int)blockDim.x, r.total - 1)];
+#pragma unroll
+        for (int k = 0; k < SU; ++k) {
+            const int sl = w[k] >> 5;
+            px[k] = s.x[sl];
+            py[k] = s.y[sl];
+            pz[k] = s.z[sl];
+        }
+#pragma unroll
+        for (int k = 0; k < SU; ++k) {
+            const int q = min(q0 + k * (int)blockDim.x, r.total - 1);
+            int sx, sy, sz;
+            decode_shift(w[k] & 31, sx, sy, sz);
+            r.pos[3 * q] = img(px[k], sx, g.L[0]);
+            r.pos[3 * q + 1] = img(py[k], sy, g.L[1]);
+            r.pos[3 * q + 2] = img(pz[k], sz, g.L[2]);
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
 // i-molecule q of the brick -> (region cell, LDS index, slot)
 __device__ inline void imol(const Region& r, const int* start, int q, int& rc, int& li, int& slot) {
     int lo = 0, hi = r.nic - 1;
@@ -587,6 +623,9 @@ struct NlqArgs {
     uint32_t* bond;
     int* bondn;
     const int* ext;
+    // optional: the bricks' staging tables for the plain force launches (stage_from_table)
+    int* btab;
+    int btab_stride;
 };
 
 __device__ inline int quantise(double x, double o, double inv_h) {
@@ -659,7 +698,12 @@ k_nlist_quant(NlqArgs a) {
             qv[q] = make_uint2((uint32_t)xq | ((uint32_t)yq << 16),
                                (uint32_t)zq | (POLY ? ((uint32_t)a.mono[sl[k]] & 0x1ffffu) << 15 : 0u));
             ls[q] = (sl[k] << 5) | sc[k];
+            if (a.btab) a.btab[(size_t)brick * a.btab_stride + 2 + q] = (sl[k] << 5) | sc[k];
         }
+    }
+    if (a.btab && threadIdx.x == 0) {
+        a.btab[(size_t)brick * a.btab_stride] = r.total;
+        a.btab[(size_t)brick * a.btab_stride + 1] = r.ni;
     }
     __syncthreads();
     int wmax = 0;
@@ -673,6 +717,11 @@ k_nlist_quant(NlqArgs a) {
         if (active) {
             int rc, li, slot;
             imol(r, a.start, qi, rc, li, slot);
+            if (a.btab) {
+                int* const T = a.btab + (size_t)brick * a.btab_stride + 2 + a.maxreg;
+                T[qi] = slot;
+                ((uint16_t*)(T + a.maxreg))[qi] = (uint16_t)li;
+            }
             const int rx = rc % R0, ry = (rc / R0) % R1, rz = rc / (R0 * R1);
             // interior cells carry no image shift: the staged position is the stored one
             const double xi = a.s.x[slot], yi = a.s.y[slot], zi = a.s.z[slot];
@@ -909,6 +958,10 @@ struct FbArgs {
     double* rd;
     double hdt2, thresh;
     int fresh, p1step;
+    // the bricks' staging tables written by the list build since the last binning, or null
+    // (stage_from_table; the LJ plain launches)
+    const int* btab;
+    int btab_stride;
 };
 
 // forward cell offsets of the reference's half-list scan (src/linklist.f90:686-688)
@@ -1241,9 +1294,12 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
         mtab = t;  // visible after the staging's barriers below
     }
     // the staged slot array carries molecule numbers here (the stress tallies' owner rule is its
-    // only reader)
-    if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags,
-                      MIE ? a.moltype : nullptr, a.ext))
+    // only reader); the LJ plain launches stage from the list build's table when there is one
+    const int* tab = SIMPLE && a.btab ? a.btab + (size_t)brick * a.btab_stride : nullptr;
+    if (tab) {
+        if (!stage_from_table(r, a.g, a.s, tab, a.maxreg)) return;
+    } else if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg,
+                             a.flags, MIE ? a.moltype : nullptr, a.ext))
         return;
     if (mop) {
         for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
@@ -1253,8 +1309,13 @@ __device__ __forceinline__ void forces_body(const FbArgs& a, int brick, char* sm
     // a record step that tallies PE and virial only (no MOP, CV or rfmol)
     const bool simple_rec = RECORD && (SIMPLE || (!mop && !cvrec && a.rf == nullptr));
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
-        int rci, li, i;
-        imol(r, a.start, q, rci, li, i);
+        int rci = 0, li, i;
+        if (tab) {
+            i = tab[2 + a.maxreg + q];
+            li = ((const uint16_t*)(tab + 2 + 2 * a.maxreg))[q];
+        } else {
+            imol(r, a.start, q, rci, li, i);
+        }
         const double xi = r.pos[3 * li], yi = r.pos[3 * li + 1], zi = r.pos[3 * li + 2];
         const int n = a.nbn[i];
         double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, vir = 0.0;
@@ -2541,6 +2602,9 @@ int launch_nlist(fm_ctx* c) {
         q.ra2 = a.ra2;
         q.xwin = a.xwin;
         q.order = a.order;
+        // the staging tables of the plain force launches, while the cells stay as binned
+        q.btab = c->btab;
+        q.btab_stride = c->btab_stride;
         double ext = 0.0;
         for (int d = 0; d < 3; ++d) {
             q.cs[d] = a.cs[d];
@@ -2562,8 +2626,10 @@ int launch_nlist(fm_ctx* c) {
         else k_nlist_quant<false><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(q);
         LAUNCH_CHECK();
         ktimer_end(c, KT_NLIST);
+        c->btab_valid = c->btab != nullptr;
         return 0;
     }
+    c->btab_valid = false;
     const size_t lds = brick_lds_bytes(c, c->maxreg, 0);
     ktimer_begin(c, KT_NLIST);
     if (a.mono) k_nlist_brick<true><<<brick_count(c), FB_BLOCK, lds, c->stream>>>(a);
@@ -2667,6 +2733,8 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     a.halt = nullptr;
     a.active = 1;
     a.band32 = band32_of(c);
+    a.btab = c->btab_valid ? c->btab : nullptr;
+    a.btab_stride = c->btab_stride;
     a.cv = (a.rec && c->cvface) ? c->cvface : nullptr;
     for (int d = 0; d < 3; ++d) {
         a.cvnbo[d] = c->cv_nbo[d];

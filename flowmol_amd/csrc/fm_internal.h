@@ -105,6 +105,12 @@ struct fm_ctx {
     size_t lds_limit = 65536;             // dynamic LDS per workgroup
     size_t lds_cu = 65536;                // LDS per compute unit
     bool tight_regions = false;           // capacity = measured largest region (no headroom)
+    // per brick, written by the quantised list build for the plain force launches: region total
+    // and interior count, each staged molecule's slot << 5 | image shift (maxreg), each interior
+    // molecule's slot (maxreg) and LDS index (maxreg 16-bit); valid until the next binning
+    int* btab = nullptr;
+    int btab_stride = 0, btab_reg = 0, btab_nbrick = 0;
+    bool btab_valid = false;
 
     // neighbour list: 16-bit indices into the brick's LDS region layout, four per 64-bit word:
     // entries 4c..4c+3 of slot i in nbl[c*cap + i]

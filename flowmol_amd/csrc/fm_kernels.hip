@@ -1116,6 +1116,7 @@ int launch_sendmols(fm_ctx* c, const int* reset, int nreset) {
 }
 
 int launch_bin(fm_ctx* c, bool keep_acc) {
+    c->btab_valid = false;  // new cells: the staging tables wait for the next list build
     if (c->np == 0) return 0;
     ktimer_begin(c, KT_BIN);
     St s = state_of(c->x(), c->cap);
