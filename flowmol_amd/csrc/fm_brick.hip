@@ -311,7 +311,10 @@ __device__ bool stage_region(Region& r, const BG& g, int brick, const int* start
 // slot and image shift of each staged molecule in the region layout (total and interior counts
 // first), then the slot and LDS index of each interior molecule.  No region tables are formed.
 // Ends with __syncthreads().
-__device__ bool stage_from_table(Region& r, const BG& g, const St& s, const int* T, int capm) {
+// (type_src: the one-byte types of the staged molecules into r.ltype8 too; key_src: the owner keys
+// of the MOP tally, key_src[slot] << 1 | (image of a periodic shift), into r.lslot)
+__device__ bool stage_from_table(Region& r, const BG& g, const St& s, const int* T, int capm,
+                                 const int* type_src = nullptr, const int* key_src = nullptr) {
     r.total = T[0];
     r.ni = T[1];
     if (r.total > capm) return false;  // (a build that met a larger region halted its loop)
@@ -337,6 +340,8 @@ __device__ bool stage_from_table(Region& r, const BG& g, const St& s, const int*
             r.pos[3 * q] = img(px[k], sx, g.L[0]);
             r.pos[3 * q + 1] = img(py[k], sy, g.L[1]);
             r.pos[3 * q + 2] = img(pz[k], sz, g.L[2]);
+            if (type_src) r.ltype8[q] = (uint8_t)type_src[w[k] >> 5];
+            if (key_src) r.lslot[q] = (key_src[w[k] >> 5] << 1) | ((w[k] & 31) != 13 ? 1 : 0);
         }
     }
     __syncthreads();
@@ -344,6 +349,18 @@ __device__ bool stage_from_table(Region& r, const BG& g, const St& s, const int*
 }
 
 // i-molecule q of the brick -> (region cell, LDS index, slot)
+__device__ inline void imol(const Region& r, const int* start, int q, int& rc, int& li, int& slot);
+// the same (without the region cell) from the brick's staging table when the region was staged
+// from it (capm: the region capacity the table was laid out with)
+__device__ inline void imol_tab(const Region& r, const int* start, const int* tab, int capm, int q, int& li, int& i) {
+    if (tab) {
+        i = tab[2 + capm + q];
+        li = ((const uint16_t*)(tab + 2 + 2 * capm))[q];
+    } else {
+        int rc;
+        imol(r, start, q, rc, li, i);
+    }
+}
 __device__ inline void imol(const Region& r, const int* start, int q, int& rc, int& li, int& slot) {
     int lo = 0, hi = r.nic - 1;
     while (lo < hi) {
@@ -1643,8 +1660,12 @@ template <int NT, int NB, bool EXTRA, bool REC, bool P1>
 __device__ __forceinline__ void forces_body_mie(const FbArgs& a, int brick, char* smem) {
     Region r;
     carve_mie(r, smem, a.g, a.maxreg);
-    if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags,
-                      a.moltype, nullptr))
+    // staged from the list build's table when there is one
+    const int* tab = a.btab ? a.btab + (size_t)brick * a.btab_stride : nullptr;
+    if (tab) {
+        if (!stage_from_table(r, a.g, a.s, tab, a.maxreg, a.moltype)) return;
+    } else if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags,
+                             a.moltype, nullptr))
         return;
     const int T2 = FM_MAX_TYPES * FM_MAX_TYPES;
     if constexpr (EXTRA) {
@@ -1652,8 +1673,8 @@ __device__ __forceinline__ void forces_body_mie(const FbArgs& a, int brick, char
         // (and bond virial) parked in this launch's output: what they read is then not held
         // through the pair loops
         for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
-            int rci, li, i;
-            imol(r, a.start, q, rci, li, i);
+            int li, i;
+            imol_tab(r, a.start, tab, a.maxreg, q, li, i);
             const double inv_mi = 1.0 / a.mie[7 * T2 + a.moltype[i]];
             double bx = 0.0, by = 0.0, bz = 0.0, vir = 0.0;
             bond_tether_forces<REC, REC>(a, i, r.pos[3 * li], r.pos[3 * li + 1], r.pos[3 * li + 2], inv_mi, false,
@@ -1665,8 +1686,8 @@ __device__ __forceinline__ void forces_body_mie(const FbArgs& a, int brick, char
         }
     }
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
-        int rci, li, i;
-        imol(r, a.start, q, rci, li, i);
+        int li, i;
+        imol_tab(r, a.start, tab, a.maxreg, q, li, i);
         const double xi = r.pos[3 * li], yi = r.pos[3 * li + 1], zi = r.pos[3 * li + 2];
         const int ti = a.moltype[i];
         MieRow<NT, REC> m;
@@ -1873,13 +1894,19 @@ __device__ __forceinline__ void forces_body_mop(const FbArgs& a, int brick, char
     double* spxy = (double*)(smem + region_lds_bytes(a.g, a.maxreg, true));
     double* planes = spxy + 3 * a.nplanes;
     for (int t = threadIdx.x; t < a.nplanes; t += blockDim.x) planes[t] = a.planes[t];
-    if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags,
-                      nullptr, a.ext))
-        return;
     for (int t = threadIdx.x; t < 3 * a.nplanes; t += blockDim.x) spxy[t] = 0.0;
-    // the staged molecule numbers become the owner keys
-    for (int t = threadIdx.x; t < r.total; t += blockDim.x) r.lslot[t] = (r.lslot[t] << 1) | (r.sh[r.rq[t]] != 13);
-    __syncthreads();
+    // staged from the list build's table when there is one, with the owner keys
+    const int* tab = a.btab ? a.btab + (size_t)brick * a.btab_stride : nullptr;
+    if (tab) {
+        if (!stage_from_table(r, a.g, a.s, tab, a.maxreg, nullptr, a.ext)) return;
+    } else {
+        if (!stage_region(r, a.g, brick, a.start, a.s, true, nullptr, a.cap, nullptr, nullptr, a.maxreg, a.flags,
+                          nullptr, a.ext))
+            return;
+        // the staged molecule numbers become the owner keys
+        for (int t = threadIdx.x; t < r.total; t += blockDim.x) r.lslot[t] = (r.lslot[t] << 1) | (r.sh[r.rq[t]] != 13);
+        __syncthreads();
+    }
     const size_t cap = a.cap;
     MopI m;
     m.hps = a.hp - a.shift;
@@ -1891,8 +1918,8 @@ __device__ __forceinline__ void forces_body_mop(const FbArgs& a, int brick, char
     m.planes = planes;
     const double rct = sqrt(a.rc2) * m.inv_sp;  // rc in plane spacings
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
-        int rci, li, i;
-        imol(r, a.start, q, rci, li, i);
+        int li, i;
+        imol_tab(r, a.start, tab, a.maxreg, q, li, i);
         const double xi = r.pos[3 * li], yi = r.pos[3 * li + 1], zi = r.pos[3 * li + 2];
         m.ti = (yi + m.hps) * m.inv_sp;
         m.keyi = r.lslot[li];
@@ -1968,6 +1995,40 @@ __device__ inline void carve32(Region& r, float4*& p4, char* smem, const BG& g, 
     r.ic_rc = ip;
 }
 
+// staged offsets and slots of the region from the brick's table (stage_from_table); o: the
+// region's lower corner
+__device__ bool stage_table32(Region& r, float4* p4, const BG& g, const St& s, const int* T, int capm,
+                              const double o[3]) {
+    r.total = T[0];
+    r.ni = T[1];
+    if (r.total > capm) return false;  // (a build that met a larger region halted its loop)
+    const int* stg = T + 2;
+    constexpr int SU = 6;
+    for (int q0 = threadIdx.x; q0 < r.total; q0 += SU * blockDim.x) {
+        int w[SU];
+        double px[SU], py[SU], pz[SU];
+#pragma unroll
+        for (int k = 0; k < SU; ++k) w[k] = stg[min(q0 + k * (int)blockDim.x, r.total - 1)];
+#pragma unroll
+        for (int k = 0; k < SU; ++k) {
+            const int sl = w[k] >> 5;
+            px[k] = s.x[sl];
+            py[k] = s.y[sl];
+            pz[k] = s.z[sl];
+        }
+#pragma unroll
+        for (int k = 0; k < SU; ++k) {
+            const int q = min(q0 + k * (int)blockDim.x, r.total - 1);
+            int sx, sy, sz;
+            decode_shift(w[k] & 31, sx, sy, sz);
+            p4[q] = make_float4((float)(img(px[k], sx, g.L[0]) - o[0]), (float)(img(py[k], sy, g.L[1]) - o[1]),
+                                (float)(img(pz[k], sz, g.L[2]) - o[2]), __int_as_float(w[k] >> 5));
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
 // staged offsets and slots of the region of `brick`; o: the region's lower corner
 __device__ bool stage_region32(Region& r, float4* p4, const BG& g, int brick, const int* start, const St& s,
                                int capm, int* flags, const double o[3]) {
@@ -2016,9 +2077,9 @@ __device__ __forceinline__ void lj_stage32(P4F& p, const float4* p4, uint64_t w)
 
 // the FP64 path's cutoff test for molecule i (interior: its stored position is its staged one)
 // and region entry j (slot sj), from their positions in HBM
-__device__ __forceinline__ bool in_rc_fp64(const FbArgs& a, const Region& r, int i, int j, int sj) {
+__device__ __forceinline__ bool in_rc_fp64(const FbArgs& a, const Region& r, int i, int j, int sj, const int* tab) {
     int sx, sy, sz;
-    decode_shift(r.sh[r.rq[j]], sx, sy, sz);
+    decode_shift(tab ? tab[2 + j] & 31 : r.sh[r.rq[j]], sx, sy, sz);
     const double dx = a.s.x[i] - img(a.s.x[sj], sx, a.g.L[0]), dy = a.s.y[i] - img(a.s.y[sj], sy, a.g.L[1]),
                  dz = a.s.z[i] - img(a.s.z[sj], sz, a.g.L[2]);
     return ((dx * dx + dy * dy) + dz * dz) < a.rc2;
@@ -2071,8 +2132,9 @@ __device__ __forceinline__ void lj_word32(const P4F& p, int kw, int left, float 
 
 // the list loop of lj_plain (two words in flight) over the FP32 staging, then the band entries
 // (REC: with the PE and virial sums)
+// (tab: the brick's staging table when the region was staged from it, for the image shifts)
 template <bool REC>
-__device__ __forceinline__ void lj_plain32(const FbArgs& a, const Region& r, const float4* p4, int i,
+__device__ __forceinline__ void lj_plain32(const FbArgs& a, const Region& r, const int* tab, const float4* p4, int i,
                                            const uint64_t* row, int n, size_t cap, float xi, float yi, float zi,
                                            double& fx, double& fy, double& fz, double& pe, double& vir) {
     float sx = 0.f, sy = 0.f, sz = 0.f;
@@ -2121,7 +2183,7 @@ __device__ __forceinline__ void lj_plain32(const FbArgs& a, const Region& r, con
             const float4 v = p4[j];
             const float dx = xi - v.x, dy = yi - v.y, dz = zi - v.z;
             if (!(fabsf(r2_32(dx, dy, dz) - rc2f) < band)) continue;
-            pair32<REC>(dx, dy, dz, in_rc_fp64(a, r, i, j, __float_as_int(v.w)), sx, sy, sz, a.potshift, &pe, &vt);
+            pair32<REC>(dx, dy, dz, in_rc_fp64(a, r, i, j, __float_as_int(v.w), tab), sx, sy, sz, a.potshift, &pe, &vt);
         }
     }
     fx = 24.0 * (double)sx;
@@ -2133,7 +2195,7 @@ __device__ __forceinline__ void lj_plain32(const FbArgs& a, const Region& r, con
 // P1: VV pass 2 of this step, pass 1 of the next one and its displacement update follow the force,
 // written to the other state buffer (the arithmetic of forces_body's P1 branch).  REC: a record
 // step tallying PE and virial only.
-template <bool P1, bool REC = false>
+template <bool P1, bool REC = false, bool TAB = false>
 __device__ __forceinline__ void forces_body32(const FbArgs& a, int brick, char* smem) {
     Region r;
     float4* p4;
@@ -2142,14 +2204,24 @@ __device__ __forceinline__ void forces_body32(const FbArgs& a, int brick, char* 
     const double o[3] = {-0.5 * a.g.L[0] + (bxi * a.g.B[0] - 1) * (a.g.L[0] / a.g.nc[0]),
                          -0.5 * a.g.L[1] + (byi * a.g.B[1] - 1) * (a.g.L[1] / a.g.nc[1]),
                          -0.5 * a.g.L[2] + (bzi * a.g.B[2] - 1) * (a.g.L[2] / a.g.nc[2])};
-    if (!stage_region32(r, p4, a.g, brick, a.start, a.s, a.maxreg, a.flags, o)) return;
+    // TAB (the one-domain launch): stage from the list build's table when there is one
+    const int* tab = TAB && a.btab ? a.btab + (size_t)brick * a.btab_stride : nullptr;
+    if (tab) {
+        if (!stage_table32(r, p4, a.g, a.s, tab, a.maxreg, o)) return;
+    } else if (!stage_region32(r, p4, a.g, brick, a.start, a.s, a.maxreg, a.flags, o))
+        return;
     const size_t cap = a.cap;
     for (int q = threadIdx.x; q < r.ni; q += blockDim.x) {
-        int rci, li, i;
-        imol(r, a.start, q, rci, li, i);
+        int rci = 0, li, i;
+        if (tab) {
+            i = tab[2 + a.maxreg + q];
+            li = ((const uint16_t*)(tab + 2 + 2 * a.maxreg))[q];
+        } else {
+            imol(r, a.start, q, rci, li, i);
+        }
         const float4 me = p4[li];
         double fx, fy, fz, pe, vir;
-        lj_plain32<REC>(a, r, p4, i, a.nbl + i, a.nbn[i], cap, me.x, me.y, me.z, fx, fy, fz, pe, vir);
+        lj_plain32<REC>(a, r, tab, p4, i, a.nbl + i, a.nbn[i], cap, me.x, me.y, me.z, fx, fy, fz, pe, vir);
         if constexpr (REC) {
             a.vir[i] = vir;
             a.pe_lj[i] = pe;
@@ -2260,7 +2332,7 @@ __global__ __launch_bounds__(FB_BLOCK) __attribute__((amdgpu_waves_per_eu(3 * FB
 k_forces_brick_plain32(FbArgs a) {
     extern __shared__ __align__(16) char smem[];
     if (halted(a.halt, a.step)) return;
-    forces_body32<P1, REC>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
+    forces_body32<P1, REC, true>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
 // packed launch over many domains: block b works on brick table[b].y of domain table[b].x.
