@@ -157,7 +157,7 @@ static void free_ctx(fm_ctx* c) {
     if (c->dev >= 0) hipSetDevice(c->dev);
     void* ptrs[] = {c->st[0], c->st[1], c->rdisp, c->ext[0], c->ext[1], c->slot_of, c->moltype[0], c->moltype[1],
                     c->tag[0], c->tag[1], c->teth[0], c->teth[1], c->mono[0], c->mono[1], c->cell_of,
-                    c->cell_count, c->cell_start, c->cell_cursor, c->perm, c->nbl, c->nbn, c->bond, c->bondn,
+                    c->cell_count, c->cell_start, c->cell_cursor, c->scan_part, c->perm, c->nbl, c->nbn, c->bond, c->bondn,
                     c->pe_lj, c->pe_poly, c->vir, c->rf, c->pxy, c->cvface, c->planes, c->out_flag, c->out_vals,
                     c->out_slot, c->out_count, c->red, c->d_macros, c->d_ring, c->d_mie, c->d_vmax, c->d_rmax,
                     c->d_flags, c->stage, c->brick_order, c->btab, c->hcurve, c->sort_key,
@@ -392,6 +392,7 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
         (rc = dalloc(&c->ext[0], cap)) || (rc = dalloc(&c->ext[1], cap)) || (rc = dalloc(&c->slot_of, cap)) ||
         (rc = dalloc(&c->cell_of, cap)) || (rc = dalloc(&c->cell_count, (size_t)c->ncell + 1)) ||
         (rc = dalloc(&c->cell_start, (size_t)c->ncell + 1)) || (rc = dalloc(&c->cell_cursor, (size_t)c->ncell)) ||
+        (rc = dalloc(&c->scan_part, (size_t)c->ncell / 1024 + 2)) ||
         (rc = dalloc(&c->perm, cap)) || (rc = dalloc(&c->nbn, cap)) || (rc = dalloc(&c->pe_lj, cap)) ||
         (rc = dalloc(&c->pe_poly, cap)) || (rc = dalloc(&c->vir, cap)) || (rc = dalloc(&c->out_flag, cap)) ||
         (rc = dalloc(&c->out_vals, cap)) || (rc = dalloc(&c->out_slot, cap)) || (rc = dalloc(&c->out_count, (cap + FM_BLOCK - 1) / FM_BLOCK + 4)) ||

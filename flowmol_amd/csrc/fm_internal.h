@@ -80,6 +80,7 @@ struct fm_ctx {
     int* cell_count = nullptr;            // ncell
     int* cell_start = nullptr;            // ncell+1
     int* cell_cursor = nullptr;           // ncell
+    int* scan_part = nullptr;             // cell scan: count sums of the 1024-cell chunks
     int* perm = nullptr;                  // cap
 
     // LDS bricks

@@ -540,71 +540,65 @@ __global__ __launch_bounds__(FM_BLOCK) void k_bin_count(St s, const int* ext, in
     wave_aggregated_add(count, cell, cell >= 0);
 }
 
-// start[0..n) = exclusive prefix of count[0..n) by one workgroup, in tiles of 14336 counts (60 KB
-// of LDS): a coalesced pass into LDS (one padding word per 14, so that the threads' runs of 14
-// consecutive counts start an odd number of words apart, in distinct banks), a scan by thread,
-// wavefront and workgroup, and a coalesced pass out.  The next tile's counts are loaded into
-// registers while a tile is scanned.
-// The counts are zeroed as they are read (ready for the next binning) and so is cursor[0..ncur).
-#define SCAN_PER 14
-#define SCAN_TILE (1024 * SCAN_PER)
-__global__ __launch_bounds__(1024) void k_scan_cells(int* count, int* start, int n, int* cursor, int ncur) {
-    __shared__ int t[SCAN_TILE + SCAN_TILE / SCAN_PER];
-    __shared__ int wsum[16];
+// start[0..n) = exclusive prefix of count[0..n), in chunks of SCAN_CH counts (FM_BLOCK threads,
+// four consecutive counts each): k_scan_part sums each chunk; k_scan_fin adds up the sums of the
+// chunks before its own, scans its chunk (by thread, wavefront and workgroup) and writes the
+// prefixes.  The counts are zeroed as they are read (ready for the next binning) and so is
+// cursor[0..ncur).
+#define SCAN_CH (4 * FM_BLOCK)
+__device__ inline int block_sum(int v, int* wsum) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (lane == 0) wsum[wid] = v;
+    __syncthreads();
+    int t = 0;
+    for (int w = 0; w < FM_BLOCK / 64; ++w) t += wsum[w];
+    return t;
+}
+
+__global__ __launch_bounds__(FM_BLOCK) void k_scan_part(const int* count, int* part, int n) {
+    __shared__ int wsum[FM_BLOCK / 64];
+    const int i0 = blockIdx.x * SCAN_CH + 4 * threadIdx.x;
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v += i0 + k < n ? count[i0 + k] : 0;
+    const int t = block_sum(v, wsum);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(FM_BLOCK) void k_scan_fin(int* count, int* start, const int* part, int n, int* cursor,
+                                                       int ncur) {
+    __shared__ int wsum[FM_BLOCK / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    int carry = 0;
-    // indices past the end re-read the last count
-    int vn[SCAN_PER];
+    // the sums of the chunks before this one
+    int off = 0;
+    for (int b = tid; b < (int)blockIdx.x; b += FM_BLOCK) off += part[b];
+    off = block_sum(off, wsum);
+    __syncthreads();  // (wsum is used again below)
+    const int i0 = blockIdx.x * SCAN_CH + 4 * tid;
+    int v[4], s = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_PER; ++k) vn[k] = count[min(k * 1024 + tid, n - 1)];
-    for (int base = 0; base < n; base += SCAN_TILE) {
-        int v[SCAN_PER], s = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = i0 + k < n ? count[i0 + k] : 0;
+        s += v[k];
+    }
+    int x = s;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    int pre = off + x - s;
+    for (int w = 0; w < wid; ++w) pre += wsum[w];
 #pragma unroll
-        for (int k = 0; k < SCAN_PER; ++k) v[k] = vn[k];
-        // the next tile's loads go out before this tile's stores (other indices)
-        if (base + SCAN_TILE < n) {
-#pragma unroll
-            for (int k = 0; k < SCAN_PER; ++k) vn[k] = count[min(base + SCAN_TILE + k * 1024 + tid, n - 1)];
-        }
-#pragma unroll
-        for (int k = 0; k < SCAN_PER; ++k) {
-            const int j = k * 1024 + tid, i = base + j;
-            if (i < n) count[i] = 0;
-            t[j + j / SCAN_PER] = i < n ? v[k] : 0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < SCAN_PER; ++k) {
-            v[k] = t[tid * (SCAN_PER + 1) + k];
-            s += v[k];
-        }
-        int x = s;
-        for (int d = 1; d < 64; d <<= 1) {
-            const int y = __shfl_up(x, d, 64);
-            if (lane >= d) x += y;
-        }
-        if (lane == 63) wsum[wid] = x;
-        __syncthreads();
-        int pre = carry + x - s, tot = 0;
-        for (int w = 0; w < 16; ++w) {
-            if (w < wid) pre += wsum[w];
-            tot += wsum[w];
-        }
-#pragma unroll
-        for (int k = 0; k < SCAN_PER; ++k) {
-            t[tid * (SCAN_PER + 1) + k] = pre;
+    for (int k = 0; k < 4; ++k)
+        if (i0 + k < n) {
+            start[i0 + k] = pre;
+            count[i0 + k] = 0;
             pre += v[k];
         }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < SCAN_PER; ++k) {
-            const int j = k * 1024 + tid, i = base + j;
-            if (i < n) start[i] = t[j + j / SCAN_PER];
-        }
-        carry += tot;
-        __syncthreads();
-    }
-    for (int i = tid; i < ncur; i += blockDim.x) cursor[i] = 0;
+    for (int i = blockIdx.x * FM_BLOCK + tid; i < ncur; i += gridDim.x * FM_BLOCK) cursor[i] = 0;
 }
 
 __global__ __launch_bounds__(FM_BLOCK) void k_bin_scatter(const int* cell_of, const int* start, int* cursor,
@@ -1127,7 +1121,11 @@ int launch_bin(fm_ctx* c, bool keep_acc) {
                                                              c->nc[1], c->nc[2], cell_tmp, c->cell_count,
                                                              c->d_flags);
     LAUNCH_CHECK();
-    k_scan_cells<<<1, 1024, 0, c->stream>>>(c->cell_count, c->cell_start, c->ncell + 1, c->cell_cursor, c->ncell);
+    const int nscan = (c->ncell + 1 + SCAN_CH - 1) / SCAN_CH;
+    k_scan_part<<<nscan, FM_BLOCK, 0, c->stream>>>(c->cell_count, c->scan_part, c->ncell + 1);
+    LAUNCH_CHECK();
+    k_scan_fin<<<nscan, FM_BLOCK, 0, c->stream>>>(c->cell_count, c->cell_start, c->scan_part, c->ncell + 1,
+                                                  c->cell_cursor, c->ncell);
     LAUNCH_CHECK();
     k_bin_scatter<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(cell_tmp, c->cell_start, c->cell_cursor, c->perm,
                                                                c->np);
