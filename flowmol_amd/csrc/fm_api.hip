@@ -560,6 +560,21 @@ extern "C" int fm_memory_footprint(const fm_params* p, int32_t np, int64_t* byte
     b += (size_t)(maxnb / 4) * cap * 8;                         // neighbour words
     b += 3 * (ncell + 1) * 4;                                   // cell counts, starts, cursors
     b += cap * 4 + 4 * 1024 * 1024;                             // scan/select temporary, reductions
+    b += 8 * 1024 * 1024;                                       // the allocator's rounding of ~45 buffers
+    {
+        // staging tables: 10 B per molecule of region capacity per brick (1.125 of the capacity);
+        // the regions hold each molecule prod((B+2)/B) times over, and the capacity is up to
+        // 1.6 times the mean region
+        int br[3];
+        if (p->brick[0] > 0 && p->brick[1] > 0 && p->brick[2] > 0) {
+            for (int d = 0; d < 3; ++d) br[d] = std::min(p->brick[d], nc[d]);
+        } else {
+            choose_brick(nc, std::max(1.0, (double)np / ncell), br);
+        }
+        double halo = 1.0;
+        for (int d = 0; d < 3; ++d) halo *= (br[d] + 2.0) / br[d];
+        b += (size_t)(10.0 * 1.125 * 1.6 * halo * (double)cap) + 4096;
+    }
     if (p->potential_flag == 1) b += FM_MAX_FUNCY * cap * 4 + cap * 4 + 2 * 7 * cap * 4;
     if (p->mie_potential == 1) b += 2 * cap * 4;
     if (p->ensemble == FM_ENSEMBLE_TAG_MOVE) b += 2 * cap * 4 + 2 * 6 * cap * 8;

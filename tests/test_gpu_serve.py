@@ -193,3 +193,50 @@ def test_lj_batch_members_match_alone(precision):
     b.close()
     for c in ctx:
         c.close()
+
+
+@pytest.mark.parametrize("kind", ["lj", "couette"])
+def test_memory_footprint_bounds_the_context(kind):
+    """fm_memory_footprint (the server's admission estimate) is not below the device memory a
+    context holds after its setup and 60 steps with rebuilds (lists, regions and staging tables
+    sized by then): measured as the drop in free device memory, after a first context has paid
+    the runtime's one-time cost, within 4 MiB."""
+    import torch
+
+    from flowmol_amd.serve import memory_footprint
+
+    if kind == "lj":
+        p, r, v = config.config_fcc((20, 20, 20), 0.8, 2.5, seed=81, tplot=10)
+        r = r + np.random.default_rng(81).uniform(-0.02, 0.02, r.shape)
+        tag = rt = None
+    else:
+        p, r, v, tag, rt = config.config_couette(nunits=(20, 20, 20), nplanes=16, seed=82, tplot=10)
+    say("case: memory footprint vs device memory, %s, %d molecules" % (kind, r.shape[0]))
+    torch.cuda.init()
+    # the runtime's one-time device memory (code objects and the like): a small context of the
+    # same kind, stepped
+    if kind == "lj":
+        pw, rw, vw = config.config_fcc((8, 8, 8), 0.8, 2.5, seed=80, tplot=10)
+        tw = rtw = None
+    else:
+        pw, rw, vw, tw, rtw = config.config_couette(nunits=(8, 8, 8), nplanes=16, seed=80, tplot=10)
+    w = MDContext(pw, rw, vw)
+    if tw is not None:
+        w.set_tags(tw, rtw)
+    w.setup(0)
+    w.run_vv(1, 20)
+    w.close()
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    g = MDContext(p, r, v)
+    if tag is not None:
+        g.set_tags(tag, rt)
+    g.setup(0)
+    g.run_vv(1, 60)
+    g.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    used = free0 - free1
+    est = memory_footprint(p, r.shape[0])
+    say("used %.1f MiB, footprint %.1f MiB" % (used / 2**20, est / 2**20))
+    assert used <= est + 4 * 2**20
+    g.close()
