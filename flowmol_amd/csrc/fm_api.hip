@@ -1317,6 +1317,8 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
     TRY(launch_reset_flags(c, reset, 2));
     int iter = iter0;
     int rb = -1;  // step whose displacement check fired (its pass 1 has run)
+    // pass 1 of `iter` ran inside the last force launch of the previous window
+    bool p1done = false;
     int scur_p1[W];
     while (iter < end) {
         ktimer_make_room(c, 4 * W + 64);
@@ -1334,13 +1336,13 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
         int wend = iter + 1;
         while (wend < end && wend - iter < wmax && !host_rebuild(wend)) ++wend;
         int lists = -1, scur_lists = 0;
-        bool fused_prev = false;
+        bool fused_prev = p1done;
         bool ringed[W];
         int want = macros ? n_macros - *nm : 0;  // record sums still wanted
         for (int k = iter; k < wend; ++k) {
             c->kt_step = k;
             if (k == iter && rebuild_first) {
-                if (rb != iter) {
+                if (rb != iter && !p1done) {
                     c->moved_since_bin = true;
                     TRY(launch_pass1(c));
                 }
@@ -1358,7 +1360,8 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
             } else {
                 scur_p1[k - iter] = c->scur;  // pass 1 of k ran inside the force launch of k-1
             }
-            const int next = (disp && k + 1 < wend) ? k + 1 : -1;
+            // (the window's last step too, unless it ends the run: its pass 1 opens the next window)
+            const int next = (disp && k + 1 < end) ? k + 1 : -1;
             TRY(launch_forces(c, k, true, disp || lists >= 0, next, &fused_prev));
             if (fused_prev) c->moved_since_bin = true;
             ringed[k - iter] = want > 0 && (k % c->p.tplot) == 0;
@@ -1387,8 +1390,20 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
             TRY(launch_forces(c, lists, true));
             TRY(finish_step(c, lists, macros, n_macros, nm));
             iter = lists + 1;
+            p1done = false;
             continue;
         }
+        if (disp && c->h_flags[FLAG_REBUILD] && c->h_flags[FLAG_RB_STEP] == wend && fused_prev) {
+            // the check fired in the pass 1 of step wend, inside the window's last force launch:
+            // every step of the window completed, and the state after that pass 1 is current
+            for (int k = iter; k < wend; ++k)
+                TRY(finish_step(c, k, macros, n_macros, nm, ringed[k - iter] ? c->h_ring + 8 * (k - iter) : nullptr));
+            rb = wend;
+            iter = wend;
+            p1done = true;
+            continue;
+        }
+        p1done = false;
         if (disp && c->h_flags[FLAG_REBUILD]) {
             const int s = c->h_flags[FLAG_RB_STEP];
             if (s < iter || s >= wend || s == lists)
@@ -1406,6 +1421,7 @@ static int run_vv_queued(fm_ctx* c, int iter0, int nsteps, fm_macros* macros, in
             for (int k = iter; k < wend; ++k)
                 TRY(finish_step(c, k, macros, n_macros, nm, ringed[k - iter] ? c->h_ring + 8 * (k - iter) : nullptr));
             iter = wend;
+            p1done = fused_prev;  // the last launch carried pass 1 of step wend
         }
     }
     return 0;
