@@ -2346,7 +2346,9 @@ k_forces_brick_multi(const FbArgs* descs, const int2* table) {
     const FbArgs& a = descs[t.x];
     if (skip_domain(a)) return;
     if constexpr (PLAIN) {
-        forces_body<MIE, false, EXTRA>(a, t.y, smem);
+        // (LJ: staged from the member's table when its last list build wrote one; a member's
+        // rebuilds open its windows, before the window's descriptors are made)
+        forces_body<MIE, false, EXTRA, false, false, !MIE>(a, t.y, smem);
         return;
     }
     if (a.rec) forces_body<MIE, true, EXTRA>(a, t.y, smem);
