@@ -717,32 +717,37 @@ __global__ __launch_bounds__(FM_BLOCK) void k_permute(PermArgs a) {
 }
 
 // ---- macroscopic sums, src/simulation_record.f90:1205-1314 ------------------------------------
-// 7 sums per block: pe_lj, pe_poly, pe_lj+pe_poly, vir, mv2, vsum, msum ; + max(pe)
+// 7 sums per block: pe_lj, pe_poly, pe_lj+pe_poly, vir, mv2, vsum, msum ; + max(pe).  Each thread
+// sums MACRO_MPT molecules a grid apart (in that order), then the wavefront and the workgroup.
 #define NRED 8
+#define MACRO_MPT 4
 __global__ __launch_bounds__(FM_BLOCK) void k_macros_partial(St s, const double* pe_lj, const double* pe_poly,
                                                              const double* vir, const int* moltype,
                                                              const double* mass, int np, int lfv, double dt,
                                                              double* part) {
     __shared__ double red[NRED][FM_BLOCK / 64];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[NRED] = {0, 0, 0, 0, 0, 0, 0, -1.0e300};
-    if (i < np) {
+    const int stride = gridDim.x * blockDim.x;
+#pragma unroll
+    for (int k = 0; k < MACRO_MPT; ++k) {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x + k * stride;
+        if (i >= np) break;
         double m = moltype ? mass[moltype[i]] : 1.0;
         double pl = pe_lj[i], pp = pe_poly ? pe_poly[i] : 0.0;
-        v[0] = pl;
-        v[1] = pp;
-        v[2] = pl + pp;
-        v[3] = vir[i];
+        v[0] = v[0] + pl;
+        v[1] = v[1] + pp;
+        v[2] = v[2] + (pl + pp);
+        v[3] = v[3] + vir[i];
         double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
         if (lfv) {  // leap-frog: velocity shifted half a step, src/simulation_record.f90:1238-1247
             vx = vx + 0.5 * s.ax[i] * dt;
             vy = vy + 0.5 * s.ay[i] * dt;
             vz = vz + 0.5 * s.az[i] * dt;
         }
-        v[4] = (m * (vx * vx) + m * (vy * vy)) + m * (vz * vz);
-        v[5] = (m * vx + m * vy) + m * vz;
-        v[6] = m;
-        v[7] = pl + pp;
+        v[4] = v[4] + ((m * (vx * vx) + m * (vy * vy)) + m * (vz * vz));
+        v[5] = v[5] + ((m * vx + m * vy) + m * vz);
+        v[6] = v[6] + m;
+        v[7] = fmax(v[7], pl + pp);
     }
     for (int q = 0; q < NRED; ++q) {
         double x = v[q];
@@ -1164,7 +1169,7 @@ int launch_bin(fm_ctx* c, bool keep_acc) {
 }
 
 int launch_macros(fm_ctx* c, int buf, double* out) {
-    int nb = nblocks(c->np);
+    int nb = nblocks((c->np + MACRO_MPT - 1) / MACRO_MPT);
     if (nb == 0) nb = 1;
     if (buf < 0) buf = c->scur;
     ktimer_begin(c, KT_MACROS);
