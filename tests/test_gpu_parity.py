@@ -247,6 +247,28 @@ def test_fused_loop_equals_routine_sequence(c1_state):
     g2.close()
 
 
+@pytest.mark.parametrize("tplot", [7, 10])
+def test_fused_loop_frequent_rebuilds_equals_routine_sequence(tplot):
+    """A thin skin (delta 0.1) makes the displacement check fire every few steps, at every
+    position in the queued windows, including the pass 1 carried by a window's last force launch;
+    the forced rebuilds before record steps (TPLOT 7 and 10) open other windows.  300 steps of the
+    fused loop equal the routine-by-routine sequence bit for bit, with the same rebuild steps."""
+    p, r, v = config.config_fcc((6, 6, 6), 0.8, 2.5, seed=91, tplot=tplot, delta=0.1)
+    r = r + np.random.default_rng(91).uniform(-0.01, 0.01, r.shape)
+    say("case: fused loop vs routine sequence, delta 0.1, TPLOT %d, 864 LJ, 300 steps" % tplot)
+    g = MDContext(p, r, v)
+    f = MDContext(p, r, v)
+    g.setup(0)
+    f.setup(0)
+    rg = [it for it in range(1, 301) if g.md_advance_vv(it)]
+    _, nrb = f.run_vv(1, 300)
+    assert nrb == len(rg) and len(rg) > 60
+    for x, y in zip(f.state(), g.state()):
+        assert np.array_equal(x, y)
+    f.close()
+    g.close()
+
+
 def test_energy_conservation_c1():
     p, r, v = config.config_c1()
     say("case: NVE drift C1 1000 steps")
