@@ -74,3 +74,27 @@ def test_struct_layout_matches_header():
     assert _abi.FmParams.binspercell.offset == obin
     assert _abi.FmParams.cv_conserve.offset == ocv
     assert _abi.FmParams.sortblocksize.offset == osb
+
+
+def test_memory_footprint_terms():
+    """fm_memory_footprint (host arithmetic, no device): grows with the molecule count; the
+    staging tables scale with the brick's halo factor (1-cell bricks hold each molecule 27 times
+    over, 3-cell bricks about 4.6 times); for a large system the estimate covers the state
+    copies, the list words and the tables, plus fixed terms of at most 16 MiB."""
+    from flowmol_amd import config
+    from flowmol_amd.serve import memory_footprint
+
+    p, r, _ = config.config_fcc((16, 16, 16), 0.8, 2.5, seed=1, tplot=10)
+    n = r.shape[0]
+    small, big = memory_footprint(p, n // 2), memory_footprint(p, n)
+    assert 0 < small < big
+    p1 = type(p).from_buffer_copy(p)
+    p3 = type(p).from_buffer_copy(p)
+    for d in range(3):
+        p1.brick[d], p3.brick[d] = 1, 3
+    f1, f3 = memory_footprint(p1, n), memory_footprint(p3, n)
+    tables = lambda halo: 10.0 * 1.125 * 1.6 * halo * n
+    assert abs((f1 - f3) - (tables(27.0) - tables(125.0 / 27.0))) < 8192
+    state = (2 * 9 + 3 + 9) * 8 * n           # state copies, rdisp, staging
+    assert f3 >= state + tables(125.0 / 27.0)
+    assert f3 <= 2 * state + 16 * 2**20 + tables(125.0 / 27.0) + 400 * n
