@@ -225,6 +225,34 @@ def test_couette_walls_short_trajectory():
     g.close()
 
 
+def test_mie_tethered_walls_trajectory():
+    """Mie types (12-6 and 14-6, per-pair cutoffs) with tethered sliding walls: the branch-free Mie
+    launch with its tether pass on steps without tallies (TPLOT 5; record steps with MOP take the
+    general loop) against the oracle, at a plain step and over a 30-step drop-in run."""
+    p, r, v, tag, rt = config.config_couette(nunits=(8, 8, 8), tplot=5, nplanes=16)
+    config.set_mie_types(p, {(0, 0): dict(lr=12.0, la=6.0), (0, 1): dict(lr=12.0, la=6.0, rc=2 ** (1 / 6)),
+                             (1, 1): dict(lr=14.0, la=6.0, eps=1.2)})
+    mt = np.random.default_rng(95).integers(0, 2, r.shape[0]).astype(np.int32)
+    say("case: Mie types + tethered walls, sliding top, 30 steps")
+    o = Oracle(p, r, v)
+    o.set_tags(tag, rt)
+    o.set_moltype(mt)
+    g = MDContext(p, r, v)
+    g.set_tags(tag, rt)
+    g.set_moltype(mt)
+    o.setup()
+    o.compute_forces(0)
+    g.setup(0)
+    o.compute_forces(1)   # a step without tallies
+    g.simulation_compute_forces(1)
+    assert rel_force_err(g.state()[2], o.state()[2], o.force_scale() + 1.0) < TOL
+    ro, rg = drop_in_steps(o, g, 1, 30)
+    assert ro == rg
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-9
+    assert np.abs(g.rtether() - o.rtether()).max() < 1e-12
+    g.close()
+
+
 def _lfv(p):
     q = type(p).from_buffer_copy(p)
     q.integration_algorithm = FM_INTEGRATION_LFV
