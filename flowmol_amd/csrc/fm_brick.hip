@@ -979,6 +979,8 @@ struct FbArgs {
     // (stage_from_table; the LJ plain launches)
     const int* btab;
     int btab_stride;
+    // the branch-free Mie launch's partner types and exponent class (mie_plain_types), or 0
+    int mie_nt, mie_nb;
 };
 
 // forward cell offsets of the reference's half-list scan (src/linklist.f90:686-688)
@@ -1656,7 +1658,9 @@ __host__ __device__ inline size_t mie_lds_bytes(const BG& g, int maxreg) {
 // Forces of a Mie step without tallies (REC: a record step tallying PE and virial only), bonds and
 // tethers added to the pair sum after it; the kick of a fused simulation_move_particles_vv(2) as
 // forces_body does it, or with P1 pass 2 of this step and pass 1 of the next (vv_pass2_pass1).
-template <int NT, int NB, bool EXTRA, bool REC, bool P1>
+// (KA: a is the launch's argument block, read again after the pair loop; packed launches pass
+// false: a is a domain's descriptor in device memory)
+template <int NT, int NB, bool EXTRA, bool REC, bool P1, bool KA = true>
 __device__ __forceinline__ void forces_body_mie(const FbArgs& a, int brick, char* smem) {
     Region r;
     carve_mie(r, smem, a.g, a.maxreg);
@@ -1695,7 +1699,7 @@ __device__ __forceinline__ void forces_body_mie(const FbArgs& a, int brick, char
         const double inv_mi = 1.0 / a.mie[7 * T2 + ti];
         double fx, fy, fz, pe, vir;
         mie_plain<NT, NB, REC>(r.pos, r.ltype8, a.nbl + i, a.nbn[i], a.cap, xi, yi, zi, m, fx, fy, fz, pe, vir);
-        const FbArgs& b = fargs_again();  // the arguments below: not held through the pair loop
+        const FbArgs& b = KA ? fargs_again() : a;  // the arguments below: not held through the pair loop
         fx = fx * inv_mi;
         fy = fy * inv_mi;
         fz = fz * inv_mi;
@@ -2335,6 +2339,20 @@ k_forces_brick_plain32(FbArgs a) {
     forces_body32<P1, REC, true>(a, brick_of_block(a.order, blockIdx.x, gridDim.x), smem);
 }
 
+// the branch-free Mie body of a packed launch's domain, by its partner types and exponent class
+template <bool EXTRA, bool REC>
+__device__ __forceinline__ void mie_dispatch(const FbArgs& a, int brick, char* smem) {
+    if (a.mie_nt == 2) {
+        if (a.mie_nb == 0) forces_body_mie<2, 0, EXTRA, REC, false, false>(a, brick, smem);
+        else if (a.mie_nb == 4) forces_body_mie<2, 4, EXTRA, REC, false, false>(a, brick, smem);
+        else forces_body_mie<2, 5, EXTRA, REC, false, false>(a, brick, smem);
+    } else {
+        if (a.mie_nb == 0) forces_body_mie<4, 0, EXTRA, REC, false, false>(a, brick, smem);
+        else if (a.mie_nb == 4) forces_body_mie<4, 4, EXTRA, REC, false, false>(a, brick, smem);
+        else forces_body_mie<4, 5, EXTRA, REC, false, false>(a, brick, smem);
+    }
+}
+
 // packed launch over many domains: block b works on brick table[b].y of domain table[b].x.
 // PLAIN: no domain of the launch is on a record or virial step (only the plain path is compiled,
 // and its registers are held to what lets two workgroups share a CU, as for the one-domain launch)
@@ -2345,6 +2363,16 @@ k_forces_brick_multi(const FbArgs* descs, const int2* table) {
     const int2 t = table[blockIdx.x];
     const FbArgs& a = descs[t.x];
     if (skip_domain(a)) return;
+    if constexpr (MIE) {
+        // a Mie domain takes the branch-free Mie body on the steps where its one-domain launch
+        // does (steps without tallies, record steps of PE and virial only)
+        const bool simple_rec = a.rec && a.pxy == nullptr && a.cv == nullptr && a.rf == nullptr;
+        if (a.mie_nt > 0 && ((!a.rec && !a.ev) || simple_rec)) {
+            if (a.rec) mie_dispatch<EXTRA, true>(a, t.y, smem);
+            else mie_dispatch<EXTRA, false>(a, t.y, smem);
+            return;
+        }
+    }
     if constexpr (PLAIN) {
         // (LJ: staged from the member's table when its last list build wrote one; a member's
         // rebuilds open its windows, before the window's descriptors are made)
@@ -2809,6 +2837,9 @@ int make_fargs(fm_ctx* c, int iter, bool kick, void* out, size_t* lds_out, int* 
     a.band32 = band32_of(c);
     a.btab = c->btab_valid ? c->btab : nullptr;
     a.btab_stride = c->btab_stride;
+    a.mie_nb = 0;
+    // (Mie tables compute in FP64 in either precision)
+    a.mie_nt = mie ? mie_plain_types(c, &a.mie_nb) : 0;
     a.cv = (a.rec && c->cvface) ? c->cvface : nullptr;
     for (int d = 0; d < 3; ++d) {
         a.cvnbo[d] = c->cv_nbo[d];
@@ -2867,8 +2898,7 @@ int launch_forces(fm_ctx* c, int iter, bool kick, bool cond, int next, bool* fus
     // FP32-staged one in mixed precision)
     const bool simple_rec = (kind == 0 || kind == 1) && a.rec && a.pxy == nullptr && a.cv == nullptr && a.rf == nullptr;
     // the branch-free Mie launch (FP64): steps without tallies, and record steps of PE and virial
-    int mnb = 0;
-    const int mnt = (mie && !mixed) ? mie_plain_types(c, &mnb) : 0;
+    const int mnb = a.mie_nb, mnt = mixed ? 0 : a.mie_nt;
     const bool mie_rec = mnt > 0 && a.rec && a.pxy == nullptr && a.cv == nullptr && a.rf == nullptr;
     const bool mie_fast = mnt > 0 && ((!a.rec && !a.ev) || mie_rec);
     if (next >= 0 && ((((kind == 0 || kind == 1) && !a.rec && !a.ev) || simple_rec) || mie_fast) && kick && fused) {
