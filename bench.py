@@ -96,8 +96,8 @@ def serve_main(a):
 
 
 def serve_result(a):
-    """The replica stream (BASELINE's batched-serving config): Poisson arrivals of Couette
-    replicas, continuous batching; p50/p99 latency and aggregate throughput."""
+    """The replica stream (batched serving of uneven Couette replicas): Poisson arrivals,
+    continuous batching; p50/p99 latency and aggregate throughput."""
     import numpy as np
 
     from flowmol_amd.serve import ReplicaServer, poisson_workload
