@@ -183,6 +183,69 @@ def test_forces_lj_rc25():
     g.close()
 
 
+@pytest.mark.parametrize("nunits,density,rc", [((8, 8, 8), 0.8, 4.0), ((8, 8, 8), 1.1, 3.0), ((6, 6, 6), 0.3, 2.5)])
+def test_other_cutoffs_and_densities(nunits, density, rc):
+    """Away from the rc 2.5, density 0.8 liquid: a 4.0 cutoff (3 cells of 4.56 a side, about 76
+    molecules per cell, more than one wavefront sorts per cell, and about 350 full-list entries
+    per molecule), a dense 1.1 state at rc 3.0 (the same cell count) and a 0.3 gas (5 cells of 2.83).
+    The neighbour set is the reference's, forces, PE and virial match on a record step and on a
+    step without tallies, 60 routine-by-routine steps take the oracle's rebuild steps and stay
+    within 1e-10, and the fused loop equals the routine sequence bit for bit."""
+    p, r, v = config.config_fcc(nunits, density, rc, seed=17, tplot=10)
+    r = r + np.random.default_rng(17).uniform(-1e-3, 1e-3, r.shape)
+    o = Oracle(p, r, v)
+    o.setup()
+    o.compute_forces(0)
+    o.run_vv(1, 100)
+    o.rebuild()
+    r, v, a = o.state()
+    say(f"case: LJ {r.shape[0]} density {density} rc {rc}")
+    o, g = _pair(p, r, v, a)
+    o.setup()
+    g.setup(0)
+    assert np.array_equal(g.cells(), o.cells())
+    oi, oj, os_ = o.neighbourlist()
+    gi, gj, gs = g.neighbourlist()
+    assert np.array_equal(canonical_pairs(oi, oj, os_), canonical_pairs(gi, gj, gs))
+    for it in (0, 3):  # record step, and a step without tallies
+        o.compute_forces(it)
+        g.simulation_compute_forces(it)
+        assert rel_force_err(g.state()[2], o.state()[2], o.force_scale()) < FORCE_TOL
+        if it == 0:
+            peo, viro = o.molecular()
+            peg, virg = g.molecular()
+            np.testing.assert_allclose(peg, peo, rtol=FORCE_TOL, atol=1e-13)
+            np.testing.assert_allclose(virg, viro, rtol=FORCE_TOL, atol=1e-12)
+    g.close()
+    o, g = _pair(p, r, v, a)
+    f = MDContext(p, r, v, a)
+    o.setup()
+    o.compute_forces(0)
+    g.setup(0)
+    f.setup(0)
+    nrb = 0
+    for it in range(1, 61):
+        o.move_particles_vv(1)
+        o.updateborders(0)
+        rbo = o.checkrebuild(it)
+        if rbo:
+            o.rebuild()
+        o.compute_forces(it)
+        o.move_particles_vv(2)
+        rbg = g.md_advance_vv(it)
+        assert rbo == rbg, f"rebuild decision differs at iter {it}"
+        nrb += bool(rbg)
+    assert nrb >= 6  # before each record step, and on the gas's displacement checks
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-10
+    assert np.abs(g.state()[1] - o.state()[1]).max() < 1e-9
+    _, nrf = f.run_vv(1, 60)
+    assert nrf == nrb
+    for x, y in zip(f.state(), g.state()):
+        assert np.array_equal(x, y)
+    f.close()
+    g.close()
+
+
 def test_sendmols_renumbering_bit_exact():
     """Molecules pushed outside the box are wrapped and renumbered exactly as the single-rank
     sendmols does (leavers of each axis/direction pass get their numbers reversed)."""
