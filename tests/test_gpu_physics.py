@@ -439,6 +439,53 @@ def test_mixed_precision_fused_loop_equals_routine_sequence():
     g.close()
 
 
+@pytest.mark.parametrize("nunits,density,rc", [((8, 8, 8), 0.8, 4.0), ((8, 8, 8), 1.1, 3.0), ((6, 6, 6), 0.3, 2.5)])
+def test_mixed_precision_other_cutoffs_and_densities(nunits, density, rc):
+    """The mixed mode's FP32 staging at a 4.0 cutoff (about 76 molecules per cell, cells of 4.56
+    a side), a 1.1 state at rc 3.0 and a 0.3 gas: forces within 1e-5 RMS of the FP64 path on a
+    record step and on a step without tallies, PE within 1e-5, and the fused loop equal to the
+    routine sequence over 60 steps with the same rebuild steps."""
+    p, r, v = config.config_fcc(nunits, density, rc, seed=19, tplot=10)
+    pm, _, _ = config.config_fcc(nunits, density, rc, seed=19, tplot=10, precision=FM_PREC_MIXED)
+    r = r + np.random.default_rng(19).uniform(-1e-3, 1e-3, r.shape)
+    o = Oracle(p, r, v)
+    o.setup()
+    o.compute_forces(0)
+    o.run_vv(1, 100)
+    o.rebuild()
+    r, v, a = o.state()
+    say(f"case: mixed precision, LJ {r.shape[0]} density {density} rc {rc}")
+    g64 = MDContext(p, r, v, a)
+    g32 = MDContext(pm, r, v, a)
+    g64.setup(0)
+    g32.setup(0)
+    for it in (0, 3):  # setup's record-step forces, then a step without tallies
+        if it:
+            g64.simulation_compute_forces(it)
+            g32.simulation_compute_forces(it)
+        a64, a32 = g64.state()[2], g32.state()[2]
+        rms = np.sqrt(((a32 - a64) ** 2).sum() / (a64 ** 2).sum())
+        say(f"  step {it} force RMS relative error {rms:.2e}")
+        assert rms < 1e-5
+        if it == 0:
+            pe64, _ = g64.molecular()
+            pe32, _ = g32.molecular()
+            assert abs(pe32.sum() - pe64.sum()) < 1e-5 * abs(pe64.sum())
+    g64.close()
+    g32.close()
+    g = MDContext(pm, r, v, a)
+    f = MDContext(pm, r, v, a)
+    g.setup(0)
+    f.setup(0)
+    rg = [it for it in range(1, 61) if g.md_advance_vv(it)]
+    _, nrb = f.run_vv(1, 60)
+    assert nrb == len(rg) and len(rg) >= 6
+    for x, y in zip(f.state(), g.state()):
+        assert np.array_equal(x, y)
+    f.close()
+    g.close()
+
+
 # 2-cell blocks on 8 lattice units: 4 cells a side, so the 2 blocks a side split the box evenly.
 # On 6 units (3 cells a side) the first block holds 8/27 of the molecules, above the estimate
 # ceil(1.5*108 + 5) = 167, so the first calls skip and raise the safety factors instead.
