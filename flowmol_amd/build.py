@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libflowmol_hip.so")
-SOURCES = ["fm_api.hip", "fm_kernels.hip", "fm_brick.hip", "fm_batch.hip", "fm_cub.hip"]
+SOURCES = ["fm_api.hip", "fm_kernels.hip", "fm_brick.hip", "fm_batch.hip"]
 ARCH = os.environ.get("FLOWMOL_ARCH", "gfx950")
 
 CFLAGS = [

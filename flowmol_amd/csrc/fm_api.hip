@@ -477,9 +477,7 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
             free_ctx(c);
             return rc;
         }
-        int end_bit = 1;
-        while ((1 << end_bit) <= (int)hc.size()) ++end_bit;
-        c->sort_tmp_bytes = cub_sort_pairs_bytes((int)cap, end_bit);
+        c->sort_tmp_bytes = radix_sort_bytes((int)cap);
         if ((rc = dalloc(&c->hcurve, hc.size())) || (rc = dalloc(&c->sort_key, cap)) ||
             (rc = dalloc(&c->sort_val, cap)) || (rc = dalloc(&c->sort_key2, cap)) || (rc = dalloc(&c->sort_val2, cap)) ||
             (rc = dalloc(&c->sort_cnt, hc.size() + 1)) || (rc = dalloc((char**)&c->sort_tmp, c->sort_tmp_bytes))) {
@@ -580,7 +578,7 @@ extern "C" int fm_memory_footprint(const fm_params* p, int32_t np, int64_t* byte
     if (p->ensemble == FM_ENSEMBLE_TAG_MOVE) b += 2 * cap * 4 + 2 * 6 * cap * 8;
     if (p->pressure_outflag == 1) b += 9 * cap * 8;
     if (p->vflux_outflag >= 1 && p->vflux_outflag <= 3) b += 4 * (size_t)std::max(p->nplanes, 0) * 8;
-    if (p->sort_flag != 0) b += 4 * cap * 4 + 2 * (ncell + 1) * 4 + cap * 8;  // keys, values, curve, counts, sort temporary
+    if (p->sort_flag != 0) b += 4 * cap * 4 + 2 * (ncell + 1) * 4 + cap * 8 + 8192;  // keys, values, curve, counts, sort temporary
     if (p->vflux_outflag == 4) {
         size_t nbo = 1;
         for (int d = 0; d < 3; ++d) {

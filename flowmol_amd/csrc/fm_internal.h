@@ -250,8 +250,8 @@ size_t brick_lds_bytes(const fm_ctx* c, int maxreg, int kind);  // kind 0 list b
 int launch_region_max(fm_ctx* c);
 int launch_decode_nlist(fm_ctx* c, uint32_t* out);
 
-// hipcub wrappers (fm_cub.hip)
-size_t cub_sort_pairs_bytes(int n_items, int end_bit);
-int cub_sort_pairs(fm_ctx* c, const int* kin, int* kout, const int* vin, int* vout, int n, int end_bit);
+// stable radix sort of (key, value) pairs (fm_kernels.hip)
+size_t radix_sort_bytes(int n_items);
+int radix_sort_pairs(fm_ctx* c, int* k0, int* v0, int* k1, int* v1, int n, int end_bit, int** vals);
 int launch_sort_mols(fm_ctx* c, int* overflow_out);
 int ctx_sync_flags(fm_ctx* c);  // read the flag words, report a device-side error

@@ -1030,6 +1030,79 @@ __global__ __launch_bounds__(FM_BLOCK) void k_max_of(const int* v, int n, int* f
     }
 }
 
+// ---- stable LSD radix sort of (key, value) pairs, 8 bits a pass ---------------------------------
+// Each workgroup takes a tile of FM_BLOCK consecutive pairs.  k_radix_hist counts the tile's digits
+// into hist[digit * ntiles + tile]: digit-major, so the exclusive prefix of the whole array (the
+// chunked scan above) is, for each tile and digit, the first output position of that tile's pairs
+// with that digit.  k_radix_scatter places each pair after the pairs of its digit that come before
+// it in its tile: the lanes of its wavefront below it whose digit matches (eight ballots, one per
+// digit bit) and the matching counts of the wavefronts before its own.  Tiles, wavefronts and lanes
+// keep the input order, so every pass is stable.
+static_assert(FM_BLOCK == 256, "one 8-bit digit bucket per thread of the tile");
+
+__global__ __launch_bounds__(FM_BLOCK) void k_radix_hist(const int* key, int n, int shift, int* hist, int ntiles) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * FM_BLOCK + threadIdx.x;
+    if (i < n) atomicAdd(&h[(key[i] >> shift) & 255], 1);
+    __syncthreads();
+    hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(FM_BLOCK) void k_radix_scatter(const int* kin, const int* vin, int* kout, int* vout,
+                                                             int n, int shift, const int* start, int ntiles) {
+    __shared__ int wc[FM_BLOCK / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (int w = 0; w < FM_BLOCK / 64; ++w) wc[w][tid] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * FM_BLOCK + tid;
+    const bool active = i < n;
+    const int k = active ? kin[i] : 0;
+    const int d = (k >> shift) & 255;
+    uint64_t m = __ballot(active);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const uint64_t on = __ballot((d >> b) & 1);
+        m &= ((d >> b) & 1) ? on : ~on;
+    }
+    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (active && rank == 0) wc[wid][d] = __popcll(m);
+    __syncthreads();
+    if (!active) return;
+    int pos = start[(size_t)d * ntiles + blockIdx.x] + rank;
+    for (int w = 0; w < wid; ++w) pos += wc[w][d];
+    kout[pos] = k;
+    vout[pos] = vin[i];
+}
+
+size_t radix_sort_bytes(int n) {
+    const size_t nt = ((size_t)std::max(n, 1) + FM_BLOCK - 1) / FM_BLOCK, nh = 256 * nt;
+    return (2 * nh + nh / SCAN_CH + 2) * sizeof(int);
+}
+
+// Sorts the pairs (k0, v0)[0..n) by key bits [0, end_bit), stably, with (k1, v1) as the other
+// buffer of each pass; *vals is the buffer that ends with the sorted values.
+int radix_sort_pairs(fm_ctx* c, int* k0, int* v0, int* k1, int* v1, int n, int end_bit, int** vals) {
+    const int nt = (n + FM_BLOCK - 1) / FM_BLOCK, nh = 256 * nt, nscan = (nh + SCAN_CH - 1) / SCAN_CH;
+    if ((size_t)(2 * (size_t)nh + nh / SCAN_CH + 2) * sizeof(int) > c->sort_tmp_bytes)
+        return fm_set_error(FM_EOVERFLOW, "radix sort of %d pairs exceeds its temporary", n);
+    int* hist = (int*)c->sort_tmp;
+    int* start = hist + nh;
+    int* part = start + nh;
+    for (int shift = 0; shift < end_bit; shift += 8) {
+        k_radix_hist<<<nt, FM_BLOCK, 0, c->stream>>>(k0, n, shift, hist, nt);
+        k_scan_part<<<nscan, FM_BLOCK, 0, c->stream>>>(hist, part, nh);
+        k_scan_fin<<<nscan, FM_BLOCK, 0, c->stream>>>(hist, start, part, nh, nullptr, 0);
+        k_radix_scatter<<<nt, FM_BLOCK, 0, c->stream>>>(k0, v0, k1, v1, n, shift, start, nt);
+        LAUNCH_CHECK();
+        std::swap(k0, k1);
+        std::swap(v0, v1);
+    }
+    *vals = v0;
+    return 0;
+}
+
 // One sort_mols renumbering (the caller handles the call counter).  *overflow_out = 1 when a
 // block exceeds the estimate and the round is skipped, as the reference does.
 int launch_sort_mols(fm_ctx* c, int* overflow_out) {
@@ -1055,8 +1128,11 @@ int launch_sort_mols(fm_ctx* c, int* overflow_out) {
     }
     int end_bit = 1;
     while ((1 << end_bit) <= blocks) ++end_bit;
-    if (int rc = cub_sort_pairs(c, c->sort_key, c->sort_key2, c->sort_val, c->sort_val2, c->np, end_bit)) return rc;
-    k_sort_newof<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(c->sort_val2, c->np, c->sort_key);
+    int* order = nullptr;
+    if (int rc = radix_sort_pairs(c, c->sort_key, c->sort_val, c->sort_key2, c->sort_val2, c->np, end_bit, &order))
+        return rc;
+    // the keys are spent: sort_key takes the new numbers (order is a value buffer)
+    k_sort_newof<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(order, c->np, c->sort_key);
     LAUNCH_CHECK();
     const bool tm = c->p.ensemble == FM_ENSEMBLE_TAG_MOVE && c->has_tags;
     k_sort_relabel<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(

@@ -533,6 +533,38 @@ def test_sort_mols_trajectory(flag, freq, bs, nu, moves):
     g.close()
 
 
+@pytest.mark.parametrize("flag", [1, 2])
+def test_sort_mols_many_blocks(flag):
+    """343 one-cell blocks (6912 molecules, 7 cells a side): block numbers up to 343 take two
+    8-bit passes of the radix sort.  The first direct call skips (a block above the estimate) and
+    raises the safety factors, the second sorts; both sides hold the same numbering after each,
+    and along 40 steps whose rebuilds sort again."""
+    p, r, v, a = _lj((12, 12, 12), seed=29, steps=100, tplot=1000, sort_flag=flag, sort_freq=1, sortblocksize=1)
+    perm = np.random.default_rng(31).permutation(r.shape[0])
+    r, v, a = r[perm].copy(), v[perm].copy(), a[perm].copy()
+    say(f"case: sort_mols flag {flag}, 343 blocks, {r.shape[0]} LJ, 40 steps")
+    o = Oracle(p, r, v, a)
+    g = MDContext(p, r, v, a)
+    o.setup()
+    o.compute_forces(0)
+    g.setup(0)
+    for moves in (False, True):
+        o.sort_mols()
+        g.sort_mols()
+        assert moves == (not np.array_equal(o.state()[0], r))
+        for x, y in zip(o.state()[:2], g.state()[:2]):
+            assert np.array_equal(x, y)
+    o.rebuild()
+    g.rebuild()
+    o.compute_forces(0)
+    g.simulation_compute_forces(0)
+    ro, rg = drop_in_steps(o, g, 1, 40)
+    assert ro == rg
+    assert np.abs(g.state()[0] - o.state()[0]).max() < 1e-10
+    assert np.abs(g.state()[1] - o.state()[1]).max() < 1e-10
+    g.close()
+
+
 def test_sort_mols_tethered_walls_lfv():
     """Tags, tether sites and sliding velocities travel with the renumbered molecules."""
     p, r, v, tag, rt = config.config_couette(nunits=(8, 8, 8), tplot=5, nplanes=16, sort_flag=2,
