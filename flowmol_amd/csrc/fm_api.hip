@@ -157,7 +157,7 @@ static void free_ctx(fm_ctx* c) {
     if (c->dev >= 0) hipSetDevice(c->dev);
     void* ptrs[] = {c->st[0], c->st[1], c->rdisp, c->ext[0], c->ext[1], c->slot_of, c->moltype[0], c->moltype[1],
                     c->tag[0], c->tag[1], c->teth[0], c->teth[1], c->mono[0], c->mono[1], c->cell_of,
-                    c->cell_count, c->cell_start, c->cell_cursor, c->scan_part, c->perm, c->nbl, c->nbn, c->bond, c->bondn,
+                    c->cell_count, c->cell_start, c->bin_pos, c->scan_part, c->perm, c->nbl, c->nbn, c->bond, c->bondn,
                     c->pe_lj, c->pe_poly, c->vir, c->rf, c->pxy, c->cvface, c->planes, c->out_flag, c->out_vals,
                     c->out_slot, c->out_count, c->red, c->d_macros, c->d_ring, c->d_mie, c->d_vmax, c->d_rmax,
                     c->d_flags, c->stage, c->brick_order, c->btab, c->hcurve, c->sort_key,
@@ -391,7 +391,7 @@ extern "C" int fm_init(const fm_params* p, int32_t np, int32_t capacity, fm_ctx*
     if ((rc = dalloc(&c->st[0], 9 * cap)) || (rc = dalloc(&c->st[1], 9 * cap)) || (rc = dalloc(&c->rdisp, 3 * cap)) ||
         (rc = dalloc(&c->ext[0], cap)) || (rc = dalloc(&c->ext[1], cap)) || (rc = dalloc(&c->slot_of, cap)) ||
         (rc = dalloc(&c->cell_of, cap)) || (rc = dalloc(&c->cell_count, (size_t)c->ncell + 1)) ||
-        (rc = dalloc(&c->cell_start, (size_t)c->ncell + 1)) || (rc = dalloc(&c->cell_cursor, (size_t)c->ncell)) ||
+        (rc = dalloc(&c->cell_start, (size_t)c->ncell + 1)) || (rc = dalloc(&c->bin_pos, cap)) ||
         (rc = dalloc(&c->scan_part, (size_t)c->ncell / 1024 + 2)) ||
         (rc = dalloc(&c->perm, cap)) || (rc = dalloc(&c->nbn, cap)) || (rc = dalloc(&c->pe_lj, cap)) ||
         (rc = dalloc(&c->pe_poly, cap)) || (rc = dalloc(&c->vir, cap)) || (rc = dalloc(&c->out_flag, cap)) ||
@@ -553,7 +553,7 @@ extern "C" int fm_memory_footprint(const fm_params* p, int32_t np, int64_t* byte
     maxnb = (maxnb + 3) & ~3;
     size_t b = 0;
     b += 2 * 9 * cap * 8 + 3 * cap * 8 + 9 * cap * 8;          // state copies, rdisp, staging
-    b += (2 + 1 + 1 + 1 + 1 + 3) * cap * 4;                     // ext, slot_of, cell_of, perm, nbn, sendmols scratch
+    b += (2 + 1 + 1 + 1 + 1 + 1 + 3) * cap * 4;                 // ext, slot_of, cell_of, bin_pos, perm, nbn, sendmols scratch
     b += 3 * cap * 8;                                           // pe_lj, pe_poly, vir
     b += (size_t)(maxnb / 4) * cap * 8;                         // neighbour words
     b += 3 * (ncell + 1) * 4;                                   // cell counts, starts, cursors

@@ -79,7 +79,7 @@ struct fm_ctx {
     int* cell_of = nullptr;               // per slot: interior cell index
     int* cell_count = nullptr;            // ncell
     int* cell_start = nullptr;            // ncell+1
-    int* cell_cursor = nullptr;           // ncell
+    int* bin_pos = nullptr;               // cap: each molecule's place among its cell's (binning)
     int* scan_part = nullptr;             // cell scan: count sums of the 1024-cell chunks
     int* perm = nullptr;                  // cap
 

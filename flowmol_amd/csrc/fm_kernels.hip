@@ -501,7 +501,7 @@ __device__ inline int wave_aggregated_add(int* counters, int key, bool active) {
 __global__ __launch_bounds__(FM_BLOCK) void k_bin_count(St s, const int* ext, int np, double hx, double hy,
                                                         double hz, double csx, double csy, double csz,
                                                         int nx, int ny, int nz, int* cell_of,
-                                                        int* count, int* flags) {
+                                                        int* count, int* pos_out, int* flags) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int cell = -1;
     if (i < np) {
@@ -537,7 +537,10 @@ __global__ __launch_bounds__(FM_BLOCK) void k_bin_count(St s, const int* ext, in
             cell_of[i] = 0;
         }
     }
-    wave_aggregated_add(count, cell, cell >= 0);
+    // the molecule's place among its cell's molecules (the order within a cell is arbitrary
+    // here: k_bin_sort orders each cell by x, ties by molecule number)
+    const int pos = wave_aggregated_add(count, cell, cell >= 0);
+    if (i < np) pos_out[i] = pos;
 }
 
 // start[0..n) = exclusive prefix of count[0..n), in chunks of SCAN_CH counts (FM_BLOCK threads,
@@ -601,13 +604,10 @@ __global__ __launch_bounds__(FM_BLOCK) void k_scan_fin(int* count, int* start, c
     for (int i = blockIdx.x * FM_BLOCK + tid; i < ncur; i += gridDim.x * FM_BLOCK) cursor[i] = 0;
 }
 
-__global__ __launch_bounds__(FM_BLOCK) void k_bin_scatter(const int* cell_of, const int* start, int* cursor,
+__global__ __launch_bounds__(FM_BLOCK) void k_bin_scatter(const int* cell_of, const int* start, const int* pos,
                                                           int* perm, int np) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = i < np;
-    const int c = active ? cell_of[i] : 0;
-    const int pos = wave_aggregated_add(cursor, c, active);
-    if (active) perm[start[c] + pos] = i;
+    if (i < np) perm[start[cell_of[i]] + pos[i]] = i;
 }
 
 // order each cell by increasing x (ties: molecule number), which lets the list build window
@@ -1200,15 +1200,15 @@ int launch_bin(fm_ctx* c, bool keep_acc) {
     k_bin_count<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(s, c->ext[c->cur], c->np, c->half[0], c->half[1],
                                                              c->half[2], c->cs[0], c->cs[1], c->cs[2], c->nc[0],
                                                              c->nc[1], c->nc[2], cell_tmp, c->cell_count,
-                                                             c->d_flags);
+                                                             c->bin_pos, c->d_flags);
     LAUNCH_CHECK();
     const int nscan = (c->ncell + 1 + SCAN_CH - 1) / SCAN_CH;
     k_scan_part<<<nscan, FM_BLOCK, 0, c->stream>>>(c->cell_count, c->scan_part, c->ncell + 1);
     LAUNCH_CHECK();
     k_scan_fin<<<nscan, FM_BLOCK, 0, c->stream>>>(c->cell_count, c->cell_start, c->scan_part, c->ncell + 1,
-                                                  c->cell_cursor, c->ncell);
+                                                  nullptr, 0);
     LAUNCH_CHECK();
-    k_bin_scatter<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(cell_tmp, c->cell_start, c->cell_cursor, c->perm,
+    k_bin_scatter<<<nblocks(c->np), FM_BLOCK, 0, c->stream>>>(cell_tmp, c->cell_start, c->bin_pos, c->perm,
                                                                c->np);
     LAUNCH_CHECK();
     int maxcell = c->p.max_cell_np > 0 ? c->p.max_cell_np : (1 << 26);
